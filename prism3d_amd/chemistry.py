@@ -1,0 +1,188 @@
+"""Per-cell chemistry on the GPU: the ChemistrySolver call surface.
+
+``ChemistrySolver`` mirrors prism3d/chemistry/solver.py:22-388: the same
+tracked species, ``solve_steady_state`` signature, return value and
+post-solve corrections.  Every solve runs the gfx950 BDF kernel
+(p3d_chem_solve); the batched entry points take (n_species, n_cells)
+structure-of-arrays device tensors and solve all cells in one launch.
+"""
+
+import numpy as np
+import torch
+
+from . import _lib
+from .device import device, stream_handle
+from .network import DeviceNetwork, Network, default_device_network
+
+X_FLOOR = 1e-30
+GAS_C = 2.69e-4 * 0.4  # gas-phase C/H (prism3d/utils/constants.py:90-99)
+GAS_O = 4.90e-4 * 0.6
+GAS_S = 1.32e-5 * 0.4
+GAS_SI = 3.24e-5 * 0.03
+GAS_FE = 3.16e-5 * 0.006
+GAS_HE = 8.51e-2
+
+
+def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=1e-4,
+               atol=1e-22, stop_rtol=1e-3, post_mode=0, max_steps=100000, x_c_total=GAS_C,
+               x_o_total=GAS_O):
+    """p3d_solve_opts; the defaults are the upstream refine-path settings
+    (solver.py:150-223)."""
+    o = _lib.SolveOpts()
+    o.t_max = -1.0 if t_max is None else float(t_max)
+    o.t_start = float(t_start)
+    o.n_checkpoints = int(n_checkpoints)
+    o.early_stop = int(bool(early_stop))
+    o.rtol, o.atol, o.stop_rtol = float(rtol), float(atol), float(stop_rtol)
+    o.post_mode = int(post_mode)
+    o.max_steps = int(max_steps)
+    o.x_c_total, o.x_o_total = float(x_c_total), float(x_o_total)
+    return o
+
+
+def _cell_arrays(n, *vals):
+    dev = device()
+    out = []
+    for v in vals:
+        t = torch.as_tensor(v, dtype=torch.float64, device=dev).reshape(-1)
+        if t.numel() == 1 and n > 1:
+            t = t.expand(n).contiguous()
+        if t.numel() != n:
+            raise ValueError(f"parameter array of length {t.numel()} for {n} cells")
+        out.append(t.contiguous())
+    return out
+
+
+class ChemistrySolver:
+    """GPU twin of prism3d.chemistry.solver.ChemistrySolver."""
+
+    def __init__(self, network=None):
+        if network is None:
+            self.dnet = default_device_network()
+        elif isinstance(network, DeviceNetwork):
+            self.dnet = network
+        elif isinstance(network, Network):
+            self.dnet = DeviceNetwork(network)
+        else:
+            raise TypeError("network must be a prism3d_amd Network or DeviceNetwork")
+        self.network = self.dnet.net
+        self.tracked_species = list(self.network.species)
+        self.n_tracked = len(self.tracked_species)
+        self.idx = {s: i for i, s in enumerate(self.tracked_species)}
+
+    # --- host conversions (solver.py:62-67) --------------------------------
+    def _abundances_to_dict(self, x_arr):
+        return {self.tracked_species[i]: max(float(x_arr[i]), X_FLOOR)
+                for i in range(self.n_tracked)}
+
+    def _dict_to_array(self, x_dict):
+        return np.array([x_dict.get(s, X_FLOOR) for s in self.tracked_species], dtype=np.float64)
+
+    def _default_initial(self, G0, A_V, gas=None):
+        """tanh initial abundances (solver.py:300-343)."""
+        g = gas or {'C': GAS_C, 'O': GAS_O, 'He': GAS_HE, 'S': GAS_S, 'Si': GAS_SI,
+                    'Fe': GAS_FE}
+        x = {}
+        f_H2 = 0.5 * (1.0 + np.tanh((A_V - 1.5) / 0.5))
+        f_CO = 0.5 * (1.0 + np.tanh((A_V - 3.0) / 0.8))
+        x['H'] = max(1.0 - f_H2, 0.003)
+        x['H2'] = f_H2 * 0.5
+        x['C+'] = g['C'] * (1.0 - f_CO)
+        x['CO'] = g['C'] * f_CO * 0.95
+        x['C'] = g['C'] * f_CO * 0.05
+        x['O'] = max(g['O'] - x['CO'], 1e-10)
+        for ion, neu, tot in (('S+', 'S', g.get('S', 0)), ('Si+', 'Si', g.get('Si', 0)),
+                              ('Fe+', 'Fe', g.get('Fe', 0))):
+            f = max(0.01, 1.0 - A_V / 3.0) if A_V < 3 else 0.01
+            x[ion] = tot * f
+            x[neu] = tot * (1 - f)
+        x['H+'] = 1e-6 if A_V < 2 else 1e-7
+        x['e-'] = max(x.get('C+', 0) + x.get('S+', 0) + x.get('Si+', 0) + x.get('Fe+', 0)
+                      + x.get('H+', 0), 1e-7)
+        x['H-'] = 1e-14
+        x['H2+'] = 1e-13
+        x['H3+'] = 5e-10 if A_V > 2 else 1e-12
+        x['OH'] = 1e-7 if A_V > 2 else 1e-9
+        x['OH+'] = 1e-13
+        x['H2O'] = 1e-7 if A_V > 3 else 1e-9
+        x['H2O+'] = 1e-14
+        x['H3O+'] = 1e-13
+        x['O2'] = 1e-8 if A_V > 3 else 1e-12
+        x['CH'] = 1e-10
+        x['CH+'] = 1e-13
+        x['CH2+'] = 1e-14
+        x['CO+'] = 1e-15
+        x['HCO+'] = 1e-9 if A_V > 2 else 1e-12
+        x['O+'] = 1e-13
+        x['He'] = g.get('He', GAS_HE)
+        x['He+'] = 1e-13
+        return x
+
+    # --- single-cell API (solver.py:145-225) --------------------------------
+    def solve_steady_state(self, n_H, T, G0, A_V, zeta_CR, x_init=None, f_shield_H2=1.0,
+                           f_shield_CO=1.0, t_max=None, rtol=1e-3, atol=1e-20, fast=False):
+        """Steady-state abundances of one cell -> (dict, converged).
+
+        ``rtol`` is the checkpoint stop criterion and ``atol`` is unused, as
+        upstream; the BDF itself runs at rtol 1e-4, atol 1e-22."""
+        if fast:
+            raise NotImplementedError("fast=True (the explicit 10-step mode) is not part of "
+                                      "the GPU solve; use fast=False")
+        if x_init is None:
+            x_init = self._default_initial(G0, A_V)
+        y = torch.tensor(self._dict_to_array(x_init), dtype=torch.float64,
+                         device=device()).reshape(-1, 1)
+        conv = self.solve_steady_batch(y, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO,
+                                       opts=solve_opts(t_max=t_max, stop_rtol=rtol))[0]
+        return self._abundances_to_dict(y[:, 0].cpu().numpy()), bool(conv[0].item())
+
+    # --- batched SoA API ----------------------------------------------------
+    def solve_steady_batch(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0, f_shield_CO=1.0,
+                           opts=None, sync=True):
+        """Solve every cell of y (n_species, n_cells), float64 device tensor,
+        in place.  Returns (converged int32[n], accepted steps int32[n])."""
+        self._check_state(y)
+        n = y.shape[1]
+        p = _cell_arrays(n, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO)
+        conv = torch.zeros(n, dtype=torch.int32, device=y.device)
+        steps = torch.zeros(n, dtype=torch.int32, device=y.device)
+        o = opts if opts is not None else solve_opts()
+        rc = _lib.lib().p3d_chem_solve(self.dnet.handle, n, y.data_ptr(),
+                                       *[t.data_ptr() for t in p], o, conv.data_ptr(),
+                                       steps.data_ptr(), stream_handle())
+        _lib.check(rc, "p3d_chem_solve")
+        if sync:
+            torch.cuda.synchronize(y.device)
+        return conv, steps
+
+    def rhs_batch(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0, f_shield_CO=1.0):
+        self._check_state(y)
+        n = y.shape[1]
+        p = _cell_arrays(n, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO)
+        out = torch.empty_like(y)
+        rc = _lib.lib().p3d_chem_rhs(self.dnet.handle, n, y.data_ptr(), *[t.data_ptr() for t in p],
+                                     out.data_ptr(), stream_handle())
+        _lib.check(rc, "p3d_chem_rhs")
+        torch.cuda.synchronize(y.device)
+        return out
+
+    def jacobian_batch(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0, f_shield_CO=1.0):
+        self._check_state(y)
+        n = y.shape[1]
+        ns = self.n_tracked
+        p = _cell_arrays(n, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO)
+        out = torch.empty((n, ns, ns), dtype=torch.float64, device=y.device)
+        rc = _lib.lib().p3d_chem_jacobian(self.dnet.handle, n, y.data_ptr(),
+                                          *[t.data_ptr() for t in p], out.data_ptr(),
+                                          stream_handle())
+        _lib.check(rc, "p3d_chem_jacobian")
+        torch.cuda.synchronize(y.device)
+        return out
+
+    def _check_state(self, y):
+        if not isinstance(y, torch.Tensor) or y.dtype != torch.float64 or not y.is_cuda:
+            raise TypeError("y must be a float64 GPU tensor of shape (n_species, n_cells)")
+        if y.ndim != 2 or y.shape[0] != self.n_tracked:
+            raise ValueError(f"y must have shape ({self.n_tracked}, n_cells), got {tuple(y.shape)}")
+        if not y.is_contiguous():
+            raise TypeError("y must be contiguous")

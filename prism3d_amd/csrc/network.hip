@@ -1,0 +1,889 @@
+// Species network on gfx950: rate coefficients, stiff RHS, analytic
+// Jacobian and the implicit variable-order BDF solve of every cell.
+//
+// Mapping: one cell per 64-lane workgroup (one wavefront).  Lane s owns
+// species s (its abundance and its backward differences live in registers);
+// lane r also evaluates reactions r, r+64, ...  The Jacobian and the
+// iteration matrix M = I - c J (row-major, padded stride) are staged in LDS;
+// the LU pivot search is a wavefront argmax, the triangular solves keep the
+// right-hand side in registers and broadcast with readlane, and every norm is
+// a butterfly reduction.  All per-wave control values (step size, order,
+// counters) are wave-uniform, so the whole solve runs with scalar branches.
+//
+// Upstream behaviour restated (paths relative to the upstream repository):
+//   rates + RHS            prism3d/chemistry/solver.py:69-143
+//   steady-state driver    prism3d/chemistry/solver.py:145-225
+//   H+/e- correction       prism3d/chemistry/solver.py:227-281
+//   conservation           prism3d/chemistry/solver.py:345-388
+//   integrator             scipy solve_ivp(method='BDF') as called at
+//                          solver.py:195-201: the quasi-constant step size NDF
+//                          of Shampine & Reichelt (1997), orders 1-5.
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "p3d_common.h"
+
+namespace p3d {
+
+constexpr int kMaxSpecies = 64;
+constexpr int kMaxOrder = 5;
+constexpr int kNumD = kMaxOrder + 3;
+
+struct NetDev {
+  int ns, nr, e_idx, nnz;
+  int roles[7];  // H, H2, H+, C+, C, CO, O
+  const int *rtype;
+  const double *alpha, *beta, *gamma;
+  const int *react;   // 2 per reaction
+  const int *dep;     // 2 per reaction: species the rate depends on (or -1)
+  const int *rhs_off; // ns+1
+  const int *rhs_rxn;
+  const double *rhs_sgn;
+  const int *jrow, *jcol, *joff;  // nnz, nnz, nnz+1
+  const int *jt_rxn, *jt_slot;
+  const double *jt_coef;
+  const double *post_w;  // 4 * ns
+};
+
+struct Network {
+  NetDev d;
+  void *blob = nullptr;  // single device allocation holding all tables
+  ~Network() {
+    if (blob) (void)hipFree(blob);
+  }
+};
+
+static std::mutex g_net_mu;
+static std::map<int64_t, std::unique_ptr<Network>> g_nets;
+static int64_t g_next_handle = 1;
+
+static Network *find_net(int64_t h) {
+  std::lock_guard<std::mutex> lk(g_net_mu);
+  auto it = g_nets.find(h);
+  return it == g_nets.end() ? nullptr : it->second.get();
+}
+
+struct CellParams {
+  double nH, T, G0, AV, zeta, fH2, fCO;
+};
+
+// LDS layout of one cell (one workgroup), all doubles unless noted.
+struct Lds {
+  double *kco;   // nr rate coefficients
+  double *part;  // 2*nr partial derivatives (also the rates)
+  double *ys;    // 64 floored abundances
+  double *J;     // ns x ld
+  double *M;     // ns x ld
+  int *piv;      // 64
+  int ld;
+};
+
+__device__ __forceinline__ Lds carve(double *base, const NetDev &N) {
+  Lds L;
+  L.ld = N.ns | 1;
+  L.kco = base;
+  L.part = L.kco + N.nr;
+  L.ys = L.part + 2 * N.nr;
+  L.J = L.ys + kMaxSpecies;
+  L.M = L.J + N.ns * L.ld;
+  L.piv = reinterpret_cast<int *>(L.M + N.ns * L.ld);
+  return L;
+}
+
+static size_t lds_bytes(int ns, int nr) {
+  int ld = ns | 1;
+  return sizeof(double) * (size_t)(3 * nr + kMaxSpecies + 2 * ns * ld) + sizeof(int) * kMaxSpecies;
+}
+
+// Rate coefficients k_r (solver.py:100-133).
+__device__ void rate_coeffs(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
+  const double Tc = fmin(fmax(p.T, 10.0), 41000.0);
+  const double lTc = log(Tc / 300.0), lT = log(p.T / 300.0);
+  for (int r = lane; r < N.nr; r += 64) {
+    const double a = N.alpha[r], b = N.beta[r], g = N.gamma[r];
+    double k;
+    switch (N.rtype[r]) {
+      case P3D_RX_TWO_BODY: k = a * exp(b * lTc) * exp(-g / Tc); break;
+      case P3D_RX_PHOTO: k = a * p.G0 * exp(-g * p.AV); break;
+      case P3D_RX_PHOTO_FH2: k = a * p.G0 * exp(-g * p.AV) * p.fH2; break;
+      case P3D_RX_PHOTO_FCO: k = a * p.G0 * exp(-g * p.AV) * p.fCO; break;
+      case P3D_RX_CR: k = a * p.zeta; break;
+      default: k = a * exp(b * lT); break;  // grain surface
+    }
+    L.kco[r] = k;
+  }
+}
+
+// PAH- abundance from the electron abundance (solver.py:74-87) and its
+// derivative with respect to x_e.
+__device__ __forceinline__ void pah_minus(double ye_floored, const CellParams &p,
+                                          double &pm, double &dpm) {
+  const double xpah = 1e-7;
+  double ye = fmax(ye_floored, 1e-10);
+  double ne = ye * p.nH;
+  if (p.G0 > 0.0 && ne > 0.0) {
+    double psi = p.G0 * sqrt(p.T) / ne;
+    double fm = fmax(500.0 / (psi + 500.0) * 0.5, 0.0);
+    pm = xpah * fm;
+    dpm = ye_floored > 1e-10 ? xpah * 250.0 * psi / ((psi + 500.0) * (psi + 500.0) * ye) : 0.0;
+  } else {
+    pm = xpah * 0.4;
+    dpm = 0.0;
+  }
+}
+
+// Publishes max(y, 1e-30) of lane s into ys[s].
+__device__ __forceinline__ void publish_y(const NetDev &N, const Lds &L, double y, int lane) {
+  if (lane < N.ns) L.ys[lane] = fmax(y, 1e-30);
+  __syncthreads();
+}
+
+// Reaction rates into part[2r] (want_jac: partials into part[2r], part[2r+1]
+// instead).  Requires ys published.
+__device__ void eval_reactions(const NetDev &N, const CellParams &p, const Lds &L, int lane,
+                               bool want_jac) {
+  double pm, dpm;
+  pah_minus(L.ys[N.e_idx], p, pm, dpm);
+  for (int r = lane; r < N.nr; r += 64) {
+    const double k = L.kco[r];
+    const int t = N.rtype[r];
+    const int a = N.react[2 * r], b = N.react[2 * r + 1];
+    const double ya = L.ys[a];
+    if (!want_jac) {
+      double rate;
+      if (t == P3D_RX_TWO_BODY) {
+        double yb = (b == P3D_SP_PAH_MINUS) ? pm : L.ys[b];
+        rate = k * ya * yb * p.nH;
+      } else if (t == P3D_RX_GRAIN) {
+        rate = k * ya * p.nH * 0.5;
+      } else {
+        rate = k * ya;
+      }
+      L.part[2 * r] = rate;
+    } else {
+      // partials with respect to dep[2r], dep[2r+1]; zero below the floor
+      const double act_a = ya > 1e-30 ? 1.0 : 0.0;
+      double d0 = 0.0, d1 = 0.0;
+      if (t == P3D_RX_TWO_BODY) {
+        if (b == P3D_SP_PAH_MINUS) {
+          d0 = k * pm * p.nH * act_a;
+          d1 = k * ya * p.nH * dpm * (L.ys[N.e_idx] > 1e-30 ? 1.0 : 0.0);
+        } else if (a == b) {
+          d0 = 2.0 * k * ya * p.nH * act_a;
+        } else {
+          const double yb = L.ys[b];
+          d0 = k * yb * p.nH * act_a;
+          d1 = k * ya * p.nH * (yb > 1e-30 ? 1.0 : 0.0);
+        }
+      } else if (t == P3D_RX_GRAIN) {
+        d0 = k * p.nH * 0.5 * act_a;
+      } else {
+        d0 = k * act_a;
+      }
+      L.part[2 * r] = d0;
+      L.part[2 * r + 1] = d1;
+    }
+  }
+  __syncthreads();
+}
+
+// dy/dt of species `lane` (upstream accumulation order).  Requires y published.
+__device__ double rhs_lane(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
+  eval_reactions(N, p, L, lane, false);
+  double acc = 0.0;
+  if (lane < N.ns) {
+    const int e0 = N.rhs_off[lane], e1 = N.rhs_off[lane + 1];
+    for (int e = e0; e < e1; ++e) acc += N.rhs_sgn[e] * L.part[2 * N.rhs_rxn[e]];
+  }
+  __syncthreads();  // part[] is reused by the next evaluation
+  return acc;
+}
+
+// Analytic Jacobian into L.J (row-major, stride ld).  Requires y published.
+__device__ void jacobian(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
+  eval_reactions(N, p, L, lane, true);
+  for (int idx = lane; idx < N.ns * L.ld; idx += 64) L.J[idx] = 0.0;
+  __syncthreads();
+  for (int z = lane; z < N.nnz; z += 64) {
+    double acc = 0.0;
+    for (int t = N.joff[z]; t < N.joff[z + 1]; ++t)
+      acc += N.jt_coef[t] * L.part[2 * N.jt_rxn[t] + N.jt_slot[t]];
+    L.J[N.jrow[z] * L.ld + N.jcol[z]] = acc;
+  }
+  __syncthreads();
+}
+
+// M = I - c J, factorised in place with partial pivoting.
+__device__ void lu_factor(const NetDev &N, const Lds &L, double c, int lane) {
+  const int n = N.ns, ld = L.ld;
+  for (int idx = lane; idx < n * ld; idx += 64) {
+    int i = idx / ld, j = idx - i * ld;
+    L.M[idx] = (i == j ? 1.0 : 0.0) - c * L.J[idx];
+  }
+  __syncthreads();
+  const bool half = n <= 32;
+  const int col = half ? (lane & 31) : lane;
+  const int r0 = half ? (lane >> 5) : 0, rs = half ? 2 : 1;
+  for (int k = 0; k < n; ++k) {
+    double v = (lane >= k && lane < n) ? fabs(L.M[lane * ld + k]) : -1.0;
+    const int pk = wave_argmax(v, lane);
+    if (pk != k && lane < n) {
+      double a = L.M[k * ld + lane];
+      L.M[k * ld + lane] = L.M[pk * ld + lane];
+      L.M[pk * ld + lane] = a;
+    }
+    if (lane == 0) L.piv[k] = pk;
+    __syncthreads();
+    const double inv = 1.0 / L.M[k * ld + k];
+    if (lane > k && lane < n) L.M[lane * ld + k] *= inv;
+    __syncthreads();
+    if (col > k && col < n) {
+      const double ukj = L.M[k * ld + col];
+      for (int i = k + 1 + r0; i < n; i += rs) L.M[i * ld + col] -= L.M[i * ld + k] * ukj;
+    }
+    __syncthreads();
+  }
+}
+
+// Solve M x = b with the factors (b and x live in registers, lane = row).
+__device__ double lu_solve(const NetDev &N, const Lds &L, double b, int lane) {
+  const int n = N.ns, ld = L.ld;
+  for (int k = 0; k < n; ++k) {
+    const int pk = uniform(L.piv[k]);
+    if (pk != k) {
+      double bk = bcast(b, k), bp = bcast(b, pk);
+      if (lane == k) b = bp;
+      if (lane == pk) b = bk;
+    }
+  }
+  for (int k = 0; k < n - 1; ++k) {
+    double bk = bcast(b, k);
+    if (lane > k && lane < n) b -= L.M[lane * ld + k] * bk;
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    if (lane == k) b /= L.M[k * ld + k];
+    double bk = bcast(b, k);
+    if (lane < k) b -= L.M[lane * ld + k] * bk;
+  }
+  return b;
+}
+
+__device__ __forceinline__ double rms(double v, int ns) { return sqrt(wave_sum(v * v) / ns); }
+
+// Step-size change of the backward differences D[0..order] by `factor`.
+__device__ void change_D(double (&D)[kNumD], int order, double factor) {
+  double R[kMaxOrder + 1][kMaxOrder + 1], U[kMaxOrder + 1][kMaxOrder + 1];
+#pragma unroll
+  for (int j = 0; j <= kMaxOrder; ++j) {
+    double cr = 1.0, cu = 1.0;
+#pragma unroll
+    for (int i = 0; i <= kMaxOrder; ++i) {
+      if (i > 0) {
+        cr *= ((double)(i - 1) - factor * (double)j) / (double)i;
+        cu *= ((double)(i - 1) - (double)j) / (double)i;
+      }
+      R[i][j] = cr;
+      U[i][j] = cu;
+    }
+  }
+  double Dn[kMaxOrder + 1];
+#pragma unroll
+  for (int j = 0; j <= kMaxOrder; ++j) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i <= kMaxOrder; ++i) {
+      if (i > order) continue;
+      double ru = 0.0;
+#pragma unroll
+      for (int m = 0; m <= kMaxOrder; ++m)
+        if (m <= order) ru += R[i][m] * U[m][j];
+      acc += ru * D[i];
+    }
+    Dn[j] = acc;
+  }
+#pragma unroll
+  for (int j = 0; j <= kMaxOrder; ++j)
+    if (j <= order) D[j] = Dn[j];
+}
+
+struct BdfStats {
+  int steps, rhs, jac, lu, rejects;
+};
+
+// Variable-order BDF from t0 to t1 for the wave's cell.  y: lane's species.
+// Returns 0 on success, -1 step underflow, -2 step limit.
+__device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L, int lane,
+                             double &y, double t0, double t1, double rtol, double atol,
+                             double max_step, int max_steps, BdfStats &st) {
+  const int ns = N.ns;
+  const bool act = lane < ns;
+  const double kappa[kMaxOrder + 1] = {0.0, -0.1850, -1.0 / 9.0, -0.0823, -0.0415, 0.0};
+  double gam[kMaxOrder + 2], alp[kMaxOrder + 1], ec[kMaxOrder + 2];
+  gam[0] = 0.0;
+#pragma unroll
+  for (int i = 1; i <= kMaxOrder + 1; ++i) gam[i] = gam[i - 1] + 1.0 / i;
+#pragma unroll
+  for (int i = 0; i <= kMaxOrder; ++i) {
+    alp[i] = (1.0 - kappa[i]) * gam[i];
+    ec[i] = kappa[i] * gam[i] + 1.0 / (i + 1);
+  }
+  ec[kMaxOrder + 1] = 0.0;
+
+  const double span = t1 - t0;
+  if (!(span > 0.0)) return 0;
+  if (max_step <= 0.0 || max_step > span) max_step = span;
+  if (!act) y = 0.0;
+
+  publish_y(N, L, y, lane);
+  double f0 = rhs_lane(N, p, L, lane);
+  st.rhs++;
+  // initial step (Hairer, Norsett & Wanner 1993, II.4), order 1
+  double scale = atol + rtol * fabs(y);
+  double d0 = rms(act ? y / scale : 0.0, ns), d1 = rms(act ? f0 / scale : 0.0, ns);
+  double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+  h0 = fmin(h0, span);
+  publish_y(N, L, y + h0 * f0, lane);
+  double f1 = rhs_lane(N, p, L, lane);
+  st.rhs++;
+  double d2 = rms(act ? (f1 - f0) / scale : 0.0, ns) / h0;
+  double dm = fmax(d1, d2);
+  double h1 = (dm <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : sqrt(0.01 / dm);
+  double h = fmin(fmin(100.0 * h0, h1), fmin(max_step, span));
+
+  double D[kNumD];
+#pragma unroll
+  for (int j = 0; j < kNumD; ++j) D[j] = 0.0;
+  D[0] = y;
+  D[1] = act ? f0 * h : 0.0;
+  int order = 1, n_equal = 0;
+  double t = t0;
+  publish_y(N, L, y, lane);
+  jacobian(N, p, L, lane);
+  st.jac++;
+  bool jac_current = true, lu_valid = false;
+  double c_lu = 0.0;
+  const double newton_tol = fmax(10.0 * 2.220446049250313e-16 / rtol, fmin(0.03, sqrt(rtol)));
+  int steps = 0;
+  double err = 0.0, d = 0.0;
+
+  while (t < t1) {
+    if (steps++ > max_steps) return -2;
+    const double hmin = 10.0 * fabs(nextafter(t, INFINITY) - t);
+    if (h > max_step) { change_D(D, order, max_step / h); h = max_step; n_equal = 0; lu_valid = false; }
+    if (h < hmin) { change_D(D, order, hmin / h); h = hmin; n_equal = 0; lu_valid = false; }
+    if (t + h > t1) { change_D(D, order, (t1 - t) / h); h = t1 - t; n_equal = 0; lu_valid = false; }
+
+    double ynew = 0.0;
+    for (;;) {  // attempt the step until accepted
+      const double tn = fmin(t + h, t1);
+      double ypred = 0.0, psi = 0.0;
+#pragma unroll
+      for (int j = 0; j < kNumD; ++j)
+        if (j <= order) ypred += D[j];
+#pragma unroll
+      for (int j = 1; j < kNumD; ++j)
+        if (j <= order) psi += gam[j] * D[j];
+      psi /= alp[order];
+      scale = atol + rtol * fabs(ypred);
+      const double c = h / alp[order];
+      bool converged = false;
+      for (;;) {
+        if (!lu_valid || c != c_lu) {
+          lu_factor(N, L, c, lane);
+          st.lu++;
+          lu_valid = true;
+          c_lu = c;
+        }
+        d = 0.0;
+        ynew = ypred;
+        double dy_old = -1.0, rate = -1.0;
+        converged = false;
+        for (int it = 0; it < 4; ++it) {
+          publish_y(N, L, ynew, lane);
+          double f = rhs_lane(N, p, L, lane);
+          st.rhs++;
+          if (!isfinite(wave_sum(act ? f : 0.0))) break;
+          double dy = act ? c * f - psi - d : 0.0;
+          dy = lu_solve(N, L, dy, lane);
+          if (!act) dy = 0.0;
+          const double dyn = rms(act ? dy / scale : 0.0, ns);
+          if (dy_old >= 0.0) rate = dyn / dy_old;
+          if (rate >= 0.0 && (rate >= 1.0 || pow(rate, (double)(4 - it)) / (1.0 - rate) * dyn > newton_tol))
+            break;
+          ynew += dy;
+          d += dy;
+          if (dyn == 0.0 || (rate >= 0.0 && rate / (1.0 - rate) * dyn < newton_tol)) {
+            converged = true;
+            break;
+          }
+          dy_old = dyn;
+        }
+        if (converged) break;
+        if (!jac_current) {
+          publish_y(N, L, ypred, lane);
+          jacobian(N, p, L, lane);
+          st.jac++;
+          jac_current = true;
+          lu_valid = false;
+          continue;
+        }
+        break;
+      }
+      if (!converged) {
+        h *= 0.5;
+        if (h < hmin) return -1;
+        change_D(D, order, 0.5);
+        n_equal = 0;
+        lu_valid = false;
+        st.rejects++;
+        continue;
+      }
+      scale = atol + rtol * fabs(ynew);
+      err = act ? ec[order] * d : 0.0;
+      const double en = rms(act ? err / scale : 0.0, ns);
+      if (en > 1.0) {
+        double factor = fmax(0.2, 0.9 * pow(en, -1.0 / (order + 1)));
+        h *= factor;
+        change_D(D, order, factor);
+        n_equal = 0;
+        lu_valid = false;
+        st.rejects++;
+        continue;
+      }
+      t = tn;
+      break;
+    }
+    st.steps++;
+    jac_current = false;
+    // update the differences (D[order+2] = d - D[order+1]; D[order+1] = d)
+#pragma unroll
+    for (int j = kNumD - 1; j >= 2; --j)
+      if (j == order + 2) D[j] = d - D[j - 1];
+#pragma unroll
+    for (int j = kNumD - 1; j >= 1; --j)
+      if (j == order + 1) D[j] = d;
+#pragma unroll
+    for (int j = kNumD - 2; j >= 0; --j)
+      if (j <= order) D[j] += D[j + 1];
+    n_equal++;
+    if (t >= t1) break;
+    if (n_equal < order + 1) continue;
+    // order selection
+    scale = atol + rtol * fabs(D[0]);
+    const double e0 = rms(act ? err / scale : 0.0, ns);
+    double em = INFINITY, ep = INFINITY;
+    if (order > 1) {
+      double dm1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < kNumD; ++j)
+        if (j == order) dm1 = D[j];
+      em = rms(act ? ec[order - 1] * dm1 / scale : 0.0, ns);
+    }
+    if (order < kMaxOrder) {
+      double dp2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < kNumD; ++j)
+        if (j == order + 2) dp2 = D[j];
+      ep = rms(act ? ec[order + 1] * dp2 / scale : 0.0, ns);
+    }
+    double fm = order > 1 ? (em > 0.0 ? pow(em, -1.0 / order) : INFINITY) : 0.0;
+    double fz = e0 > 0.0 ? pow(e0, -1.0 / (order + 1)) : INFINITY;
+    double fp = order < kMaxOrder ? (ep > 0.0 ? pow(ep, -1.0 / (order + 2)) : INFINITY) : 0.0;
+    int dord = 0;
+    double best = fz;
+    if (fm > best) { best = fm; dord = -1; }
+    if (fp > best) { best = fp; dord = 1; }
+    order += dord;
+    const double factor = fmin(10.0, 0.9 * best);
+    h *= factor;
+    change_D(D, order, factor);
+    n_equal = 0;
+    lu_valid = false;
+  }
+  y = D[0];
+  return 0;
+}
+
+// H+/e- steady-state correction (solver.py:227-281) and conservation
+// (solver.py:345-388); mode 0 = upstream clipped scales, 1 = exact projection.
+__device__ void post_solve(const NetDev &N, const CellParams &p, int lane, double &y, int mode,
+                           double x_c_total, double x_o_total) {
+  const int ns = N.ns;
+  const bool act = lane < ns;
+  const int iH = N.roles[0], iH2 = N.roles[1], iHp = N.roles[2], iCp = N.roles[3],
+            iC = N.roles[4], iCO = N.roles[5], iO = N.roles[6], ie = N.e_idx;
+  const double *wH = N.post_w, *wC = N.post_w + ns, *wO = N.post_w + 2 * ns,
+               *wI = N.post_w + 3 * ns;
+  {
+    const double xH = bcast(y, iH), xCp = bcast(y, iCp);
+    const double R = p.zeta * 0.46 * xH;
+    const double Tc = fmax(p.T, 10.0);
+    const double ae = 2.753e-14 * pow(Tc / 300.0, -0.745) + 1.0e-14 * sqrt(Tc / 100.0);
+    const double a = ae * p.nH, b = ae * p.nH * xCp, c = -R;
+    if (a > 0.0) {
+      const double disc = b * b - 4.0 * a * c;
+      if (disc >= 0.0 && xCp < 1e-6) {
+        const double xhp = fmax((-b + sqrt(disc)) / (2.0 * a), 1e-30);
+        if (lane == iHp) y = xhp;
+        const double npos = wave_sum(act ? wI[lane] * y : 0.0);
+        if (lane == ie) y = fmax(npos, 1e-30);
+      }
+    }
+  }
+  {
+    const double Hmaj = bcast(y, iH) + 2.0 * bcast(y, iH2);
+    const double Hmin = wave_sum(act ? wH[lane] * y : 0.0);
+    if (Hmaj > 0.0) {
+      double s = mode == 0 ? fmin(fmax(fmax(1.0 - Hmin, 0.9) / Hmaj, 0.8), 1.2) : (1.0 - Hmin) / Hmaj;
+      if (lane == iH || lane == iH2) y *= s;
+    }
+    const double Cmaj = bcast(y, iCp) + bcast(y, iC) + bcast(y, iCO);
+    const double Cmin = wave_sum(act ? wC[lane] * y : 0.0);
+    if (Cmaj > 0.0) {
+      double s = mode == 0
+                     ? fmin(fmax(fmax(x_c_total - Cmin, x_c_total * 0.9) / Cmaj, 0.8), 1.2)
+                     : (x_c_total - Cmin) / Cmaj;
+      if (lane == iCp || lane == iC || lane == iCO) y *= s;
+    }
+    const double Omol = wave_sum(act ? wO[lane] * y : 0.0);
+    if (lane == iO) y = fmax(x_o_total - Omol, 1e-30);
+    const double npos = wave_sum(act ? wI[lane] * y : 0.0);
+    if (lane == ie) y = fmax(npos, 1e-30);
+    y = fmax(y, 1e-30);
+  }
+}
+
+__device__ __forceinline__ CellParams load_params(int64_t c, const double *nH, const double *T,
+                                                  const double *G0, const double *AV,
+                                                  const double *zeta, const double *fH2,
+                                                  const double *fCO) {
+  CellParams p;
+  p.nH = nH[c]; p.T = T[c]; p.G0 = G0[c]; p.AV = AV[c];
+  p.zeta = zeta[c]; p.fH2 = fH2[c]; p.fCO = fCO[c];
+  return p;
+}
+
+__global__ void __launch_bounds__(64)
+chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
+                  const double *__restrict__ T, const double *__restrict__ G0,
+                  const double *__restrict__ AV, const double *__restrict__ zeta,
+                  const double *__restrict__ fH2, const double *__restrict__ fCO,
+                  p3d_solve_opts o, int32_t *__restrict__ conv_out,
+                  int32_t *__restrict__ steps_out) {
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  const Lds L = carve(smem, N);
+  const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
+  const bool act = lane < N.ns;
+  double yc = act ? y[(int64_t)lane * n + cell] : 0.0;
+  rate_coeffs(N, p, L, lane);
+  __syncthreads();
+
+  double tmax = o.t_max;
+  if (tmax <= 0.0) tmax = fmin(3e17, fmax(1e16, 3e17 / fmax(p.nH, 1.0)));
+  const double l0 = log10(o.t_start), l1 = log10(tmax);
+  BdfStats st = {0, 0, 0, 0, 0};
+  int conv = 0;
+  const int nk = o.n_checkpoints;
+  for (int i = 0; i < nk - 1; ++i) {
+    double ta = (i == 0) ? o.t_start : pow(10.0, l0 + (l1 - l0) * i / (nk - 1));
+    double tb = (i + 1 == nk - 1) ? tmax : pow(10.0, l0 + (l1 - l0) * (i + 1) / (nk - 1));
+    double yn = yc;
+    int rc = bdf_integrate(N, p, L, lane, yn, ta, tb, o.rtol, o.atol, tb - ta, o.max_steps, st);
+    if (rc != 0) continue;  // a failed span leaves the state unchanged (solver.py:216-217)
+    yn = fmax(yn, 1e-30);
+    if (o.early_stop) {
+      double rel = -1.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int k = N.roles[q == 0 ? 0 : q == 1 ? 1 : q == 2 ? 3 : q == 3 ? 4 : q == 4 ? 5 : 6];
+        const double old = bcast(yc, k), nw = bcast(yn, k);
+        if (old > 1e-12) rel = fmax(rel, fabs(nw - old) / fmax(old, 1e-20));
+      }
+      if (rel >= 0.0 && rel < o.stop_rtol && i > 3) {
+        conv = 1;
+        yc = yn;
+        break;
+      }
+    }
+    yc = yn;
+  }
+  if (o.post_mode >= 0) post_solve(N, p, lane, yc, o.post_mode, o.x_c_total, o.x_o_total);
+  if (act) y[(int64_t)lane * n + cell] = yc;
+  if (lane == 0) {
+    if (conv_out) conv_out[cell] = conv;
+    if (steps_out) steps_out[cell] = st.steps;
+  }
+}
+
+__global__ void __launch_bounds__(64)
+chem_rhs_kernel(NetDev N, int64_t n, const double *__restrict__ y, const double *__restrict__ nH,
+                const double *__restrict__ T, const double *__restrict__ G0,
+                const double *__restrict__ AV, const double *__restrict__ zeta,
+                const double *__restrict__ fH2, const double *__restrict__ fCO,
+                double *__restrict__ dydt) {
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  const Lds L = carve(smem, N);
+  const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
+  rate_coeffs(N, p, L, lane);
+  __syncthreads();
+  double yv = lane < N.ns ? y[(int64_t)lane * n + cell] : 0.0;
+  publish_y(N, L, yv, lane);
+  double f = rhs_lane(N, p, L, lane);
+  if (lane < N.ns) dydt[(int64_t)lane * n + cell] = f;
+}
+
+__global__ void __launch_bounds__(64)
+chem_jac_kernel(NetDev N, int64_t n, const double *__restrict__ y, const double *__restrict__ nH,
+                const double *__restrict__ T, const double *__restrict__ G0,
+                const double *__restrict__ AV, const double *__restrict__ zeta,
+                const double *__restrict__ fH2, const double *__restrict__ fCO,
+                double *__restrict__ jac) {
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  const Lds L = carve(smem, N);
+  const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
+  rate_coeffs(N, p, L, lane);
+  __syncthreads();
+  double yv = lane < N.ns ? y[(int64_t)lane * n + cell] : 0.0;
+  publish_y(N, L, yv, lane);
+  jacobian(N, p, L, lane);
+  const int ns = N.ns;
+  double *out = jac + cell * (int64_t)ns * ns;
+  for (int idx = lane; idx < ns * ns; idx += 64) {
+    int i = idx / ns, j = idx - i * ns;
+    out[idx] = L.J[i * L.ld + j];
+  }
+}
+
+}  // namespace p3d
+
+using namespace p3d;
+
+extern "C" {
+
+int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const double *alpha,
+                       const double *beta, const double *gamma, const int32_t *reactants,
+                       const int32_t *products, int32_t e_index, const int32_t *roles7,
+                       const double *post_w4, int64_t *handle) {
+  P3D_REQUIRE(handle, "handle pointer is required");
+  P3D_REQUIRE(ns >= 1 && ns <= kMaxSpecies, "n_species must be in [1, %d] (got %d)",
+              kMaxSpecies, ns);
+  P3D_REQUIRE(nr >= 1 && nr <= 16384, "n_reactions must be in [1, 16384] (got %d)", nr);
+  P3D_REQUIRE(rtype && alpha && beta && gamma && reactants && products, "null table");
+  P3D_REQUIRE(e_index >= 0 && e_index < ns, "e_index out of range");
+  for (int r = 0; r < nr; ++r) {
+    P3D_REQUIRE(rtype[r] >= P3D_RX_TWO_BODY && rtype[r] <= P3D_RX_GRAIN,
+                "reaction %d: unknown type %d", r, rtype[r]);
+    const int a = reactants[2 * r], b = reactants[2 * r + 1];
+    P3D_REQUIRE(a >= 0 && a < ns, "reaction %d: first reactant must be a species", r);
+    if (rtype[r] == P3D_RX_TWO_BODY)
+      P3D_REQUIRE((b >= 0 && b < ns) || b == P3D_SP_PAH_MINUS,
+                  "reaction %d: two-body reaction needs a second reactant", r);
+    else
+      P3D_REQUIRE(b == P3D_SP_NONE || (b >= 0 && b < ns) || b == P3D_SP_PAH_MINUS,
+                  "reaction %d: bad second reactant", r);
+    for (int m = 0; m < 3; ++m)
+      P3D_REQUIRE(products[3 * r + m] >= -2 && products[3 * r + m] < ns,
+                  "reaction %d: product index out of range", r);
+  }
+  // dependency species of each rate
+  std::vector<int> dep(2 * nr, -1);
+  for (int r = 0; r < nr; ++r) {
+    const int a = reactants[2 * r], b = reactants[2 * r + 1];
+    dep[2 * r] = a;
+    if (rtype[r] == P3D_RX_TWO_BODY) {
+      if (b == P3D_SP_PAH_MINUS) dep[2 * r + 1] = e_index;
+      else if (b != a) dep[2 * r + 1] = b;
+    }
+  }
+  // RHS CSR in the upstream accumulation order: reaction by reaction,
+  // reactant occurrences (-1) then product occurrences (+1).
+  std::vector<std::vector<std::pair<int, double>>> rows(ns);
+  for (int r = 0; r < nr; ++r) {
+    const int slots = rtype[r] == P3D_RX_TWO_BODY || rtype[r] == P3D_RX_GRAIN ? 2 : 1;
+    for (int m = 0; m < slots; ++m) {
+      int s = reactants[2 * r + m];
+      if (s >= 0) rows[s].push_back({r, -1.0});
+    }
+    for (int m = 0; m < 3; ++m) {
+      int s = products[3 * r + m];
+      if (s >= 0) rows[s].push_back({r, +1.0});
+    }
+  }
+  std::vector<int> rhs_off(ns + 1, 0), rhs_rxn;
+  std::vector<double> rhs_sgn;
+  for (int s = 0; s < ns; ++s) {
+    for (auto &e : rows[s]) { rhs_rxn.push_back(e.first); rhs_sgn.push_back(e.second); }
+    rhs_off[s + 1] = (int)rhs_rxn.size();
+  }
+  // Jacobian structure: entry (i, j) collects stoich(i, r) * d rate_r / d y_j
+  std::map<std::pair<int, int>, std::vector<std::pair<int, double>>> jm;  // -> (2r+slot, coef)
+  for (int r = 0; r < nr; ++r) {
+    std::map<int, double> net;
+    const int slots = rtype[r] == P3D_RX_TWO_BODY || rtype[r] == P3D_RX_GRAIN ? 2 : 1;
+    for (int m = 0; m < slots; ++m)
+      if (reactants[2 * r + m] >= 0) net[reactants[2 * r + m]] -= 1.0;
+    for (int m = 0; m < 3; ++m)
+      if (products[3 * r + m] >= 0) net[products[3 * r + m]] += 1.0;
+    for (int sl = 0; sl < 2; ++sl) {
+      const int j = dep[2 * r + sl];
+      if (j < 0) continue;
+      for (auto &kv : net)
+        if (kv.second != 0.0) jm[{kv.first, j}].push_back({2 * r + sl, kv.second});
+    }
+  }
+  std::vector<int> jrow, jcol, joff(1, 0), jt_rxn, jt_slot;
+  std::vector<double> jt_coef;
+  for (auto &kv : jm) {
+    jrow.push_back(kv.first.first);
+    jcol.push_back(kv.first.second);
+    for (auto &t : kv.second) {
+      jt_rxn.push_back(t.first / 2);
+      jt_slot.push_back(t.first % 2);
+      jt_coef.push_back(t.second);
+    }
+    joff.push_back((int)jt_rxn.size());
+  }
+  const int nnz = (int)jrow.size();
+  std::vector<double> post(4 * (size_t)ns, 0.0);
+  if (post_w4) post.assign(post_w4, post_w4 + 4 * (size_t)ns);
+  int roles[7] = {-1, -1, -1, -1, -1, -1, -1};
+  if (roles7)
+    for (int k = 0; k < 7; ++k) {
+      P3D_REQUIRE(roles7[k] >= -1 && roles7[k] < ns, "roles7[%d] out of range", k);
+      roles[k] = roles7[k];
+    }
+
+  // pack all tables into one device allocation (8-byte aligned sections)
+  std::vector<char> host;
+  auto put = [&](const void *src, size_t bytes) {
+    size_t off = host.size();
+    size_t pad = (8 - off % 8) % 8;
+    host.resize(off + pad + bytes);
+    if (bytes) memcpy(host.data() + off + pad, src, bytes);
+    return off + pad;
+  };
+  std::vector<int> rt(rtype, rtype + nr), re(reactants, reactants + 2 * nr);
+  size_t o_rt = put(rt.data(), 4 * nr), o_a = put(alpha, 8 * nr), o_b = put(beta, 8 * nr),
+         o_g = put(gamma, 8 * nr), o_re = put(re.data(), 8 * nr), o_dep = put(dep.data(), 8 * nr),
+         o_ro = put(rhs_off.data(), 4 * (ns + 1)), o_rr = put(rhs_rxn.data(), 4 * rhs_rxn.size()),
+         o_rs = put(rhs_sgn.data(), 8 * rhs_sgn.size()), o_jr = put(jrow.data(), 4 * nnz),
+         o_jc = put(jcol.data(), 4 * nnz), o_jo = put(joff.data(), 4 * (nnz + 1)),
+         o_tr = put(jt_rxn.data(), 4 * jt_rxn.size()), o_ts = put(jt_slot.data(), 4 * jt_slot.size()),
+         o_tc = put(jt_coef.data(), 8 * jt_coef.size()), o_pw = put(post.data(), 8 * post.size());
+  auto net = std::make_unique<Network>();
+  if (hipMalloc(&net->blob, host.size()) != hipSuccess) {
+    set_error("hipMalloc of %zu bytes for network tables failed", host.size());
+    return P3D_ENOMEM;
+  }
+  P3D_CHECK_HIP(hipMemcpy(net->blob, host.data(), host.size(), hipMemcpyHostToDevice));
+  char *b = static_cast<char *>(net->blob);
+  NetDev &d = net->d;
+  d.ns = ns; d.nr = nr; d.e_idx = e_index; d.nnz = nnz;
+  for (int k = 0; k < 7; ++k) d.roles[k] = roles[k];
+  d.rtype = (const int *)(b + o_rt);
+  d.alpha = (const double *)(b + o_a);
+  d.beta = (const double *)(b + o_b);
+  d.gamma = (const double *)(b + o_g);
+  d.react = (const int *)(b + o_re);
+  d.dep = (const int *)(b + o_dep);
+  d.rhs_off = (const int *)(b + o_ro);
+  d.rhs_rxn = (const int *)(b + o_rr);
+  d.rhs_sgn = (const double *)(b + o_rs);
+  d.jrow = (const int *)(b + o_jr);
+  d.jcol = (const int *)(b + o_jc);
+  d.joff = (const int *)(b + o_jo);
+  d.jt_rxn = (const int *)(b + o_tr);
+  d.jt_slot = (const int *)(b + o_ts);
+  d.jt_coef = (const double *)(b + o_tc);
+  d.post_w = (const double *)(b + o_pw);
+  std::lock_guard<std::mutex> lk(g_net_mu);
+  int64_t h = g_next_handle++;
+  g_nets[h] = std::move(net);
+  *handle = h;
+  return P3D_OK;
+}
+
+int p3d_network_destroy(int64_t handle) {
+  std::lock_guard<std::mutex> lk(g_net_mu);
+  auto it = g_nets.find(handle);
+  P3D_REQUIRE(it != g_nets.end(), "unknown network handle %lld", (long long)handle);
+  g_nets.erase(it);
+  return P3D_OK;
+}
+
+static int check_cells(const Network *net, int64_t n, const void *y, const double *nH,
+                       const double *T, const double *G0, const double *AV, const double *zeta,
+                       const double *fH2, const double *fCO) {
+  P3D_REQUIRE(net, "unknown network handle");
+  P3D_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX * 64, "n out of range");
+  if (n == 0) return P3D_OK;
+  P3D_REQUIRE(y && nH && T && G0 && AV && zeta && fH2 && fCO, "null device array");
+  return P3D_OK;
+}
+
+int p3d_chem_rhs(int64_t handle, int64_t n, const double *y, const double *n_h,
+                 const double *t_gas, const double *g0, const double *a_v, const double *zeta,
+                 const double *fsh_h2, const double *fsh_co, double *dydt, void *stream) {
+  Network *net = find_net(handle);
+  int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
+  if (rc != P3D_OK || n == 0) return rc;
+  P3D_REQUIRE(dydt, "null output");
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr);
+  P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
+  hipLaunchKernelGGL(chem_rhs_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream), net->d,
+                     n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, dydt);
+  return launch_status("chem_rhs_kernel");
+}
+
+int p3d_chem_jacobian(int64_t handle, int64_t n, const double *y, const double *n_h,
+                      const double *t_gas, const double *g0, const double *a_v,
+                      const double *zeta, const double *fsh_h2, const double *fsh_co,
+                      double *jac, void *stream) {
+  Network *net = find_net(handle);
+  int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
+  if (rc != P3D_OK || n == 0) return rc;
+  P3D_REQUIRE(jac, "null output");
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr);
+  P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
+  hipLaunchKernelGGL(chem_jac_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream), net->d,
+                     n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, jac);
+  return launch_status("chem_jac_kernel");
+}
+
+int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, const double *t_gas,
+                   const double *g0, const double *a_v, const double *zeta,
+                   const double *fsh_h2, const double *fsh_co, const p3d_solve_opts *opts,
+                   int32_t *converged, int32_t *steps, void *stream) {
+  Network *net = find_net(handle);
+  int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
+  if (rc != P3D_OK || n == 0) return rc;
+  P3D_REQUIRE(opts, "opts is required");
+  const p3d_solve_opts o = *opts;
+  P3D_REQUIRE(o.n_checkpoints >= 2 && o.n_checkpoints <= 10000, "n_checkpoints must be >= 2");
+  P3D_REQUIRE(o.t_start > 0.0, "t_start must be > 0");
+  P3D_REQUIRE(o.t_max <= 0.0 || o.t_max > o.t_start, "t_max must exceed t_start");
+  P3D_REQUIRE(o.rtol > 0.0 && o.atol > 0.0, "tolerances must be > 0");
+  P3D_REQUIRE(o.max_steps > 0, "max_steps must be > 0");
+  P3D_REQUIRE(o.post_mode >= -1 && o.post_mode <= 1, "post_mode must be -1, 0 or 1");
+  if (o.post_mode >= 0 || o.early_stop)
+    for (int k = 0; k < 7; ++k)
+      P3D_REQUIRE(net->d.roles[k] >= 0,
+                  "post-solve corrections / early stop need the species roles of the network");
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr);
+  P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
+  hipLaunchKernelGGL(chem_solve_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream),
+                     net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o, converged,
+                     steps);
+  return launch_status("chem_solve_kernel");
+}
+
+}  // extern "C"

@@ -1,0 +1,168 @@
+"""Device-resident whole-grid operations around the per-cell step.
+
+Thin wrappers over the grid kernels of libp3d_hip.so; every argument is a
+float64 CUDA tensor (the grid in C order) and every result is allocated on
+the device.  The ray set is host data (a few hundred doubles).
+"""
+
+import numpy as np
+import torch
+
+from . import _lib
+from .device import device, stream_handle
+
+AV_PER_NH = 5.34e-22  # prism3d/utils/constants.py:52
+LINES = {  # (freq, A, T_ul, g_u, g_l) of prism3d/utils/constants.py:103-109
+    "CII_158": (1.9005e12, 2.321e-6, 91.21, 4.0, 2.0),
+    "OI_63": (4.7448e12, 8.910e-5, 227.71, 3.0, 5.0),
+    "OI_145": (2.0604e12, 1.750e-5, 326.58, 1.0, 3.0),
+    "CI_609": (4.9231e11, 7.880e-8, 23.62, 3.0, 1.0),
+    "CI_370": (8.0929e11, 2.650e-7, 62.46, 5.0, 3.0),
+}
+
+
+def line_params():
+    """The four parameter vectors of the thermal call (solver_3d.py:787-790)."""
+    return (np.array(LINES["CII_158"]), np.array(LINES["OI_63"]), np.array(LINES["OI_145"]),
+            np.array(LINES["CI_609"] + LINES["CI_370"]))
+
+
+def healpix_directions(nside=1):
+    """Fibonacci-spiral ray directions and solid-angle weights
+    (radiative_transfer/fuv_rt.py:19-57)."""
+    n = 12 * nside ** 2
+    i = np.arange(n, dtype=np.float64)
+    ct = 1.0 - 2.0 * (i + 0.5) / n
+    st = np.sqrt(1.0 - ct ** 2)
+    phi = np.pi * (3.0 - np.sqrt(5.0)) * i
+    return np.stack([st * np.cos(phi), st * np.sin(phi), ct], axis=1), np.full(n, 4.0 * np.pi / n)
+
+
+def directional_G0(G0, source_direction, nside=1, background=0.0):
+    """Per-ray external field for a distant source (fuv_rt_3d.py:21-71)."""
+    d, w = healpix_directions(nside)
+    s = np.asarray(source_direction, dtype=np.float64)
+    s = s / np.linalg.norm(s)
+    cth = d @ s
+    lit = cth > 0
+    ws = np.sum(np.where(lit, cth * w, 0.0))
+    g = np.where(lit, G0 * cth * (4.0 * np.pi) / ws, 0.0) if ws > 0 else np.full(len(d), G0)
+    return g + background
+
+
+def _t(x):
+    if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64):
+        raise TypeError("expected a float64 CUDA tensor")
+    return x.contiguous()
+
+
+def _empty(like, k=1):
+    return [torch.empty_like(like) for _ in range(k)]
+
+
+def fuv_columns(density, x_h2, x_co, cell_size, nside=1, G0_external=1.0,
+                av_per_nh=AV_PER_NH, workspace=None):
+    """G0, A_V, N_H, N_H2, N_CO on the grid (fuv_rt_3d.py:215-294)."""
+    density, x_h2, x_co = _t(density), _t(x_h2), _t(x_co)
+    if density.ndim != 3 or x_h2.shape != density.shape or x_co.shape != density.shape:
+        raise ValueError("density, x_h2 and x_co must share one 3-D shape")
+    nx, ny, nz = density.shape
+    dirs, _ = healpix_directions(nside)
+    nr = len(dirs)
+    g0r = np.asarray(G0_external, dtype=np.float64).reshape(-1)
+    if g0r.size == 1:
+        g0r = np.full(nr, float(g0r[0]))
+    elif g0r.size < nr:  # upstream: rays beyond the array reuse element 0
+        g0r = np.concatenate([g0r, np.full(nr - g0r.size, g0r[0])])
+    dirs = np.ascontiguousarray(dirs.reshape(-1))
+    g0r = np.ascontiguousarray(g0r[:nr])
+    if workspace is None:
+        workspace = torch.empty(9 * density.numel(), dtype=torch.float64, device=density.device)
+    outs = _empty(density, 5)
+    rc = _lib.lib().p3d_fuv_columns(nx, ny, nz, density.data_ptr(), x_h2.data_ptr(),
+                                    x_co.data_ptr(), float(cell_size), nr, _lib.dptr(dirs),
+                                    _lib.dptr(g0r), float(av_per_nh),
+                                    *[o.data_ptr() for o in outs], workspace.data_ptr(),
+                                    stream_handle())
+    _lib.check(rc, "p3d_fuv_columns")
+    return tuple(outs)
+
+
+def mean(x, out=None):
+    x = _t(x)
+    if out is None:
+        out = torch.empty(1, dtype=torch.float64, device=x.device)
+    _lib.check(_lib.lib().p3d_mean(x.numel(), x.data_ptr(), out.data_ptr(), stream_handle()),
+               "p3d_mean")
+    return out
+
+
+def shield_cr(n_h2_col, n_co_col, n_h_col, t_mean, zeta0):
+    """f_shield_H2(N_H2, mean T), f_shield_CO(N_CO, N_H2), zeta_CR(N_H)."""
+    a, b, c = _t(n_h2_col), _t(n_co_col), _t(n_h_col)
+    outs = _empty(a, 3)
+    rc = _lib.lib().p3d_shield_cr(a.numel(), a.data_ptr(), b.data_ptr(), c.data_ptr(),
+                                  _t(t_mean).data_ptr(), float(zeta0),
+                                  *[o.data_ptr() for o in outs], stream_handle())
+    _lib.check(rc, "p3d_shield_cr")
+    return tuple(outs)
+
+
+def dust_state(g0, a_v, n_h, t_gas, f_nano, sigma_nano, sigma_large_c, sigma_sil):
+    """T_dust and R_H2 (grains/themis_tables.py:278-354)."""
+    ins = [_t(v) for v in (g0, a_v, n_h, t_gas, f_nano)]
+    outs = _empty(ins[0], 2)
+    rc = _lib.lib().p3d_dust_state(ins[0].numel(), *[v.data_ptr() for v in ins],
+                                   float(sigma_nano), float(sigma_large_c), float(sigma_sil),
+                                   *[o.data_ptr() for o in outs], stream_handle())
+    _lib.check(rc, "p3d_dust_state")
+    return tuple(outs)
+
+
+def euler_rates(n_h, g0, a_v, t_gas, zeta, fsh_h2, fsh_co, r_h2):
+    """k_pd_base, k_ci, alpha_cp, k_co_total, r_h2_half_nh, r_cr_base."""
+    ins = [_t(v) for v in (n_h, g0, a_v, t_gas, zeta, fsh_h2, fsh_co, r_h2)]
+    outs = _empty(ins[0], 6)
+    rc = _lib.lib().p3d_euler_rates(ins[0].numel(), *[v.data_ptr() for v in ins],
+                                    *[o.data_ptr() for o in outs], stream_handle())
+    _lib.check(rc, "p3d_euler_rates")
+    return tuple(outs)
+
+
+def thermal_ctx(n_h, g0, a_v, fsh_h2, zeta, t_dust, gamma_pe, r_h2, x_hi, x_h2, x_e, x_co,
+                x_cp, x_o, x_c):
+    """The 9 computed context arrays of solver_3d.py:693-731 (t_dust passes through)."""
+    ins = [_t(v) for v in (n_h, g0, a_v, fsh_h2, zeta, t_dust, gamma_pe, r_h2, x_hi, x_h2, x_e,
+                           x_co, x_cp, x_o, x_c)]
+    outs = _empty(ins[0], 9)
+    rc = _lib.lib().p3d_thermal_ctx(ins[0].numel(), *[v.data_ptr() for v in ins],
+                                    *[o.data_ptr() for o in outs], stream_handle())
+    _lib.check(rc, "p3d_thermal_ctx")
+    return tuple(outs)
+
+
+def heating_cooling(t_gas, n_h, g0, a_v, zeta, fsh_h2, x_e, x_hi, x_h2, x_c, x_cp, x_o, x_co,
+                    t_dust, gamma_pe, r_h2):
+    """Gamma and Lambda at t_gas (solver_3d.py:898-1050)."""
+    ins = [_t(v) for v in (t_gas, n_h, g0, a_v, zeta, fsh_h2, x_e, x_hi, x_h2, x_c, x_cp, x_o,
+                           x_co, t_dust, gamma_pe, r_h2)]
+    outs = _empty(ins[0], 2)
+    lp = [np.ascontiguousarray(p) for p in line_params()]
+    rc = _lib.lib().p3d_heating_cooling(ins[0].numel(), *[v.data_ptr() for v in ins],
+                                        *[_lib.dptr(p) for p in lp],
+                                        *[o.data_ptr() for o in outs], stream_handle())
+    _lib.check(rc, "p3d_heating_cooling")
+    return tuple(outs)
+
+
+def solve_thermal(gamma_fixed, gg_coeff, t_dust, n_e, n_hi, n_h2, n_co, n_cp, n_o, n_c,
+                  t_gas, n_bisect=10, n_newton=3):
+    """Thermal root-find on device tensors, t_gas overwritten."""
+    ins = [_t(v) for v in (gamma_fixed, gg_coeff, t_dust, n_e, n_hi, n_h2, n_co, n_cp, n_o,
+                           n_c)]
+    lp = [np.ascontiguousarray(p) for p in line_params()]
+    rc = _lib.lib().p3d_solve_thermal_vec(ins[0].numel(), *[v.data_ptr() for v in ins],
+                                          *[_lib.dptr(p) for p in lp], int(n_bisect),
+                                          int(n_newton), _t(t_gas).data_ptr(), stream_handle())
+    _lib.check(rc, "p3d_solve_thermal_vec")
+    return t_gas
