@@ -170,6 +170,9 @@ euler_kernel(int64_t n, const double *__restrict__ nh, const double *__restrict_
              double *__restrict__ x_h2, double *__restrict__ x_cp, double *__restrict__ x_c,
              double *__restrict__ x_co, double *__restrict__ x_o, double *__restrict__ x_e,
              const double *__restrict__ x_hcop, int64_t n_substeps, double conv_tol) {
+  // No fused multiply-adds: the step-size limiter compares max_rate * dt with
+  // 0.1, so the operations are kept exactly as the native core rounds them.
+#pragma clang fp contract(off)
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double hi = x_hi[i], h2 = x_h2[i], cp = x_cp[i], c = x_c[i], co = x_co[i];
