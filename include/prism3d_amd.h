@@ -156,14 +156,15 @@ typedef struct p3d_solve_opts {
 /* Integrate every cell to its steady state / end time with the
  * variable-order BDF (orders 1-5, Newton with dense partially pivoted LU,
  * one cell per 64-lane workgroup).  y (n_species, n) is updated in place;
- * converged[i] = early-stop flag, steps[i] = accepted BDF steps (either may
- * be NULL).  Replaces ChemistrySolver.solve_steady_state
+ * converged[i] = early-stop flag; stats (n x 4, row per cell) = accepted BDF
+ * steps, LU factorisations, RHS evaluations, Jacobian evaluations (either
+ * output may be NULL).  Replaces ChemistrySolver.solve_steady_state
  * (prism3d/chemistry/solver.py:145-225) over a batch of cells. */
 int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h,
                    const double *t_gas, const double *g0, const double *a_v,
                    const double *zeta, const double *fsh_h2,
                    const double *fsh_co, const p3d_solve_opts *opts,
-                   int32_t *converged, int32_t *steps, void *stream);
+                   int32_t *converged, int32_t *stats, void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Whole-grid kernels around the per-cell step                              */

@@ -140,20 +140,22 @@ class ChemistrySolver:
     def solve_steady_batch(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0, f_shield_CO=1.0,
                            opts=None, sync=True):
         """Solve every cell of y (n_species, n_cells), float64 device tensor,
-        in place.  Returns (converged int32[n], accepted steps int32[n])."""
+        in place.  Returns (converged int32[n], stats int32[n, 4]) with stats
+        columns accepted steps, LU factorisations, RHS and Jacobian
+        evaluations."""
         self._check_state(y)
         n = y.shape[1]
         p = _cell_arrays(n, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO)
         conv = torch.zeros(n, dtype=torch.int32, device=y.device)
-        steps = torch.zeros(n, dtype=torch.int32, device=y.device)
+        stats = torch.zeros((n, 4), dtype=torch.int32, device=y.device)
         o = opts if opts is not None else solve_opts()
         rc = _lib.lib().p3d_chem_solve(self.dnet.handle, n, y.data_ptr(),
                                        *[t.data_ptr() for t in p], o, conv.data_ptr(),
-                                       steps.data_ptr(), stream_handle())
+                                       stats.data_ptr(), stream_handle())
         _lib.check(rc, "p3d_chem_solve")
         if sync:
             torch.cuda.synchronize(y.device)
-        return conv, steps
+        return conv, stats
 
     def rhs_batch(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0, f_shield_CO=1.0):
         self._check_state(y)

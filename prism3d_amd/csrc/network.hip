@@ -572,7 +572,7 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
                   const double *__restrict__ AV, const double *__restrict__ zeta,
                   const double *__restrict__ fH2, const double *__restrict__ fCO,
                   p3d_solve_opts o, int32_t *__restrict__ conv_out,
-                  int32_t *__restrict__ steps_out) {
+                  int32_t *__restrict__ stats_out) {
   extern __shared__ double smem[];
   const int64_t cell = blockIdx.x;
   const int lane = threadIdx.x;
@@ -616,7 +616,12 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
   if (act) y[(int64_t)lane * n + cell] = yc;
   if (lane == 0) {
     if (conv_out) conv_out[cell] = conv;
-    if (steps_out) steps_out[cell] = st.steps;
+    if (stats_out) {
+      stats_out[4 * cell + 0] = st.steps;
+      stats_out[4 * cell + 1] = st.lu;
+      stats_out[4 * cell + 2] = st.rhs;
+      stats_out[4 * cell + 3] = st.jac;
+    }
   }
 }
 
@@ -862,7 +867,7 @@ int p3d_chem_jacobian(int64_t handle, int64_t n, const double *y, const double *
 int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, const double *t_gas,
                    const double *g0, const double *a_v, const double *zeta,
                    const double *fsh_h2, const double *fsh_co, const p3d_solve_opts *opts,
-                   int32_t *converged, int32_t *steps, void *stream) {
+                   int32_t *converged, int32_t *stats, void *stream) {
   Network *net = find_net(handle);
   int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
   if (rc != P3D_OK || n == 0) return rc;
@@ -882,7 +887,7 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
   hipLaunchKernelGGL(chem_solve_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream),
                      net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o, converged,
-                     steps);
+                     stats);
   return launch_status("chem_solve_kernel");
 }
 

@@ -183,8 +183,9 @@ def test_solve_tight_vs_oracle_and_upstream(gpu):
     t_end = float(g['t_end'])
     opts = solve_opts(t_max=t_end, n_checkpoints=2, early_stop=False, rtol=1e-9, atol=1e-24,
                       post_mode=-1)
-    _, steps = cs.solve_steady_batch(y, g['n_H'], g['T'], g['G0'], g['A_V'], g['zeta'],
+    _, stats = cs.solve_steady_batch(y, g['n_H'], g['T'], g['G0'], g['A_V'], g['zeta'],
                                      g['fsh_H2'], g['fsh_CO'], opts=opts)
+    steps = stats[:, 0]
     got = y.cpu().numpy()
     ref = g['y_end'].T
     m = ref > 1e-14
