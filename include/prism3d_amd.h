@@ -151,6 +151,10 @@ typedef struct p3d_solve_opts {
                           1 H+/e- correction then exact projection          */
   int32_t max_steps;   /* per span                                          */
   double x_c_total, x_o_total; /* element totals for the conservation step */
+  int32_t variant;     /* kernel variant: 0 auto, 1 LU in LDS, 2 LU in registers
+                          (n_species <= 32), 3 LU in LDS with unrolled loops
+                          (n_species <= 32) */
+  int32_t reserved;
 } p3d_solve_opts;
 
 /* Integrate every cell to its steady state / end time with the

@@ -27,7 +27,8 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("t_max", _dbl), ("t_start", _dbl), ("n_checkpoints", _i32),
                 ("early_stop", _i32), ("rtol", _dbl), ("atol", _dbl),
                 ("stop_rtol", _dbl), ("post_mode", _i32), ("max_steps", _i32),
-                ("x_c_total", _dbl), ("x_o_total", _dbl)]
+                ("x_c_total", _dbl), ("x_o_total", _dbl), ("variant", _i32),
+                ("reserved", _i32)]
 
 
 # (name, restype, argtypes) of every exported entry point

@@ -25,7 +25,7 @@ GAS_HE = 8.51e-2
 
 def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=1e-4,
                atol=1e-22, stop_rtol=1e-3, post_mode=0, max_steps=100000, x_c_total=GAS_C,
-               x_o_total=GAS_O):
+               x_o_total=GAS_O, variant=0):
     """p3d_solve_opts; the defaults are the upstream refine-path settings
     (solver.py:150-223)."""
     o = _lib.SolveOpts()
@@ -37,6 +37,7 @@ def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=
     o.post_mode = int(post_mode)
     o.max_steps = int(max_steps)
     o.x_c_total, o.x_o_total = float(x_c_total), float(x_o_total)
+    o.variant = int(variant)
     return o
 
 

@@ -33,7 +33,7 @@ constexpr int kMaxOrder = 5;
 constexpr int kNumD = kMaxOrder + 3;
 
 struct NetDev {
-  int ns, nr, e_idx, nnz;
+  int ns, nr, e_idx, nnz, n_rhs_e;
   int roles[7];  // H, H2, H+, C+, C, CO, O
   const int *rtype;
   const double *alpha, *beta, *gamma;
@@ -42,6 +42,7 @@ struct NetDev {
   const int *rhs_off; // ns+1
   const int *rhs_rxn;
   const double *rhs_sgn;
+  const int *rhs_pk;  // packed RHS entries: (reaction << 1) | (product ? 1 : 0)
   const int *jrow, *jcol, *joff;  // nnz, nnz, nnz+1
   const int *jt_rxn, *jt_slot;
   const double *jt_coef;
@@ -78,6 +79,9 @@ struct Lds {
   double *J;     // ns x ld
   double *M;     // ns x ld
   int *piv;      // 64
+  double *ru;    // 36: step-size change matrix RU of the backward differences
+  int *csr_off;  // 65: per-species offsets into csr (LDS copy of rhs_off)
+  int *csr;      // n_rhs_e packed RHS entries (LDS copy of rhs_pk)
   int ld;
 };
 
@@ -90,13 +94,41 @@ __device__ __forceinline__ Lds carve(double *base, const NetDev &N) {
   L.J = L.ys + kMaxSpecies;
   L.M = L.J + N.ns * L.ld;
   L.piv = reinterpret_cast<int *>(L.M + N.ns * L.ld);
+  L.ru = reinterpret_cast<double *>(L.piv + kMaxSpecies);
+  L.csr_off = reinterpret_cast<int *>(L.ru + 36);
+  L.csr = L.csr_off + kMaxSpecies + 1;
   return L;
 }
 
-static size_t lds_bytes(int ns, int nr) {
+static size_t lds_bytes(int ns, int nr, int n_rhs_e) {
   int ld = ns | 1;
-  return sizeof(double) * (size_t)(3 * nr + kMaxSpecies + 2 * ns * ld) + sizeof(int) * kMaxSpecies;
+  return sizeof(double) * (size_t)(3 * nr + kMaxSpecies + 2 * ns * ld + 36) +
+         sizeof(int) * (size_t)(2 * kMaxSpecies + 1 + n_rhs_e);
 }
+
+// Copies the RHS stoichiometry lists into LDS (once per workgroup).
+__device__ __forceinline__ void stage_tables(const NetDev &N, const Lds &L, int lane) {
+  for (int i = lane; i <= N.ns; i += 64) L.csr_off[i] = N.rhs_off[i];
+  for (int i = lane; i < N.n_rhs_e; i += 64) L.csr[i] = N.rhs_pk[i];
+}
+
+// Reactions evaluated by one lane (r = lane + 64 s), kept in registers.
+constexpr int kRxPerLane = 2;
+struct RxCache {
+  int t[kRxPerLane], a[kRxPerLane], b[kRxPerLane];
+  double k[kRxPerLane];
+  __device__ __forceinline__ void load(const NetDev &N, const Lds &L, int lane) {
+#pragma unroll
+    for (int q = 0; q < kRxPerLane; ++q) {
+      const int r = lane + 64 * q;
+      const bool v = r < N.nr;
+      t[q] = v ? N.rtype[r] : -1;
+      a[q] = v ? N.react[2 * r] : 0;
+      b[q] = v ? N.react[2 * r + 1] : 0;
+      k[q] = v ? L.kco[r] : 0.0;
+    }
+  }
+};
 
 // Rate coefficients k_r (solver.py:100-133).
 __device__ void rate_coeffs(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
@@ -190,6 +222,53 @@ __device__ void eval_reactions(const NetDev &N, const CellParams &p, const Lds &
   __syncthreads();
 }
 
+// dy/dt of species `lane` from the register reaction cache and the LDS
+// stoichiometry lists (networks with <= 128 reactions).  Same accumulation
+// order as rhs_lane.  Requires y published.
+__device__ __forceinline__ double rhs_cached(const NetDev &N, const CellParams &p, const Lds &L,
+                                             const RxCache &rc, int lane) {
+  double pm, dpm;
+  pah_minus(L.ys[N.e_idx], p, pm, dpm);
+#pragma unroll
+  for (int q = 0; q < kRxPerLane; ++q) {
+    const int t = rc.t[q];
+    if (t < 0) continue;
+    const double ya = L.ys[rc.a[q]];
+    double rate;
+    if (t == P3D_RX_TWO_BODY) {
+      const double yb = (rc.b[q] == P3D_SP_PAH_MINUS) ? pm : L.ys[rc.b[q]];
+      rate = rc.k[q] * ya * yb * p.nH;
+    } else if (t == P3D_RX_GRAIN) {
+      rate = rc.k[q] * ya * p.nH * 0.5;
+    } else {
+      rate = rc.k[q] * ya;
+    }
+    L.part[2 * (lane + 64 * q)] = rate;
+  }
+  __syncthreads();
+  double acc = 0.0;
+  if (lane < N.ns) {
+    const int e0 = L.csr_off[lane], e1 = L.csr_off[lane + 1];
+    int e = e0;
+    for (; e + 4 <= e1; e += 4) {
+      const int c0 = L.csr[e], c1 = L.csr[e + 1], c2 = L.csr[e + 2], c3 = L.csr[e + 3];
+      const double v0 = L.part[2 * (c0 >> 1)], v1 = L.part[2 * (c1 >> 1)],
+                   v2 = L.part[2 * (c2 >> 1)], v3 = L.part[2 * (c3 >> 1)];
+      acc = (c0 & 1) ? acc + v0 : acc - v0;
+      acc = (c1 & 1) ? acc + v1 : acc - v1;
+      acc = (c2 & 1) ? acc + v2 : acc - v2;
+      acc = (c3 & 1) ? acc + v3 : acc - v3;
+    }
+    for (; e < e1; ++e) {
+      const int c0 = L.csr[e];
+      const double v0 = L.part[2 * (c0 >> 1)];
+      acc = (c0 & 1) ? acc + v0 : acc - v0;
+    }
+  }
+  __syncthreads();  // part[] is reused by the next evaluation
+  return acc;
+}
+
 // dy/dt of species `lane` (upstream accumulation order).  Requires y published.
 __device__ double rhs_lane(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
   eval_reactions(N, p, L, lane, false);
@@ -271,43 +350,206 @@ __device__ double lu_solve(const NetDev &N, const Lds &L, double b, int lane) {
   return b;
 }
 
-__device__ __forceinline__ double rms(double v, int ns) { return sqrt(wave_sum(v * v) / ns); }
+// Linear-algebra policies of the Newton iteration matrix M = I - c J.
+//  LinLds: J and M in LDS, any n <= 64 (lane = column in the elimination).
+//  LinReg: n <= 32; lane i keeps row i of J and of M in registers.  The
+//          elimination never moves rows between lanes: step k picks the
+//          pivot lane p_k by a wavefront argmax, reads the pivot row with
+//          readlane (scalar operands for every lane) and updates the rows not
+//          yet used as pivots with plain FMAs.  The solves replay the same
+//          lane order and return component j on lane j.
+struct LinLds {
+  __device__ void load_jac(const NetDev &, const Lds &, int) {}
+  __device__ void factor(const NetDev &N, const Lds &L, double c, int lane) {
+    lu_factor(N, L, c, lane);
+  }
+  __device__ double solve(const NetDev &N, const Lds &L, double b, int lane) {
+    return lu_solve(N, L, b, lane);
+  }
+};
 
-// Step-size change of the backward differences D[0..order] by `factor`.
-__device__ void change_D(double (&D)[kNumD], int order, double factor) {
-  double R[kMaxOrder + 1][kMaxOrder + 1], U[kMaxOrder + 1][kMaxOrder + 1];
-#pragma unroll
-  for (int j = 0; j <= kMaxOrder; ++j) {
-    double cr = 1.0, cu = 1.0;
-#pragma unroll
-    for (int i = 0; i <= kMaxOrder; ++i) {
-      if (i > 0) {
-        cr *= ((double)(i - 1) - factor * (double)j) / (double)i;
-        cu *= ((double)(i - 1) - (double)j) / (double)i;
+// LinLdsU: M in LDS for n <= NP (32), every loop fully unrolled so that the
+// LDS loads of a step issue together, ahead of the dependency chain.  Lane L
+// handles column L & 31 of the trailing update on rows of parity L >> 5.
+template <int NP>
+struct LinLdsU {
+  __device__ void load_jac(const NetDev &, const Lds &, int) {}
+
+  __device__ __forceinline__ void factor(const NetDev &N, const Lds &L, double c, int lane) {
+    const int n = N.ns, ld = L.ld;
+    for (int idx = lane; idx < n * ld; idx += 64) {
+      const int i = idx / ld, j = idx - i * ld;
+      L.M[idx] = (i == j ? 1.0 : 0.0) - c * L.J[idx];
+    }
+    __syncthreads();
+    const int col = lane & 31, par = lane >> 5;
+#pragma clang loop unroll(full)
+    for (int k = 0; k < NP; ++k) {
+      if (k < n) {
+        const double v = (lane >= k && lane < n) ? fabs(L.M[lane * ld + k]) : -1.0;
+        const int pk = wave_argmax(v, lane);
+        if (pk != k && lane < n) {
+          const double a = L.M[k * ld + lane];
+          L.M[k * ld + lane] = L.M[pk * ld + lane];
+          L.M[pk * ld + lane] = a;
+        }
+        if (lane == 0) L.piv[k] = pk;
+        __syncthreads();
+        const double inv = 1.0 / L.M[k * ld + k];
+        const double ukj = (col < n) ? L.M[k * ld + col] : 0.0;
+        double lik[NP];
+#pragma clang loop unroll(full)
+        for (int i = k + 1; i < NP; ++i) lik[i] = (i < n) ? L.M[i * ld + k] * inv : 0.0;
+#pragma clang loop unroll(full)
+        for (int i = k + 1; i < NP; ++i) {
+          if (i < n && ((i - k - 1) & 1) == par) {
+            if (col == k) L.M[i * ld + k] = lik[i];
+            else if (col > k && col < n) L.M[i * ld + col] -= lik[i] * ukj;
+          }
+        }
+        __syncthreads();
       }
-      R[i][j] = cr;
-      U[i][j] = cu;
     }
   }
+
+  __device__ __forceinline__ double solve(const NetDev &N, const Lds &L, double b, int lane) {
+    const int n = N.ns, ld = L.ld;
+    double mrow[NP];
+#pragma clang loop unroll(full)
+    for (int k = 0; k < NP; ++k) mrow[k] = (lane < n && k < n) ? L.M[lane * ld + k] : 0.0;
+#pragma clang loop unroll(full)
+    for (int k = 0; k < NP; ++k) {
+      if (k < n) {
+        const int pk = uniform(L.piv[k]);
+        if (pk != k) {
+          const double bk = bcast(b, k), bp = bcast(b, pk);
+          if (lane == k) b = bp;
+          if (lane == pk) b = bk;
+        }
+      }
+    }
+#pragma clang loop unroll(full)
+    for (int k = 0; k < NP - 1; ++k) {
+      if (k < n - 1) {
+        const double bk = bcast(b, k);
+        if (lane > k && lane < n) b -= mrow[k] * bk;
+      }
+    }
+#pragma clang loop unroll(full)
+    for (int k = NP - 1; k >= 0; --k) {
+      if (k < n) {
+        if (lane == k) b /= mrow[k];
+        const double bk = bcast(b, k);
+        if (lane < k) b -= mrow[k] * bk;
+      }
+    }
+    return b;
+  }
+};
+
+template <int NP>
+struct LinReg {
+  // rows >= n are identity rows, so the elimination always runs over NP;
+  // the pivot lane of step k is kept in LDS (L.piv[k])
+  double Mr[NP];
+  int step_of;  // elimination step at which this lane's row became the pivot
+  int src;      // lane holding solution component `lane`
+
+  __device__ void load_jac(const NetDev &, const Lds &, int) {}
+
+  __device__ __forceinline__ void factor(const NetDev &N, const Lds &L, double c, int lane) {
+    const int n = N.ns;
+    const bool row = lane < n;
+    const double *Jrow = L.J + (row ? lane : 0) * L.ld;
+#pragma clang loop unroll(full)
+    for (int j = 0; j < NP; ++j) {
+      const double jv = (row && j < n) ? Jrow[j] : 0.0;
+      Mr[j] = (j == lane ? 1.0 : 0.0) - c * jv;
+    }
+    step_of = NP;
+    src = lane;
+#pragma clang loop unroll(full)
+    for (int k = 0; k < NP; ++k) {
+      const double v = (lane < NP && step_of == NP) ? fabs(Mr[k]) : -1.0;
+      const int p = wave_argmax(v, lane);
+      if (lane == 0) L.piv[k] = p;
+      if (lane == p) step_of = k;
+      if (lane == k) src = p;
+      const double inv = 1.0 / bcast(Mr[k], p);
+      const bool upd = step_of == NP;
+      const double l = Mr[k] * inv;
+      if (upd) Mr[k] = l;
+#pragma clang loop unroll(full)
+      for (int j = k + 1; j < NP; ++j) {
+        const double u = bcast(Mr[j], p);
+        if (upd) Mr[j] -= l * u;
+      }
+    }
+    __syncthreads();
+  }
+
+  __device__ __forceinline__ double solve(const NetDev &N, const Lds &L, double b, int lane) {
+    if (lane >= N.ns) b = 0.0;
+#pragma clang loop unroll(full)
+    for (int k = 0; k < NP; ++k) {  // forward: unit-lower factor in pivot order
+      const double yk = bcast(b, uniform(L.piv[k]));
+      if (step_of > k) b -= Mr[k] * yk;
+    }
+#pragma clang loop unroll(full)
+    for (int k = NP - 1; k >= 0; --k) {  // backward: upper factor
+      const int pk = uniform(L.piv[k]);
+      const double q = b / Mr[k];
+      if (lane == pk) b = q;
+      const double xk = bcast(b, pk);
+      if (step_of < k) b -= Mr[k] * xk;
+    }
+    return __shfl(b, src, 64);
+  }
+};
+
+__device__ __forceinline__ double rms(double v, int ns) { return sqrt(wave_sum(v * v) / ns); }
+
+// Step-size change of the backward differences D[0..order] by `factor`:
+// D <- (R(factor) U)^T D with R_im = prod_{q=1..i} (q-1 - factor m)/q and
+// U = R(1).  Lanes 0..35 build RU (entry (i, j) per lane) in LDS, then every
+// species lane applies it to its own differences.
+__device__ void change_D(const Lds &L, int lane, double (&D)[kNumD], int order, double factor) {
+  if (lane < 36) {
+    const int i = lane / 6, j = lane % 6;
+    double acc = 0.0;
+    if (i <= order && j <= order) {
+      for (int m = 0; m <= order; ++m) {
+        double r = 1.0, u = 1.0;
+        for (int q = 1; q <= i; ++q) r *= ((double)(q - 1) - factor * (double)m) / (double)q;
+        for (int q = 1; q <= m; ++q) u *= ((double)(q - 1) - (double)j) / (double)q;
+        acc += r * u;
+      }
+    }
+    L.ru[lane] = acc;
+  }
+  __syncthreads();
   double Dn[kMaxOrder + 1];
 #pragma unroll
   for (int j = 0; j <= kMaxOrder; ++j) {
     double acc = 0.0;
 #pragma unroll
-    for (int i = 0; i <= kMaxOrder; ++i) {
-      if (i > order) continue;
-      double ru = 0.0;
-#pragma unroll
-      for (int m = 0; m <= kMaxOrder; ++m)
-        if (m <= order) ru += R[i][m] * U[m][j];
-      acc += ru * D[i];
-    }
+    for (int i = 0; i <= kMaxOrder; ++i) acc += L.ru[6 * i + j] * D[i];
     Dn[j] = acc;
+    // keep the 36 RU loads from being hoisted into registers all at once
+    __asm__ volatile("" ::: "memory");
   }
 #pragma unroll
   for (int j = 0; j <= kMaxOrder; ++j)
     if (j <= order) D[j] = Dn[j];
+  __syncthreads();
 }
+
+// NDF coefficients (Shampine & Reichelt 1997): kappa_k, gamma_k = sum 1/j,
+// alpha_k = (1 - kappa_k) gamma_k, error constants kappa_k gamma_k + 1/(k+1).
+__constant__ double c_gam[kMaxOrder + 2] = {0.0, 1.0, 1.5, 1.8333333333333333,
+                                            2.0833333333333333, 2.2833333333333333,
+                                            2.45};
+__constant__ double c_kappa[kMaxOrder + 1] = {0.0, -0.1850, -1.0 / 9.0, -0.0823, -0.0415, 0.0};
 
 struct BdfStats {
   int steps, rhs, jac, lu, rejects;
@@ -315,22 +557,19 @@ struct BdfStats {
 
 // Variable-order BDF from t0 to t1 for the wave's cell.  y: lane's species.
 // Returns 0 on success, -1 step underflow, -2 step limit.
-__device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L, int lane,
-                             double &y, double t0, double t1, double rtol, double atol,
-                             double max_step, int max_steps, BdfStats &st) {
+template <class Lin>
+__device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
+                                             Lin &lin, const RxCache &rc, int lane, double &y,
+                                             double t0, double t1, double rtol, double atol,
+                                             double max_step, int max_steps, BdfStats &st) {
+  const bool cached = N.nr <= 64 * kRxPerLane;
+  auto rhs = [&]() { return cached ? rhs_cached(N, p, L, rc, lane) : rhs_lane(N, p, L, lane); };
   const int ns = N.ns;
   const bool act = lane < ns;
-  const double kappa[kMaxOrder + 1] = {0.0, -0.1850, -1.0 / 9.0, -0.0823, -0.0415, 0.0};
-  double gam[kMaxOrder + 2], alp[kMaxOrder + 1], ec[kMaxOrder + 2];
-  gam[0] = 0.0;
-#pragma unroll
-  for (int i = 1; i <= kMaxOrder + 1; ++i) gam[i] = gam[i - 1] + 1.0 / i;
-#pragma unroll
-  for (int i = 0; i <= kMaxOrder; ++i) {
-    alp[i] = (1.0 - kappa[i]) * gam[i];
-    ec[i] = kappa[i] * gam[i] + 1.0 / (i + 1);
-  }
-  ec[kMaxOrder + 1] = 0.0;
+  // coefficients indexed by the (wave-uniform) order: scalar loads
+  auto gam = [](int k) { return c_gam[k]; };
+  auto alp = [](int k) { return (1.0 - c_kappa[k]) * c_gam[k]; };
+  auto ec = [](int k) { return c_kappa[k] * c_gam[k] + 1.0 / (k + 1); };
 
   const double span = t1 - t0;
   if (!(span > 0.0)) return 0;
@@ -338,7 +577,7 @@ __device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
   if (!act) y = 0.0;
 
   publish_y(N, L, y, lane);
-  double f0 = rhs_lane(N, p, L, lane);
+  double f0 = rhs();
   st.rhs++;
   // initial step (Hairer, Norsett & Wanner 1993, II.4), order 1
   double scale = atol + rtol * fabs(y);
@@ -346,7 +585,7 @@ __device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
   double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
   h0 = fmin(h0, span);
   publish_y(N, L, y + h0 * f0, lane);
-  double f1 = rhs_lane(N, p, L, lane);
+  double f1 = rhs();
   st.rhs++;
   double d2 = rms(act ? (f1 - f0) / scale : 0.0, ns) / h0;
   double dm = fmax(d1, d2);
@@ -362,6 +601,7 @@ __device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
   double t = t0;
   publish_y(N, L, y, lane);
   jacobian(N, p, L, lane);
+  lin.load_jac(N, L, lane);
   st.jac++;
   bool jac_current = true, lu_valid = false;
   double c_lu = 0.0;
@@ -372,9 +612,9 @@ __device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
   while (t < t1) {
     if (steps++ > max_steps) return -2;
     const double hmin = 10.0 * fabs(nextafter(t, INFINITY) - t);
-    if (h > max_step) { change_D(D, order, max_step / h); h = max_step; n_equal = 0; lu_valid = false; }
-    if (h < hmin) { change_D(D, order, hmin / h); h = hmin; n_equal = 0; lu_valid = false; }
-    if (t + h > t1) { change_D(D, order, (t1 - t) / h); h = t1 - t; n_equal = 0; lu_valid = false; }
+    if (h > max_step) { change_D(L, lane, D, order, max_step / h); h = max_step; n_equal = 0; lu_valid = false; }
+    if (h < hmin) { change_D(L, lane, D, order, hmin / h); h = hmin; n_equal = 0; lu_valid = false; }
+    if (t + h > t1) { change_D(L, lane, D, order, (t1 - t) / h); h = t1 - t; n_equal = 0; lu_valid = false; }
 
     double ynew = 0.0;
     for (;;) {  // attempt the step until accepted
@@ -385,14 +625,14 @@ __device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
         if (j <= order) ypred += D[j];
 #pragma unroll
       for (int j = 1; j < kNumD; ++j)
-        if (j <= order) psi += gam[j] * D[j];
-      psi /= alp[order];
+        if (j <= order) psi += gam(j) * D[j];
+      psi /= alp(order);
       scale = atol + rtol * fabs(ypred);
-      const double c = h / alp[order];
+      const double c = h / alp(order);
       bool converged = false;
       for (;;) {
         if (!lu_valid || c != c_lu) {
-          lu_factor(N, L, c, lane);
+          lin.factor(N, L, c, lane);
           st.lu++;
           lu_valid = true;
           c_lu = c;
@@ -403,11 +643,11 @@ __device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
         converged = false;
         for (int it = 0; it < 4; ++it) {
           publish_y(N, L, ynew, lane);
-          double f = rhs_lane(N, p, L, lane);
+          double f = rhs();
           st.rhs++;
           if (!isfinite(wave_sum(act ? f : 0.0))) break;
           double dy = act ? c * f - psi - d : 0.0;
-          dy = lu_solve(N, L, dy, lane);
+          dy = lin.solve(N, L, dy, lane);
           if (!act) dy = 0.0;
           const double dyn = rms(act ? dy / scale : 0.0, ns);
           if (dy_old >= 0.0) rate = dyn / dy_old;
@@ -425,6 +665,7 @@ __device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
         if (!jac_current) {
           publish_y(N, L, ypred, lane);
           jacobian(N, p, L, lane);
+          lin.load_jac(N, L, lane);
           st.jac++;
           jac_current = true;
           lu_valid = false;
@@ -435,19 +676,19 @@ __device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
       if (!converged) {
         h *= 0.5;
         if (h < hmin) return -1;
-        change_D(D, order, 0.5);
+        change_D(L, lane, D, order, 0.5);
         n_equal = 0;
         lu_valid = false;
         st.rejects++;
         continue;
       }
       scale = atol + rtol * fabs(ynew);
-      err = act ? ec[order] * d : 0.0;
+      err = act ? ec(order) * d : 0.0;
       const double en = rms(act ? err / scale : 0.0, ns);
       if (en > 1.0) {
         double factor = fmax(0.2, 0.9 * pow(en, -1.0 / (order + 1)));
         h *= factor;
-        change_D(D, order, factor);
+        change_D(L, lane, D, order, factor);
         n_equal = 0;
         lu_valid = false;
         st.rejects++;
@@ -480,18 +721,22 @@ __device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
 #pragma unroll
       for (int j = 0; j < kNumD; ++j)
         if (j == order) dm1 = D[j];
-      em = rms(act ? ec[order - 1] * dm1 / scale : 0.0, ns);
+      em = rms(act ? ec(order - 1) * dm1 / scale : 0.0, ns);
     }
     if (order < kMaxOrder) {
       double dp2 = 0.0;
 #pragma unroll
       for (int j = 0; j < kNumD; ++j)
         if (j == order + 2) dp2 = D[j];
-      ep = rms(act ? ec[order + 1] * dp2 / scale : 0.0, ns);
+      ep = rms(act ? ec(order + 1) * dp2 / scale : 0.0, ns);
     }
-    double fm = order > 1 ? (em > 0.0 ? pow(em, -1.0 / order) : INFINITY) : 0.0;
-    double fz = e0 > 0.0 ? pow(e0, -1.0 / (order + 1)) : INFINITY;
-    double fp = order < kMaxOrder ? (ep > 0.0 ? pow(ep, -1.0 / (order + 2)) : INFINITY) : 0.0;
+    // the three candidate factors e^(-1/(q+1)) in one pow: lanes 0, 1, 2
+    const double ev = lane == 0 ? em : (lane == 1 ? e0 : ep);
+    const double ex = -1.0 / (double)(order + (lane == 0 ? 0 : (lane == 1 ? 1 : 2)));
+    const double fv = ev > 0.0 ? pow(ev, ex) : INFINITY;
+    double fm = order > 1 ? bcast(fv, 0) : 0.0;
+    double fz = bcast(fv, 1);
+    double fp = order < kMaxOrder ? bcast(fv, 2) : 0.0;
     int dord = 0;
     double best = fz;
     if (fm > best) { best = fm; dord = -1; }
@@ -499,7 +744,7 @@ __device__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
     order += dord;
     const double factor = fmin(10.0, 0.9 * best);
     h *= factor;
-    change_D(D, order, factor);
+    change_D(L, lane, D, order, factor);
     n_equal = 0;
     lu_valid = false;
   }
@@ -566,6 +811,7 @@ __device__ __forceinline__ CellParams load_params(int64_t c, const double *nH, c
   return p;
 }
 
+template <class Lin>
 __global__ void __launch_bounds__(64)
 chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
                   const double *__restrict__ T, const double *__restrict__ G0,
@@ -581,20 +827,25 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
   const bool act = lane < N.ns;
   double yc = act ? y[(int64_t)lane * n + cell] : 0.0;
   rate_coeffs(N, p, L, lane);
+  stage_tables(N, L, lane);
   __syncthreads();
+  RxCache rc;
+  rc.load(N, L, lane);
 
   double tmax = o.t_max;
   if (tmax <= 0.0) tmax = fmin(3e17, fmax(1e16, 3e17 / fmax(p.nH, 1.0)));
   const double l0 = log10(o.t_start), l1 = log10(tmax);
   BdfStats st = {0, 0, 0, 0, 0};
+  Lin lin;
   int conv = 0;
   const int nk = o.n_checkpoints;
   for (int i = 0; i < nk - 1; ++i) {
     double ta = (i == 0) ? o.t_start : pow(10.0, l0 + (l1 - l0) * i / (nk - 1));
     double tb = (i + 1 == nk - 1) ? tmax : pow(10.0, l0 + (l1 - l0) * (i + 1) / (nk - 1));
     double yn = yc;
-    int rc = bdf_integrate(N, p, L, lane, yn, ta, tb, o.rtol, o.atol, tb - ta, o.max_steps, st);
-    if (rc != 0) continue;  // a failed span leaves the state unchanged (solver.py:216-217)
+    int status = bdf_integrate(N, p, L, lin, rc, lane, yn, ta, tb, o.rtol, o.atol, tb - ta,
+                               o.max_steps, st);
+    if (status != 0) continue;  // a failed span leaves the state unchanged (solver.py:216-217)
     yn = fmax(yn, 1e-30);
     if (o.early_stop) {
       double rel = -1.0;
@@ -723,10 +974,14 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
       if (s >= 0) rows[s].push_back({r, +1.0});
     }
   }
-  std::vector<int> rhs_off(ns + 1, 0), rhs_rxn;
+  std::vector<int> rhs_off(ns + 1, 0), rhs_rxn, rhs_pk;
   std::vector<double> rhs_sgn;
   for (int s = 0; s < ns; ++s) {
-    for (auto &e : rows[s]) { rhs_rxn.push_back(e.first); rhs_sgn.push_back(e.second); }
+    for (auto &e : rows[s]) {
+      rhs_rxn.push_back(e.first);
+      rhs_sgn.push_back(e.second);
+      rhs_pk.push_back((e.first << 1) | (e.second > 0.0 ? 1 : 0));
+    }
     rhs_off[s + 1] = (int)rhs_rxn.size();
   }
   // Jacobian structure: entry (i, j) collects stoich(i, r) * d rate_r / d y_j
@@ -780,7 +1035,8 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   size_t o_rt = put(rt.data(), 4 * nr), o_a = put(alpha, 8 * nr), o_b = put(beta, 8 * nr),
          o_g = put(gamma, 8 * nr), o_re = put(re.data(), 8 * nr), o_dep = put(dep.data(), 8 * nr),
          o_ro = put(rhs_off.data(), 4 * (ns + 1)), o_rr = put(rhs_rxn.data(), 4 * rhs_rxn.size()),
-         o_rs = put(rhs_sgn.data(), 8 * rhs_sgn.size()), o_jr = put(jrow.data(), 4 * nnz),
+         o_rs = put(rhs_sgn.data(), 8 * rhs_sgn.size()), o_pk = put(rhs_pk.data(), 4 * rhs_pk.size()),
+         o_jr = put(jrow.data(), 4 * nnz),
          o_jc = put(jcol.data(), 4 * nnz), o_jo = put(joff.data(), 4 * (nnz + 1)),
          o_tr = put(jt_rxn.data(), 4 * jt_rxn.size()), o_ts = put(jt_slot.data(), 4 * jt_slot.size()),
          o_tc = put(jt_coef.data(), 8 * jt_coef.size()), o_pw = put(post.data(), 8 * post.size());
@@ -792,7 +1048,7 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   P3D_CHECK_HIP(hipMemcpy(net->blob, host.data(), host.size(), hipMemcpyHostToDevice));
   char *b = static_cast<char *>(net->blob);
   NetDev &d = net->d;
-  d.ns = ns; d.nr = nr; d.e_idx = e_index; d.nnz = nnz;
+  d.ns = ns; d.nr = nr; d.e_idx = e_index; d.nnz = nnz; d.n_rhs_e = (int)rhs_pk.size();
   for (int k = 0; k < 7; ++k) d.roles[k] = roles[k];
   d.rtype = (const int *)(b + o_rt);
   d.alpha = (const double *)(b + o_a);
@@ -803,6 +1059,7 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   d.rhs_off = (const int *)(b + o_ro);
   d.rhs_rxn = (const int *)(b + o_rr);
   d.rhs_sgn = (const double *)(b + o_rs);
+  d.rhs_pk = (const int *)(b + o_pk);
   d.jrow = (const int *)(b + o_jr);
   d.jcol = (const int *)(b + o_jc);
   d.joff = (const int *)(b + o_jo);
@@ -842,7 +1099,7 @@ int p3d_chem_rhs(int64_t handle, int64_t n, const double *y, const double *n_h,
   int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
   if (rc != P3D_OK || n == 0) return rc;
   P3D_REQUIRE(dydt, "null output");
-  const size_t sh = lds_bytes(net->d.ns, net->d.nr);
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
   hipLaunchKernelGGL(chem_rhs_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream), net->d,
                      n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, dydt);
@@ -857,7 +1114,7 @@ int p3d_chem_jacobian(int64_t handle, int64_t n, const double *y, const double *
   int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
   if (rc != P3D_OK || n == 0) return rc;
   P3D_REQUIRE(jac, "null output");
-  const size_t sh = lds_bytes(net->d.ns, net->d.nr);
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
   hipLaunchKernelGGL(chem_jac_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream), net->d,
                      n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, jac);
@@ -883,11 +1140,25 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
     for (int k = 0; k < 7; ++k)
       P3D_REQUIRE(net->d.roles[k] >= 0,
                   "post-solve corrections / early stop need the species roles of the network");
-  const size_t sh = lds_bytes(net->d.ns, net->d.nr);
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
-  hipLaunchKernelGGL(chem_solve_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream),
-                     net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o, converged,
-                     stats);
+  P3D_REQUIRE(o.variant >= 0 && o.variant <= 3, "variant must be 0, 1, 2 or 3");
+  P3D_REQUIRE((o.variant != 2 && o.variant != 3) || net->d.ns <= 32,
+              "variants 2 and 3 need n_species <= 32");
+  int var = o.variant;
+  if (var == 0) var = net->d.ns <= 32 ? 2 : 1;
+  if (var == 3)
+    hipLaunchKernelGGL(chem_solve_kernel<LinLdsU<32>>, dim3((unsigned)n), dim3(64), sh,
+                       as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,
+                       o, converged, stats);
+  else if (var == 2)
+    hipLaunchKernelGGL(chem_solve_kernel<LinReg<32>>, dim3((unsigned)n), dim3(64), sh,
+                       as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,
+                       o, converged, stats);
+  else
+    hipLaunchKernelGGL(chem_solve_kernel<LinLds>, dim3((unsigned)n), dim3(64), sh,
+                       as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,
+                       o, converged, stats);
   return launch_status("chem_solve_kernel");
 }
 

@@ -41,8 +41,8 @@ class PDRSolverGPU:
 
     def __init__(self, density, box_size, G0_external=1.0, zeta_CR_0=2e-16, nside_rt=1,
                  fixed_T=None, chemistry='euler', x_c_total=GAS_C, x_o_total=GAS_O,
-                 av_per_nh=grid.AV_PER_NH, t_chem=1e7 * YR, chem_rtol=1e-9, chem_atol=1e-24,
-                 initial_abundances=None, T_init=50.0):
+                 av_per_nh=grid.AV_PER_NH, t_chem=1e7 * YR, chem_rtol=1e-9, chem_atol=1e-20,
+                 initial_abundances=None, T_init=50.0, chem_mode='steady', chem_variant=0):
         self.dev = device()
         dens = np.asarray(density, dtype=np.float64)
         self.shape = dens.shape
@@ -58,6 +58,17 @@ class PDRSolverGPU:
         self.x_c_total, self.x_o_total = float(x_c_total), float(x_o_total)
         self.av_per_nh = float(av_per_nh)
         self.t_chem, self.chem_rtol, self.chem_atol = float(t_chem), chem_rtol, chem_atol
+        # 'steady': every iteration continues each cell from its current
+        # abundances to the upstream steady-state horizon
+        # t_max = min(3e17, max(1e16, 3e17 / n_H)) s (chemistry/solver.py:150-154),
+        # as refine() does; 'integrate': every iteration restarts from the
+        # initial abundances and integrates to t_chem.
+        if chem_mode not in ('steady', 'integrate'):
+            raise ValueError("chem_mode must be 'steady' or 'integrate'")
+        self.chem_mode = chem_mode
+        self.chem_variant = int(chem_variant)
+        self.record_inputs = None  # cell indices whose per-iteration inputs are kept
+        self.recorded = []
         t = lambda v: torch.full(self.shape, float(v), dtype=torch.float64, device=self.dev)
         self.density = torch.tensor(dens, device=self.dev)
         g0i = float(np.max(G0_external)) if not np.isscalar(G0_external) else float(G0_external)
@@ -141,11 +152,23 @@ class PDRSolverGPU:
 
     def chemistry_step(self):
         if self.chemistry == 'network':
-            # integrate every cell from the initial abundances to t_chem at the
-            # current temperature and radiation field
-            self.y.copy_(self.y_init)
-            o = solve_opts(t_max=self.t_chem, t_start=1e4, n_checkpoints=2, early_stop=False,
-                           rtol=self.chem_rtol, atol=self.chem_atol, post_mode=-1)
+            if self.chem_mode == 'integrate':
+                self.y.copy_(self.y_init)
+                o = solve_opts(t_max=self.t_chem, t_start=1e4, n_checkpoints=2,
+                               early_stop=False, rtol=self.chem_rtol, atol=self.chem_atol,
+                               post_mode=-1, variant=self.chem_variant)
+            else:
+                o = solve_opts(t_max=None, t_start=1e4, n_checkpoints=2, early_stop=False,
+                               rtol=self.chem_rtol, atol=self.chem_atol, post_mode=-1,
+                               variant=self.chem_variant)
+            if self.record_inputs is not None:
+                idx = self.record_inputs
+                flat = lambda a: a.reshape(-1)[idx].cpu().numpy()
+                self.recorded.append(dict(
+                    y=self.y[:, idx].cpu().numpy(), n_H=flat(self.density), T=flat(self.T_gas),
+                    G0=flat(self.G0), A_V=flat(self.A_V), zeta=flat(self.zeta_CR),
+                    fsh_H2=flat(self.fsh_H2), fsh_CO=flat(self.fsh_CO),
+                    t_max=o.t_max))
             if self.record_chem:
                 ev0 = torch.cuda.Event(enable_timing=True)
                 ev1 = torch.cuda.Event(enable_timing=True)
