@@ -107,6 +107,9 @@ def main():
     ap.add_argument("--max-iter", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="chemistry kernel variant (0 auto)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this size (a known-traffic "
+                         "kernel that calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the per-cell "
@@ -115,6 +118,13 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calib_copy_mib > 0:
+        nb = args.calib_copy_mib * 2 ** 20 // 8
+        a = torch.ones(nb, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     solver = make_solver(args.n, args.variant)
     print(f"[bench] {args.n}^3 = {solver.n_cells} cells; warmup {args.warmup}", file=sys.stderr,
           flush=True)
