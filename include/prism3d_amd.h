@@ -139,6 +139,13 @@ int p3d_chem_jacobian(int64_t handle, int64_t n, const double *y,
                       const double *fsh_h2, const double *fsh_co, double *jac,
                       void *stream);
 
+/* Post-solve corrections alone on y (n_species, n): steady-state H+ and e-
+ * (solver.py:227-281) then conservation (solver.py:345-388); post_mode 0 =
+ * upstream clipped scaling, 1 = exact projection onto the element totals. */
+int p3d_chem_post(int64_t handle, int64_t n, double *y, const double *n_h,
+                  const double *t_gas, const double *zeta, int32_t post_mode,
+                  double x_c_total, double x_o_total, void *stream);
+
 /* Options of the per-cell implicit solve. */
 typedef struct p3d_solve_opts {
   double t_max;        /* end time [s]; <= 0: min(3e17, max(1e16, 3e17/n_H)) */
@@ -179,7 +186,7 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h,
  * columns N_H, N_H2, N_CO are summed along the ray's dominant axis from its
  * entry face (half-cell offset), A_V = N_H * av_per_nh, and
  * G0 = sum_r g0_rays[r] / n_rays * exp(-1.8 A_V); the minimum-A_V ray's
- * columns are kept.  workspace: device buffer of 9 * nx*ny*nz doubles.
+ * columns are kept.  workspace: device buffer of 18 * nx*ny*nz doubles.
  * Replaces radiative_transfer/fuv_rt_3d.py:215-392. */
 int p3d_fuv_columns(int64_t nx, int64_t ny, int64_t nz, const double *density,
                     const double *x_h2, const double *x_co, double cell_size,

@@ -44,6 +44,7 @@ SIGNATURES = {
     "p3d_network_destroy": (_i32, [_i64]),
     "p3d_chem_rhs": (_i32, [_i64, _i64] + [_VP] * 9 + [_VP]),
     "p3d_chem_jacobian": (_i32, [_i64, _i64] + [_VP] * 9 + [_VP]),
+    "p3d_chem_post": (_i32, [_i64, _i64, _VP, _VP, _VP, _VP, _i32, _dbl, _dbl, _VP]),
     "p3d_chem_solve": (_i32, [_i64, _i64] + [_VP] * 8 + [ctypes.POINTER(SolveOpts), _VP, _VP,
                                                         _VP]),
     "p3d_fuv_columns": (_i32, [_i64, _i64, _i64, _VP, _VP, _VP, _dbl, _i32, _D, _D, _dbl]

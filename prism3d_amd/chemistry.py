@@ -158,6 +158,19 @@ class ChemistrySolver:
             torch.cuda.synchronize(y.device)
         return conv, stats
 
+    def post_batch(self, y, n_H, T, zeta_CR, mode=0, x_c_total=GAS_C, x_o_total=GAS_O):
+        """H+/e- correction and conservation on every cell of y, in place
+        (solver.py:227-388); mode 1 projects exactly onto the totals."""
+        self._check_state(y)
+        n = y.shape[1]
+        p = _cell_arrays(n, n_H, T, zeta_CR)
+        rc = _lib.lib().p3d_chem_post(self.dnet.handle, n, y.data_ptr(),
+                                      *[t.data_ptr() for t in p], int(mode), float(x_c_total),
+                                      float(x_o_total), stream_handle())
+        _lib.check(rc, "p3d_chem_post")
+        torch.cuda.synchronize(y.device)
+        return y
+
     def rhs_batch(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0, f_shield_CO=1.0):
         self._check_state(y)
         n = y.shape[1]

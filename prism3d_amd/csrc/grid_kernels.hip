@@ -17,63 +17,73 @@ namespace p3d {
 
 // ===========================================================================
 // Columns.  Along a ray whose dominant axis is `ax`, every cell's column is
-// ds * (sum of the layer values from the entry face up to and including the
-// cell, minus half the cell's own layer), with ds = cell / max(|d_ax|, 0.01).
-// The sum from the low face is the inclusive prefix F(i); from the high face
-// it is Tot - F(i) + l(i).  So one prefix pass per axis serves every ray.
+// ds * (the layer values summed from the entry face up to and including the
+// cell, minus half the cell's own layer), ds = cell / max(|d_ax|, 0.01).
+// The sums from the low face (lo) and from the high face (hi) are computed
+// once per axis and shared by every ray with that dominant axis; both run
+// sequentially in the same order as a cumulative sum from that face.
+// Workspace: F[axis][face][quantity][cell], 18 n doubles.
 // ===========================================================================
 
-// Inclusive prefix sums along axis 0 or 1 (stride > 1): one thread per line,
+// Prefix sums along axis 0 or 1 (stride > 1): one thread per line and face,
 // consecutive threads on consecutive inner indices (coalesced).
 __global__ void __launch_bounds__(256)
 prefix_strided(int64_t nx, int64_t ny, int64_t nz, int axis, const double *__restrict__ dens,
                const double *__restrict__ xh2, const double *__restrict__ xco,
-               double *__restrict__ F /* 3 quantities x n */) {
+               double *__restrict__ F /* 2 faces x 3 quantities x n */) {
   const int64_t n = nx * ny * nz;
-  int64_t line = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nlines = axis == 0 ? ny * nz : nx * nz;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * nlines) return;
+  const int face = t >= nlines ? 1 : 0;
+  const int64_t line = t - face * nlines;
   int64_t len, stride, base;
   if (axis == 0) {
-    if (line >= ny * nz) return;
     len = nx; stride = ny * nz; base = line;
   } else {
-    if (line >= nx * nz) return;
-    int64_t ix = line / nz, iz = line - ix * nz;
+    const int64_t ix = line / nz, iz = line - ix * nz;
     len = ny; stride = nz; base = ix * ny * nz + iz;
   }
+  double *Fq = F + (int64_t)face * 3 * n;
   double a = 0.0, b = 0.0, c = 0.0;
   for (int64_t k = 0; k < len; ++k) {
-    const int64_t i = base + k * stride;
+    const int64_t i = base + (face ? (len - 1 - k) : k) * stride;
     const double d = dens[i];
     a += d;
     b += d * xh2[i] * 2.0;
     c += d * xco[i];
-    F[i] = a;
-    F[n + i] = b;
-    F[2 * n + i] = c;
+    Fq[i] = a;
+    Fq[n + i] = b;
+    Fq[2 * n + i] = c;
   }
 }
 
-// Inclusive prefix sums along the contiguous axis 2: one wavefront per line,
+// Prefix sums along the contiguous axis 2: one wavefront per line and face,
 // 64 cells per chunk read coalesced, scanned with shuffles, carried across
-// chunks.
+// chunks (the within-chunk association differs from a strictly sequential
+// sum by rounding only).
 __global__ void __launch_bounds__(256)
 prefix_contig(int64_t nx, int64_t ny, int64_t nz, const double *__restrict__ dens,
               const double *__restrict__ xh2, const double *__restrict__ xco,
               double *__restrict__ F) {
   const int64_t n = nx * ny * nz;
   const int lane = threadIdx.x & 63;
-  const int64_t line = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (line >= nx * ny) return;
-  const int64_t base = line * nz;
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nlines = nx * ny;
+  if (w >= 2 * nlines) return;
+  const int face = w >= nlines ? 1 : 0;
+  const int64_t base = (w - face * nlines) * nz;
+  double *Fq = F + (int64_t)face * 3 * n;
   double ca = 0.0, cb = 0.0, cc = 0.0;
   for (int64_t k0 = 0; k0 < nz; k0 += 64) {
     const int64_t k = k0 + lane;
+    const int64_t i = base + (face ? (nz - 1 - k) : k);
     double a = 0.0, b = 0.0, c = 0.0;
     if (k < nz) {
-      const double d = dens[base + k];
+      const double d = dens[i];
       a = d;
-      b = d * xh2[base + k] * 2.0;
-      c = d * xco[base + k];
+      b = d * xh2[i] * 2.0;
+      c = d * xco[i];
     }
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -82,9 +92,9 @@ prefix_contig(int64_t nx, int64_t ny, int64_t nz, const double *__restrict__ den
     }
     a += ca; b += cb; c += cc;
     if (k < nz) {
-      F[base + k] = a;
-      F[n + base + k] = b;
-      F[2 * n + base + k] = c;
+      Fq[i] = a;
+      Fq[n + i] = b;
+      Fq[2 * n + i] = c;
     }
     ca = bcast(a, 63); cb = bcast(b, 63); cc = bcast(c, 63);
   }
@@ -102,44 +112,37 @@ struct RaySet {  // passed by value (kernel argument, < 4 KiB)
 __global__ void __launch_bounds__(256)
 ray_combine(int64_t nx, int64_t ny, int64_t nz, const double *__restrict__ dens,
             const double *__restrict__ xh2, const double *__restrict__ xco,
-            const double *__restrict__ F0, const double *__restrict__ F1,
-            const double *__restrict__ F2, const RaySet R, double av_per_nh,
+            const double *__restrict__ F, const RaySet R, double av_per_nh,
             double *__restrict__ g0, double *__restrict__ av, double *__restrict__ nh,
             double *__restrict__ nh2, double *__restrict__ nco) {
   const int64_t n = nx * ny * nz;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int64_t ix = i / (ny * nz), iy = (i / nz) % ny, iz = i % nz;
   const double d = dens[i];
   const double l[3] = {d, d * xh2[i] * 2.0, d * xco[i]};
-  // column sums (before the ds factor) from the low and high faces per axis
-  double lo[3][3], hi[3][3];
-  const double *Fa[3] = {F0, F1, F2};
-  const int64_t last[3] = {i + (nx - 1 - ix) * ny * nz, i + (ny - 1 - iy) * nz, i + (nz - 1 - iz)};
+  // half-cell-corrected sums per (axis, face, quantity)
+  double S[3][2][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const double f = Fa[a][q * n + i], tot = Fa[a][q * n + last[a]];
-      lo[a][q] = f - 0.5 * l[q];
-      hi[a][q] = (tot - f + l[q]) - 0.5 * l[q];
-    }
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) S[a][f][q] = F[((a * 2 + f) * 3 + q) * n + i] - 0.5 * l[q];
   double G = 0.0, avm = 1e10, bh = 0.0, bh2 = 0.0, bco = 0.0;
   const int nr = R.n;
   for (int r = 0; r < nr; ++r) {
     const int a = R.ax[r];
-    const bool low = R.from_low[r] != 0;
+    const int f = R.from_low[r] ? 0 : 1;
     const double ds = R.ds[r];
-    const double s0 = low ? (a == 0 ? lo[0][0] : a == 1 ? lo[1][0] : lo[2][0])
-                          : (a == 0 ? hi[0][0] : a == 1 ? hi[1][0] : hi[2][0]);
+    double s0, s1, s2;
+    // select without dynamic register indexing
+    if (a == 0) { s0 = f ? S[0][1][0] : S[0][0][0]; s1 = f ? S[0][1][1] : S[0][0][1]; s2 = f ? S[0][1][2] : S[0][0][2]; }
+    else if (a == 1) { s0 = f ? S[1][1][0] : S[1][0][0]; s1 = f ? S[1][1][1] : S[1][0][1]; s2 = f ? S[1][1][2] : S[1][0][2]; }
+    else { s0 = f ? S[2][1][0] : S[2][0][0]; s1 = f ? S[2][1][1] : S[2][0][1]; s2 = f ? S[2][1][2] : S[2][0][2]; }
     const double cnh = fmax(ds * s0, 0.0);
     const double cav = cnh * av_per_nh;
     G += R.g0w[r] * exp(-1.8 * cav);
     if (cav < avm) {
-      const double s1 = low ? (a == 0 ? lo[0][1] : a == 1 ? lo[1][1] : lo[2][1])
-                            : (a == 0 ? hi[0][1] : a == 1 ? hi[1][1] : hi[2][1]);
-      const double s2 = low ? (a == 0 ? lo[0][2] : a == 1 ? lo[1][2] : lo[2][2])
-                            : (a == 0 ? hi[0][2] : a == 1 ? hi[1][2] : hi[2][2]);
       avm = cav;
       bh = cnh;
       bh2 = fmax(ds * s1, 0.0);
@@ -398,15 +401,14 @@ int p3d_fuv_columns(int64_t nx, int64_t ny, int64_t nz, const double *density,
     R.g0w[r] = g0_rays[r] * ((4.0 * kPi / n_rays) / (4.0 * kPi));
   }
   hipStream_t s = as_stream(stream);
-  double *F0 = workspace, *F1 = workspace + 3 * n, *F2 = workspace + 6 * n;
-  hipLaunchKernelGGL(prefix_strided, dim3(grid_for(ny * nz, 256)), dim3(256), 0, s, nx, ny, nz, 0,
-                     density, x_h2, x_co, F0);
-  hipLaunchKernelGGL(prefix_strided, dim3(grid_for(nx * nz, 256)), dim3(256), 0, s, nx, ny, nz, 1,
-                     density, x_h2, x_co, F1);
-  hipLaunchKernelGGL(prefix_contig, dim3(grid_for(nx * ny * 64, 256)), dim3(256), 0, s, nx, ny,
-                     nz, density, x_h2, x_co, F2);
+  hipLaunchKernelGGL(prefix_strided, dim3(grid_for(2 * ny * nz, 256)), dim3(256), 0, s, nx, ny,
+                     nz, 0, density, x_h2, x_co, workspace);
+  hipLaunchKernelGGL(prefix_strided, dim3(grid_for(2 * nx * nz, 256)), dim3(256), 0, s, nx, ny,
+                     nz, 1, density, x_h2, x_co, workspace + 6 * n);
+  hipLaunchKernelGGL(prefix_contig, dim3(grid_for(2 * nx * ny * 64, 256)), dim3(256), 0, s, nx,
+                     ny, nz, density, x_h2, x_co, workspace + 12 * n);
   hipLaunchKernelGGL(ray_combine, dim3(grid_for(n, 256)), dim3(256), 0, s, nx, ny, nz, density,
-                     x_h2, x_co, F0, F1, F2, R, av_per_nh, g0, a_v, n_h, n_h2, n_co);
+                     x_h2, x_co, workspace, R, av_per_nh, g0, a_v, n_h, n_h2, n_co);
   return launch_status("fuv_columns");
 }
 

@@ -877,6 +877,20 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
 }
 
 __global__ void __launch_bounds__(64)
+chem_post_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
+                 const double *__restrict__ T, const double *__restrict__ zeta, int mode,
+                 double x_c_total, double x_o_total) {
+  const int64_t cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  CellParams p;
+  p.nH = nH[cell]; p.T = T[cell]; p.zeta = zeta[cell];
+  p.G0 = 0.0; p.AV = 0.0; p.fH2 = 1.0; p.fCO = 1.0;
+  double yc = lane < N.ns ? y[(int64_t)lane * n + cell] : 0.0;
+  post_solve(N, p, lane, yc, mode, x_c_total, x_o_total);
+  if (lane < N.ns) y[(int64_t)lane * n + cell] = yc;
+}
+
+__global__ void __launch_bounds__(64)
 chem_rhs_kernel(NetDev N, int64_t n, const double *__restrict__ y, const double *__restrict__ nH,
                 const double *__restrict__ T, const double *__restrict__ G0,
                 const double *__restrict__ AV, const double *__restrict__ zeta,
@@ -1119,6 +1133,22 @@ int p3d_chem_jacobian(int64_t handle, int64_t n, const double *y, const double *
   hipLaunchKernelGGL(chem_jac_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream), net->d,
                      n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, jac);
   return launch_status("chem_jac_kernel");
+}
+
+int p3d_chem_post(int64_t handle, int64_t n, double *y, const double *n_h,
+                  const double *t_gas, const double *zeta, int32_t post_mode, double x_c_total,
+                  double x_o_total, void *stream) {
+  Network *net = find_net(handle);
+  P3D_REQUIRE(net, "unknown network handle");
+  P3D_REQUIRE(n >= 0, "n must be >= 0");
+  if (n == 0) return P3D_OK;
+  P3D_REQUIRE(y && n_h && t_gas && zeta, "null device array");
+  P3D_REQUIRE(post_mode == 0 || post_mode == 1, "post_mode must be 0 or 1");
+  for (int k = 0; k < 7; ++k)
+    P3D_REQUIRE(net->d.roles[k] >= 0, "the network has no species roles for the corrections");
+  hipLaunchKernelGGL(chem_post_kernel, dim3((unsigned)n), dim3(64), 0, as_stream(stream), net->d,
+                     n, y, n_h, t_gas, zeta, (int)post_mode, x_c_total, x_o_total);
+  return launch_status("chem_post_kernel");
 }
 
 int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, const double *t_gas,

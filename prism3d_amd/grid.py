@@ -77,7 +77,7 @@ def fuv_columns(density, x_h2, x_co, cell_size, nside=1, G0_external=1.0,
     dirs = np.ascontiguousarray(dirs.reshape(-1))
     g0r = np.ascontiguousarray(g0r[:nr])
     if workspace is None:
-        workspace = torch.empty(9 * density.numel(), dtype=torch.float64, device=density.device)
+        workspace = torch.empty(18 * density.numel(), dtype=torch.float64, device=density.device)
     outs = _empty(density, 5)
     rc = _lib.lib().p3d_fuv_columns(nx, ny, nz, density.data_ptr(), x_h2.data_ptr(),
                                     x_co.data_ptr(), float(cell_size), nr, _lib.dptr(dirs),

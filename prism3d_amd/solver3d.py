@@ -99,7 +99,7 @@ class PDRSolverGPU:
         self._pe_table = torch.tensor(self.pe.table, device=self.dev)
         self._pe_bins = [torch.tensor(a, device=self.dev) for a in
                          (self.pe.sigma_abs, self.pe.dn_da_da, self.pe.is_nano)]
-        self._ws = torch.empty(9 * self.n_cells, dtype=torch.float64, device=self.dev)
+        self._ws = torch.empty(18 * self.n_cells, dtype=torch.float64, device=self.dev)
         self._tmean = torch.empty(1, dtype=torch.float64, device=self.dev)
         self.chem = ChemistrySolver() if chemistry == 'network' else None
         self.chem_stats = None

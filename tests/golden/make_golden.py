@@ -371,6 +371,16 @@ def gen_thermal_euler_pe():
     save("diag_golden.npz", T=sv2.T_gas.ravel(), Gamma=sv2.Gamma.ravel(),
          Lambda=sv2.Lambda.ravel())
 
+    # --- the fixture state itself (inputs of the context, rates, dust and
+    # diagnostics kernels)
+    save("state8_golden.npz", density=base.density.ravel(), cell=base.cell_size,
+         **{k: np.asarray(getattr(base, k)).ravel() for k in
+            ['G0', 'A_V', 'N_H', 'N_H2', 'N_CO', 'T_gas', 'T_dust', 'Gamma_PE', '_R_H2',
+             'zeta_CR', 'f_nano', 'E_g', 'x_HI', 'x_H2', 'x_Cp', 'x_C', 'x_CO', 'x_O',
+             'x_e', 'x_OH', 'x_H2O', 'x_HCOp']},
+         fsh_H2=np.asarray(f_sh_H2).ravel(), fsh_CO=np.asarray(f_sh_CO).ravel(),
+         T_mean=float(np.mean(base.T_gas)))
+
 
 def gen_rt():
     print("radiative transfer / shielding / cosmic rays")

@@ -276,3 +276,36 @@ def test_run_trace_inputs():
     r = golden("run_golden.npz")
     d, _ = O.fractal_turbulent(8, 2.0 * O.PC, n_mean=500.0, sigma_ln=1.5, seed=42)
     assert np.array_equal(d.ravel(), r['density'].ravel())
+
+
+def test_thermal_context_and_diagnostics_match_upstream():
+    s = golden("state8_golden.npz")
+    t = golden("thermal_golden.npz")
+    d = golden("diag_golden.npz")
+    ctx = O.thermal_context(s['density'], s['G0'], s['A_V'], s['fsh_H2'], s['zeta_CR'],
+                            s['T_dust'], s['Gamma_PE'], s['_R_H2'], s['x_HI'], s['x_H2'],
+                            s['x_e'], s['x_CO'], s['x_Cp'], s['x_O'], s['x_C'])
+    for k, name in enumerate(THERMAL_KEYS):
+        assert np.array_equal(ctx[k], t[name]), name
+    G, L = O.heating_cooling(d['T'], s['density'], s['G0'], s['A_V'], s['zeta_CR'], s['fsh_H2'],
+                             s['x_e'], s['x_HI'], s['x_H2'], s['x_C'], s['x_Cp'], s['x_O'],
+                             s['x_CO'], s['T_dust'], s['Gamma_PE'], s['_R_H2'])
+    assert np.allclose(G, d['Gamma'], rtol=1e-14, atol=0)
+    assert np.allclose(L, d['Lambda'], rtol=1e-14, atol=0)
+
+
+def test_run_loop_against_upstream_trace():
+    """The restated inner loop reproduces the upstream run() exactly through
+    the first iteration (RT, analytic IC, shielding, CR, dust, chemistry,
+    thermal); later iterations differ only by the explicit-chemistry exit
+    rule (the native core exits per cell, the numpy fallback on the grid
+    maximum), so they are compared in distribution."""
+    r = golden("run_golden.npz")
+    st, hist = O.run_loop(r['density'].reshape(8, 8, 8), float(r['box']), 100.0, 2e-16, 1, 4)
+    assert hist[0]['dT'] == pytest.approx(r['dT'][0], rel=1e-12)
+    assert hist[0]['dH2'] == pytest.approx(r['dH2'][0], rel=1e-12)
+    for k in ('G0', 'A_V'):  # the last iteration's columns use the previous abundances
+        assert np.allclose(st[k].ravel(), r[k], rtol=5e-2)
+    for k in ('T_gas', 'x_H2', 'x_CO'):
+        rel = np.abs(st[k].ravel() - r[k]) / np.abs(r[k])
+        assert np.median(rel) < 0.02, k
