@@ -160,8 +160,12 @@ typedef struct p3d_solve_opts {
   double x_c_total, x_o_total; /* element totals for the conservation step */
   int32_t variant;     /* kernel variant: 0 auto, 1 LU in LDS, 2 LU in registers
                           (n_species <= 32), 3 LU in LDS with unrolled loops
-                          (n_species <= 32) */
-  int32_t reserved;
+                          (n_species <= 32): one cell per 64-lane workgroup;
+                          4 persistent one-cell-per-lane kernel with a device
+                          work queue (the bundled network structure only) */
+  int32_t waves_per_cu; /* variant 4: resident wavefronts per compute unit
+                          (0: library default 8); the kernel is compiled for
+                          waves_per_cu / 4 wavefronts per SIMD (2..4) */
 } p3d_solve_opts;
 
 /* Integrate every cell to its steady state / end time with the

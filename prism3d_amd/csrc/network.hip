@@ -18,12 +18,14 @@
 //   integrator             scipy solve_ivp(method='BDF') as called at
 //                          solver.py:195-201: the quasi-constant step size NDF
 //                          of Shampine & Reichelt (1997), orders 1-5.
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <vector>
 
+#include "lane_launch.h"
 #include "p3d_common.h"
 
 namespace p3d {
@@ -52,8 +54,14 @@ struct NetDev {
 struct Network {
   NetDev d;
   void *blob = nullptr;  // single device allocation holding all tables
+  bool lane_ok = false;  // structure matches the generated one-cell-per-lane code
+  // slot memory of the persistent lane kernel (grown on demand) and its
+  // work counter; calls on one handle share it, so they run in stream order
+  void *lane_ws = nullptr;
+  size_t lane_ws_bytes = 0;
   ~Network() {
     if (blob) (void)hipFree(blob);
+    if (lane_ws) (void)hipFree(lane_ws);
   }
 };
 
@@ -675,7 +683,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
       }
       if (!converged) {
         h *= 0.5;
-        if (h < hmin) return -1;
+        if (!(h >= hmin)) return -1;  // also ends the span on a NaN step size
         change_D(L, lane, D, order, 0.5);
         n_equal = 0;
         lu_valid = false;
@@ -688,6 +696,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
       if (en > 1.0) {
         double factor = fmax(0.2, 0.9 * pow(en, -1.0 / (order + 1)));
         h *= factor;
+        if (!(h > 0.0)) return -1;  // step size underflow or NaN
         change_D(L, lane, D, order, factor);
         n_equal = 0;
         lu_valid = false;
@@ -933,6 +942,48 @@ chem_jac_kernel(NetDev N, int64_t n, const double *__restrict__ y, const double 
   }
 }
 
+// Persistent one-cell-per-lane solve (variant 4): sizes the resident grid,
+// grows the handle's slot memory and launches.
+static int lane_solve(Network *net, int64_t n, double *y, const double *n_h, const double *t_gas,
+                      const double *g0, const double *a_v, const double *zeta,
+                      const double *fsh_h2, const double *fsh_co, const p3d_solve_opts &o,
+                      int32_t *converged, int32_t *stats, void *stream) {
+  int dev = 0, cus = 0;
+  P3D_CHECK_HIP(hipGetDevice(&dev));
+  P3D_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 8;
+  int64_t n_waves = std::min<int64_t>((n + 63) / 64, (int64_t)cus * wpc);
+  // slot memory: one 64-lane block per resident wavefront (bounded to 16 GiB)
+  const int64_t max_waves = ((int64_t)16 << 30) / (64 * (int64_t)lane_slot_doubles() * 8);
+  n_waves = std::max<int64_t>(1, std::min(n_waves, max_waves));
+  const size_t slot_bytes = (size_t)n_waves * 64 * lane_slot_doubles() * sizeof(double);
+  const size_t need = slot_bytes + 256;
+  if (net->lane_ws_bytes < need) {
+    if (net->lane_ws) {
+      P3D_CHECK_HIP(hipStreamSynchronize(as_stream(stream)));
+      P3D_CHECK_HIP(hipFree(net->lane_ws));
+      net->lane_ws = nullptr;
+      net->lane_ws_bytes = 0;
+    }
+    if (hipMalloc(&net->lane_ws, need) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("hipMalloc of %zu bytes of lane slot memory failed", need);
+      return P3D_ENOMEM;
+    }
+    net->lane_ws_bytes = need;
+  }
+  LaneArgs a;
+  a.n = n; a.y = y; a.nH = n_h; a.T = t_gas; a.G0 = g0; a.AV = a_v; a.zeta = zeta;
+  a.fH2 = fsh_h2; a.fCO = fsh_co; a.conv = converged; a.stats = stats;
+  a.rtype = net->d.rtype; a.alpha = net->d.alpha; a.beta = net->d.beta; a.gamma = net->d.gamma;
+  a.post_w = net->d.post_w; a.opts = o; a.stream = stream;
+  double *ws = static_cast<double *>(net->lane_ws);
+  auto *ctr = reinterpret_cast<unsigned long long *>(static_cast<char *>(net->lane_ws) + slot_bytes);
+  // wavefronts per SIMD the kernel is compiled for: 4 SIMDs per CU
+  const int occ = std::min(4, std::max(2, (wpc + 3) / 4));
+  return lane_launch(a, n_waves, occ, ws, ctr);
+}
+
 }  // namespace p3d
 
 using namespace p3d;
@@ -1081,6 +1132,8 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   d.jt_slot = (const int *)(b + o_ts);
   d.jt_coef = (const double *)(b + o_tc);
   d.post_w = (const double *)(b + o_pw);
+  net->lane_ok = roles7 != nullptr &&
+                 lane_network_matches(ns, nr, rtype, reactants, products, e_index, roles);
   std::lock_guard<std::mutex> lk(g_net_mu);
   int64_t h = g_next_handle++;
   g_nets[h] = std::move(net);
@@ -1170,11 +1223,17 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
     for (int k = 0; k < 7; ++k)
       P3D_REQUIRE(net->d.roles[k] >= 0,
                   "post-solve corrections / early stop need the species roles of the network");
-  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
-  P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
-  P3D_REQUIRE(o.variant >= 0 && o.variant <= 3, "variant must be 0, 1, 2 or 3");
+  P3D_REQUIRE(o.variant >= 0 && o.variant <= 4, "variant must be 0 .. 4");
   P3D_REQUIRE((o.variant != 2 && o.variant != 3) || net->d.ns <= 32,
               "variants 2 and 3 need n_species <= 32");
+  P3D_REQUIRE(o.variant != 4 || net->lane_ok,
+              "variant 4 (one cell per lane) is generated for the bundled network structure; "
+              "this network differs");
+  P3D_REQUIRE(o.waves_per_cu >= 0 && o.waves_per_cu <= 64, "waves_per_cu must be in [0, 64]");
+  if (o.variant == 4) return lane_solve(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o,
+                                        converged, stats, stream);
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
+  P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
   int var = o.variant;
   if (var == 0) var = net->d.ns <= 32 ? 2 : 1;
   if (var == 3)

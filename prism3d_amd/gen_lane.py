@@ -1,0 +1,340 @@
+"""Generate the straight-line network code of the one-cell-per-lane solver.
+
+    python -m prism3d_amd.gen_lane            # writes prism3d_amd/csrc/lane_net.h
+
+The lane solver keeps one cell per GPU lane, so every per-species array of a
+cell must be indexed with compile-time constants (registers cannot be
+indexed per lane).  This script turns the network structure (species,
+reactants, products; prism3d_amd/network.py) into plain C++ for that:
+
+  * rhs:       reaction rates and dy/dt, accumulated per species in the
+               upstream order (reaction by reaction, reactant occurrences then
+               product occurrences: chemistry/solver.py:104-143);
+  * jacobian:  the analytic Jacobian, one row at a time (row entries in
+               registers, then stored), every entry summed over its reaction
+               terms in reaction order;
+  * lu_factor: M = I - c J factorised without pivoting in a fill-reducing
+               order (greedy minimum Markowitz product on the structural
+               pattern), row by row (left-looking), storing the unit-lower
+               multipliers, the upper factor and the reciprocal diagonal;
+  * lu_solve:  the two triangular solves in that order.
+
+Rate coefficients stay table driven (any alpha/beta/gamma of the same
+structure), so the code depends only on which species each reaction reads and
+writes.  The emitted header is included by csrc/lane_bdf.h.
+"""
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "csrc", "lane_net.h")
+
+# reaction types (include/prism3d_amd.h)
+TWO_BODY, PHOTO, PHOTO_FH2, PHOTO_FCO, CR, GRAIN = 0, 1, 2, 3, 4, 5
+PAH_MINUS, NONE = -2, -1
+
+# compiler-fence spacing in the emitted code (0: none): bounds how many slot
+# loads the compiler may keep in flight at once, i.e. register pressure
+# against memory-latency overlap
+RHS_FENCE_EVERY = 8
+JAC_FENCE_EVERY = 0
+LU_PIVOT_FENCE = False
+
+
+def structure(net):
+    """Dependency species per rate slot, net stoichiometry per reaction and
+    the Jacobian term lists, built the way the library builds them for a
+    network handle."""
+    ns, nr = net.n_species, net.n_reactions
+    rt, re, pr, e = net.rtype, net.reactants, net.products, net.e_index
+    dep = np.full(2 * nr, -1, dtype=np.int64)
+    for r in range(nr):
+        a, b = re[2 * r], re[2 * r + 1]
+        dep[2 * r] = a
+        if rt[r] == TWO_BODY:
+            if b == PAH_MINUS:
+                dep[2 * r + 1] = e
+            elif b != a:
+                dep[2 * r + 1] = b
+    nets = []
+    for r in range(nr):
+        slots = 2 if rt[r] in (TWO_BODY, GRAIN) else 1
+        m = {}
+        for k in range(slots):
+            s = re[2 * r + k]
+            if s >= 0:
+                m[s] = m.get(s, 0.0) - 1.0
+        for k in range(3):
+            s = pr[3 * r + k]
+            if s >= 0:
+                m[s] = m.get(s, 0.0) + 1.0
+        nets.append(sorted((s, v) for s, v in m.items()))
+    terms = {}  # (i, j) -> [(r, slot, coef)] in reaction order
+    for r in range(nr):
+        for sl in range(2):
+            j = dep[2 * r + sl]
+            if j < 0:
+                continue
+            for i, v in nets[r]:
+                if v != 0.0:
+                    terms.setdefault((int(i), int(j)), []).append((r, sl, v))
+    return dep, nets, terms
+
+
+def markowitz(pattern, n):
+    """Greedy minimum Markowitz product ordering (lowest index on ties)."""
+    A = pattern.copy()
+    left = list(range(n))
+    order = []
+    while left:
+        best = None
+        for k in left:
+            rc = sum(1 for j in left if j != k and A[k, j])
+            cc = sum(1 for i in left if i != k and A[i, k])
+            if best is None or rc * cc < best[0]:
+                best = (rc * cc, k)
+        k = best[1]
+        order.append(k)
+        left.remove(k)
+        rows = [i for i in left if A[i, k]]
+        cols = [j for j in left if A[k, j]]
+        for i in rows:
+            for j in cols:
+                A[i, j] = True
+    return order
+
+
+def fill(pattern_pos, n):
+    """Filled pattern of the no-pivot LU in position order."""
+    A = pattern_pos.copy()
+    for k in range(n):
+        rows = [i for i in range(k + 1, n) if A[i, k]]
+        cols = [j for j in range(k + 1, n) if A[k, j]]
+        for i in rows:
+            for j in cols:
+                A[i, j] = True
+    return A
+
+
+def generate(net=None):
+    from .network import Network
+    net = net or Network()
+    ns, nr, e = net.n_species, net.n_reactions, net.e_index
+    rt, re, pr = net.rtype, net.reactants, net.products
+    dep, nets, terms = structure(net)
+    pat = np.zeros((ns, ns), bool)
+    for (i, j) in terms:
+        pat[i, j] = True
+    # every row needs its diagonal for the no-pivot factorisation
+    for i in range(ns):
+        if not pat[i, i]:
+            raise ValueError(f"species {net.species[i]} has no diagonal Jacobian entry")
+    perm = markowitz(pat, ns)               # position -> species
+    pos = {s: p for p, s in enumerate(perm)}
+    ppat = pat[np.ix_(perm, perm)]
+    F = fill(ppat, ns)
+    # storage: J entries in position-major order of the structural pattern,
+    # LU entries in position-major order of the filled pattern
+    jz = {}
+    for pi in range(ns):
+        for pj in range(ns):
+            if ppat[pi, pj]:
+                jz[(pi, pj)] = len(jz)
+    lz = {}
+    for pi in range(ns):
+        for pj in range(ns):
+            if F[pi, pj]:
+                lz[(pi, pj)] = len(lz)
+    NJ, NLU = len(jz), len(lz)
+    L = []
+    w = L.append
+    w("// Generated by prism3d_amd/gen_lane.py from the network structure of")
+    w("// prism3d_amd/network.py.  Do not edit; rerun the generator.")
+    w("//")
+    w(f"// {ns} species, {nr} reactions, {NJ} structural Jacobian entries,")
+    w(f"// {NLU} entries in the filled no-pivot LU factors.")
+    w("#pragma once")
+    w("")
+    w("namespace p3d {")
+    w("namespace lane {")
+    w("")
+    w(f"constexpr int kNS = {ns}, kNR = {nr}, kNJ = {NJ}, kNLU = {NLU}, kE = {e};")
+    w("constexpr int kRoles[7] = {" + ", ".join(str(int(x)) for x in net.roles) + "};")
+    w("constexpr int kRtype[kNR] = {" + ", ".join(str(int(x)) for x in rt) + "};")
+    w("constexpr int kReact[2 * kNR] = {" + ", ".join(str(int(x)) for x in re) + "};")
+    w("constexpr int kProd[3 * kNR] = {" + ", ".join(str(int(x)) for x in pr) + "};")
+    w("// elimination order: position -> species")
+    w("constexpr int kPerm[kNS] = {" + ", ".join(str(int(x)) for x in perm) + "};")
+    w("")
+    # ------------------------------------------------------------------ rhs
+    w("// dy/dt (chemistry/solver.py:69-143) at the abundances held in slot")
+    w("// entries yoff..yoff+kNS, floored at 1e-30 as the rates see them;")
+    w("// accumulation per species follows the reaction order.  Abundances are")
+    w("// read per reaction and the fences keep a few reactions' loads together,")
+    w("// so only dy/dt stays in registers.")
+    w("template <class Slot>")
+    w("P3D_HD P3D_INLINE void rhs(const Cell &p, const Slot &m, int yoff, double (&f)[kNS]) {")
+    w("#pragma unroll")
+    w("  for (int i = 0; i < kNS; ++i) f[i] = 0.0;")
+    w("  double pm, dpm;")
+    w("  pah_minus(fmax((double)m.at(yoff + kE), 1e-30), p, pm, dpm);")
+    w("  (void)dpm;")
+    for r in range(nr):
+        a, b = int(re[2 * r]), int(re[2 * r + 1])
+        t = int(rt[r])
+        ya = f"fmax((double)m.at(yoff + {a}), 1e-30)"
+        if t == TWO_BODY:
+            if b == PAH_MINUS:
+                expr = f"m.k({r}) * ya * pm * p.nH"
+                loads = f"const double ya = {ya}; "
+            elif b == a:
+                expr = f"m.k({r}) * ya * ya * p.nH"
+                loads = f"const double ya = {ya}; "
+            else:
+                expr = f"m.k({r}) * ya * yb * p.nH"
+                loads = f"const double ya = {ya}; const double yb = fmax((double)m.at(yoff + {b}), 1e-30); "
+            slots = 2
+        elif t == GRAIN:
+            expr = f"m.k({r}) * ya * p.nH * 0.5"
+            loads = f"const double ya = {ya}; "
+            slots = 2
+        else:
+            expr = f"m.k({r}) * ya"
+            loads = f"const double ya = {ya}; "
+            slots = 1
+        upd = []
+        for k in range(slots):
+            s_ = int(re[2 * r + k])
+            if s_ >= 0:
+                upd.append(f"f[{s_}] -= q;")
+        for k in range(3):
+            s_ = int(pr[3 * r + k])
+            if s_ >= 0:
+                upd.append(f"f[{s_}] += q;")
+        w(f"  {{ {loads}const double q = {expr}; {' '.join(upd)} }}  // {net.names[r]}")
+        if RHS_FENCE_EVERY and r % RHS_FENCE_EVERY == RHS_FENCE_EVERY - 1:
+            w("  P3D_FENCE();")
+    w("}")
+    w("")
+    # ------------------------------------------------------------- jacobian
+    def partial(r, sl):
+        a, b = int(re[2 * r]), int(re[2 * r + 1])
+        t = int(rt[r])
+        act_a = "(ya > 1e-30 ? 1.0 : 0.0)"
+        if t == TWO_BODY:
+            if b == PAH_MINUS:
+                return ([a], f"k * pm * p.nH * {act_a}") if sl == 0 else \
+                    ([a], "k * ya * p.nH * dpm * (ye > 1e-30 ? 1.0 : 0.0)")
+            if a == b:
+                return [a], f"2.0 * k * ya * p.nH * {act_a}"
+            return ([a, b], f"k * yb * p.nH * {act_a}") if sl == 0 else \
+                ([a, b], "k * ya * p.nH * (yb > 1e-30 ? 1.0 : 0.0)")
+        if t == GRAIN:
+            return [a], f"k * p.nH * 0.5 * {act_a}"
+        return [a], f"k * {act_a}"
+
+    w("// Analytic Jacobian at the abundances held in slot entries yoff..yoff+kNS")
+    w("// (floored at 1e-30 as the rates see them) into the slot's J entries, one")
+    w("// row at a time; abundances are read per term so that only the current")
+    w("// row's entries stay in registers.")
+    w("template <class Slot>")
+    w("P3D_HD P3D_INLINE void jacobian(const Cell &p, const Slot &m, int yoff) {")
+    w("  const double ye = fmax((double)m.at(yoff + kE), 1e-30);")
+    w("  double pm, dpm;")
+    w("  pah_minus(ye, p, pm, dpm);")
+    for pi in range(ns):
+        i = perm[pi]
+        cols = [pj for pj in range(ns) if ppat[pi, pj]]
+        w(f"  {{  // row {net.species[i]}")
+        for pj in cols:
+            w(f"    double j{pj} = 0.0;")
+        by_r = {}
+        for pj in cols:
+            for (r, sl, v) in terms[(i, perm[pj])]:
+                by_r.setdefault((r, sl), []).append((pj, v))
+        for q, (r, sl) in enumerate(sorted(by_r)):
+            upd = " ".join(f"j{pj} += {v!r} * d;" for pj, v in by_r[(r, sl)])
+            used, expr = partial(r, sl)
+            loads = f"const double ya = fmax((double)m.at(yoff + {used[0]}), 1e-30); "
+            if len(used) > 1:
+                loads += f"const double yb = fmax((double)m.at(yoff + {used[1]}), 1e-30); "
+            w(f"    {{ const double k = m.k({r}); {loads}const double d = {expr}; {upd} }}")
+            if JAC_FENCE_EVERY and q % JAC_FENCE_EVERY == JAC_FENCE_EVERY - 1:
+                w("    P3D_FENCE();")
+        for pj in cols:
+            w(f"    m.J({jz[(pi, pj)]}) = j{pj};")
+        w("  }")
+        w("  P3D_FENCE();")
+    w("}")
+    w("")
+    # ------------------------------------------------------------ lu_factor
+    w("// M = I - c J factorised without pivoting in kPerm order (left-looking,")
+    w("// row by row).  Stores the unit-lower multipliers, the upper factor and,")
+    w("// on the diagonal, 1 / u_ii.")
+    w("template <class Slot>")
+    w("P3D_HD P3D_INLINE void lu_factor(const Slot &m, double c) {")
+    for pi in range(ns):
+        cols = [pj for pj in range(ns) if F[pi, pj]]
+        w(f"  {{  // row {net.species[perm[pi]]}")
+        for pj in cols:
+            if (pi, pj) in jz:
+                one = "1.0" if pi == pj else "0.0"
+                w(f"    double a{pj} = {one} - c * m.J({jz[(pi, pj)]});")
+            else:
+                w(f"    double a{pj} = 0.0;")
+        for pk in [pj for pj in cols if pj < pi]:
+            w(f"    a{pk} *= m.LU({lz[(pk, pk)]});")
+            for pj in [q for q in range(pk + 1, ns) if F[pk, q]]:
+                w(f"    a{pj} -= a{pk} * m.LU({lz[(pk, pj)]});")
+            if LU_PIVOT_FENCE:
+                w("    P3D_FENCE();")
+        for pj in cols:
+            if pj == pi:
+                w(f"    m.LU({lz[(pi, pj)]}) = 1.0 / a{pi};")
+            else:
+                w(f"    m.LU({lz[(pi, pj)]}) = a{pj};")
+        w("  }")
+        w("  P3D_FENCE();")
+    w("}")
+    w("")
+    # ------------------------------------------------------------- lu_solve
+    w("// Solve M x = b in place (b in species order).")
+    w("template <class Slot>")
+    w("P3D_HD P3D_INLINE void lu_solve(const Slot &m, double (&b)[kNS]) {")
+    for pi in range(ns):
+        for pk in [q for q in range(pi) if F[pi, q]]:
+            w(f"  b[{perm[pi]}] -= m.LU({lz[(pi, pk)]}) * b[{perm[pk]}];")
+        if pi % 4 == 3:
+            w("  P3D_FENCE();")
+    w("  P3D_FENCE();")
+    for pi in range(ns - 1, -1, -1):
+        for pj in [q for q in range(ns - 1, pi, -1) if F[pi, q]]:
+            w(f"  b[{perm[pi]}] -= m.LU({lz[(pi, pj)]}) * b[{perm[pj]}];")
+        w(f"  b[{perm[pi]}] *= m.LU({lz[(pi, pi)]});")
+        if pi % 4 == 0:
+            w("  P3D_FENCE();")
+    w("}")
+    w("")
+    w("}  // namespace lane")
+    w("}  // namespace p3d")
+    jcells = [None] * NJ
+    for (pi, pj), z in jz.items():
+        jcells[z] = (perm[pi], perm[pj])
+    return "\n".join(L) + "\n", dict(NJ=NJ, NLU=NLU, perm=perm, jcells=jcells)
+
+
+def write(path=OUT):
+    text, info = generate()
+    old = open(path).read() if os.path.exists(path) else None
+    if old != text:
+        with open(path, "w") as f:
+            f.write(text)
+    return path, info
+
+
+if __name__ == "__main__":
+    p, info = write()
+    print(p, "J entries", info["NJ"], "LU entries", info["NLU"], file=sys.stderr)

@@ -42,7 +42,8 @@ class PDRSolverGPU:
     def __init__(self, density, box_size, G0_external=1.0, zeta_CR_0=2e-16, nside_rt=1,
                  fixed_T=None, chemistry='euler', x_c_total=GAS_C, x_o_total=GAS_O,
                  av_per_nh=grid.AV_PER_NH, t_chem=1e7 * YR, chem_rtol=1e-9, chem_atol=1e-20,
-                 initial_abundances=None, T_init=50.0, chem_mode='steady', chem_variant=0):
+                 initial_abundances=None, T_init=50.0, chem_mode='steady', chem_variant=0,
+                 chem_waves_per_cu=0):
         self.dev = device()
         dens = np.asarray(density, dtype=np.float64)
         self.shape = dens.shape
@@ -67,6 +68,7 @@ class PDRSolverGPU:
             raise ValueError("chem_mode must be 'steady' or 'integrate'")
         self.chem_mode = chem_mode
         self.chem_variant = int(chem_variant)
+        self.chem_waves_per_cu = int(chem_waves_per_cu)
         self.record_inputs = None  # cell indices whose per-iteration inputs are kept
         self.recorded = []
         t = lambda v: torch.full(self.shape, float(v), dtype=torch.float64, device=self.dev)
@@ -156,11 +158,12 @@ class PDRSolverGPU:
                 self.y.copy_(self.y_init)
                 o = solve_opts(t_max=self.t_chem, t_start=1e4, n_checkpoints=2,
                                early_stop=False, rtol=self.chem_rtol, atol=self.chem_atol,
-                               post_mode=-1, variant=self.chem_variant)
+                               post_mode=-1, variant=self.chem_variant,
+                               waves_per_cu=self.chem_waves_per_cu)
             else:
                 o = solve_opts(t_max=None, t_start=1e4, n_checkpoints=2, early_stop=False,
                                rtol=self.chem_rtol, atol=self.chem_atol, post_mode=-1,
-                               variant=self.chem_variant)
+                               variant=self.chem_variant, waves_per_cu=self.chem_waves_per_cu)
             if self.record_inputs is not None:
                 idx = self.record_inputs
                 flat = lambda a: a.reshape(-1)[idx].cpu().numpy()
