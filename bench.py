@@ -61,6 +61,15 @@ def make_solver(n=64, variant=0, waves_per_cu=0):
                         chem_waves_per_cu=waves_per_cu)
 
 
+def chem_form(variant, n_cells):
+    """Which chemistry kernel p3d_chem_solve runs for this configuration."""
+    import torch
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    if variant == 4 or (variant == 0 and n_cells >= cus * 8 * 64):
+        return "batched: persistent one-cell-per-lane kernel with a device work queue"
+    return "plain drop-in: one cell per 64-lane workgroup"
+
+
 def solve(solver, max_iter):
     solver.reset()
     conv = solver.run(max_iterations=max_iter, convergence_tol=0.05, verbose=True)
@@ -173,7 +182,7 @@ def main():
                                   "network, steady-state BDF (t_max 1e16-3e17 s) at rtol 1e-9, "
                                   "coupled thermal",
                       "cells": solver.n_cells, "fixed_point_iterations": its,
-                      "chem_form": "plain drop-in (one cell per 64-lane workgroup)"},
+                      "chem_form": chem_form(args.variant, solver.n_cells)},
            "adjoint_over_forward": None,
            "roofline": roof}
     if not args.no_cpu:

@@ -158,7 +158,9 @@ typedef struct p3d_solve_opts {
                           1 H+/e- correction then exact projection          */
   int32_t max_steps;   /* per span                                          */
   double x_c_total, x_o_total; /* element totals for the conservation step */
-  int32_t variant;     /* kernel variant: 0 auto, 1 LU in LDS, 2 LU in registers
+  int32_t variant;     /* kernel variant: 0 auto (4 when the network is the
+                          bundled one and n >= resident lanes, else 2 or 1),
+                          1 LU in LDS, 2 LU in registers
                           (n_species <= 32), 3 LU in LDS with unrolled loops
                           (n_species <= 32): one cell per 64-lane workgroup;
                           4 persistent one-cell-per-lane kernel with a device

@@ -289,10 +289,17 @@ P3D_HD P3D_INLINE void lane_init(Lane &L) {
 
 // One trip of one lane.  Src provides:
 //   int64_t fetch();                                       next cell or -1
+//   bool should_abandon();    hand the current cell over (it is then solved
+//   void abandon(int64_t);    elsewhere from its unchanged inputs)
 //   void load(int64_t cell, Cell &p, double (&y)[kNS]);    inputs of a cell
 //   void store(int64_t cell, const double (&y)[kNS], int conv, const int (&st)[4]);
 template <class S, class Src>
 P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o, Src &src) {
+  // ---- hand a still-integrating cell over when the source asks for it -----
+  if (L.phase >= kSpan && L.phase <= kSpanEnd && src.should_abandon()) {
+    src.abandon(L.cell);
+    L.phase = kExit;
+  }
   // ---- finish a cell: corrections and output --------------------------
   P3D_FENCE();
   if (L.phase == kFinish) {

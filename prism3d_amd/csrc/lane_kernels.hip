@@ -20,16 +20,17 @@ namespace {
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
-// Slot memory through one buffer resource per wavefront.  Layout is
-// wavefront-major: wavefront w owns one contiguous block of kSlotSize x 64
-// doubles, entry e of lane l at byte e * 512 + l * 8 of that block.  Each
-// entry access of a wavefront is one contiguous 512-byte line set, a
-// wavefront's whole working set spans a few pages, and the entry offsets are
-// compile-time constants (scalar literals, no per-access address arithmetic).
-// The resource's record count is the wavefront's block, so the hardware range
+// Slot memory through one buffer resource per wavefront.  Wavefront w owns
+// one contiguous block of kSlotSize x 64 doubles; inside it each lane's
+// entries are contiguous (entry e of lane l at byte l * kLaneBlock + e * 8).
+// Blocks that only a few lanes execute (factorisations, Jacobians, step-size
+// changes) then fetch only those lanes' lines, a wavefront's working set
+// spans a few pages, and entry offsets are compile-time constants.  The
+// resource's record count is the wavefront's block, so the hardware range
 // check confines every access to it.
-constexpr unsigned kEntryStride = 64 * sizeof(double);
-constexpr unsigned kWaveBlock = lane::kSlotSize * kEntryStride;
+constexpr unsigned kEntryStride = sizeof(double);
+constexpr unsigned kLaneBlock = lane::kSlotSize * sizeof(double);
+constexpr unsigned kWaveBlock = kLaneBlock * 64;
 
 struct SlotBuf {
   __amdgpu_buffer_rsrc_t r;
@@ -67,13 +68,28 @@ struct SlotBuf {
 struct DevSrc {
   int64_t n;
   unsigned long long *ctr;
+  unsigned *exited, *n_rescue;
+  int64_t *rescue;
+  unsigned exit_threshold;
   double *y;
   const double *nH, *T, *G0, *AV, *zeta, *fH2, *fCO;
   int32_t *conv, *stats;
 
   __device__ int64_t fetch() {
     const unsigned long long c = atomicAdd(ctr, 1ull);
-    return c < (unsigned long long)n ? (int64_t)c : -1;
+    if (c < (unsigned long long)n) return (int64_t)c;
+    atomicAdd(exited, 1u);
+    return -1;
+  }
+  // true once enough lanes found the queue empty: the remaining cells are
+  // the long tail and go to the one-cell-per-workgroup kernel
+  __device__ bool should_abandon() const {
+    return __hip_atomic_load(exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+           exit_threshold;
+  }
+  __device__ void abandon(int64_t c) {
+    const unsigned k = atomicAdd(n_rescue, 1u);
+    rescue[k] = c;
   }
   __device__ void load(int64_t c, lane::Cell &p, double (&yv)[lane::kNS]) {
     p.nH = nH[c]; p.T = T[c]; p.G0 = G0[c]; p.AV = AV[c];
@@ -103,8 +119,13 @@ __device__ __forceinline__ void launder(T &x) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(64) lane_counter_reset(unsigned long long *ctr) {
-  if (threadIdx.x == 0) *ctr = 0ull;
+__global__ void __launch_bounds__(64)
+lane_counter_reset(unsigned long long *head, unsigned *exited, unsigned *n_rescue) {
+  if (threadIdx.x == 0) {
+    *head = 0ull;
+    *exited = 0u;
+    *n_rescue = 0u;
+  }
 }
 
 // W: requested wavefronts per SIMD (register budget 512 / W); the spill the
@@ -115,7 +136,7 @@ chem_lane_kernel(lane::Tables tb, lane::Opts o, DevSrc src, double *__restrict__
   SlotBuf m;
   m.r = __builtin_amdgcn_make_buffer_rsrc(ws + (size_t)blockIdx.x * lane::kSlotSize * 64, (short)0,
                                           (int)kWaveBlock, 0x00020000);
-  m.voff = threadIdx.x * 8u;
+  m.voff = threadIdx.x * kLaneBlock;
   lane::Lane L;
   lane::lane_init(L);
   for (;;) {
@@ -154,7 +175,7 @@ bool lane_network_matches(int ns, int nr, const int *rtype, const int *reactants
 int lane_slot_doubles() { return lane::kSlotSize; }
 
 int lane_launch(const LaneArgs &a, int64_t n_waves, int occupancy, double *ws,
-                unsigned long long *ctr) {
+                const LaneQueue &wq) {
   const p3d_solve_opts &q = a.opts;
   lane::Opts o;
   o.t_max = q.t_max; o.t_start = q.t_start; o.rtol = q.rtol; o.atol = q.atol;
@@ -163,9 +184,12 @@ int lane_launch(const LaneArgs &a, int64_t n_waves, int occupancy, double *ws,
   o.nk = q.n_checkpoints; o.early_stop = q.early_stop; o.post_mode = q.post_mode;
   o.max_steps = q.max_steps;
   lane::Tables tb{a.rtype, a.alpha, a.beta, a.gamma, a.post_w};
-  DevSrc src{a.n, ctr, a.y, a.nH, a.T, a.G0, a.AV, a.zeta, a.fH2, a.fCO, a.conv, a.stats};
+  DevSrc src{a.n,     wq.head, wq.exited, wq.n_rescue, wq.rescue, wq.exit_threshold,
+             a.y,     a.nH,    a.T,      a.G0,       a.AV,     a.zeta,
+             a.fH2,   a.fCO,   a.conv,   a.stats};
   hipStream_t s = as_stream(a.stream);
-  hipLaunchKernelGGL(lane_counter_reset, dim3(1), dim3(64), 0, s, ctr);
+  hipLaunchKernelGGL(lane_counter_reset, dim3(1), dim3(64), 0, s, wq.head, wq.exited,
+                     wq.n_rescue);
   const dim3 grid((unsigned)n_waves), block(64);
   if (occupancy == 4)
     hipLaunchKernelGGL(chem_lane_kernel<4>, grid, block, 0, s, tb, o, src, ws);

@@ -26,10 +26,25 @@ struct LaneArgs {
 // Slot-memory entries per lane (doubles).
 int lane_slot_doubles();
 
+// Fraction of lanes that must have found the work queue empty before the
+// cells still integrating are handed to the one-cell-per-workgroup kernel.
+constexpr double kLaneRescueFrac = 0.95;
+
+// Device work queue of one launch: head counter, number of lanes that found
+// it empty, and the list of cells handed over (count + indices).
+struct LaneQueue {
+  unsigned long long *head;
+  unsigned *exited;
+  unsigned *n_rescue;
+  int64_t *rescue;
+  unsigned exit_threshold;
+};
+
 // Launches the persistent kernel over `n_waves` wavefronts compiled for
 // `occupancy` (2, 3 or 4) wavefronts per SIMD; ws holds
-// n_waves * 64 * lane_slot_doubles() doubles, ctr one 64-bit counter.
+// n_waves * 64 * lane_slot_doubles() doubles.  The queue counters are reset
+// on the stream first.
 int lane_launch(const LaneArgs &a, int64_t n_waves, int occupancy, double *ws,
-                unsigned long long *ctr);
+                const LaneQueue &q);
 
 }  // namespace p3d

@@ -827,9 +827,13 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
                   const double *__restrict__ AV, const double *__restrict__ zeta,
                   const double *__restrict__ fH2, const double *__restrict__ fCO,
                   p3d_solve_opts o, int32_t *__restrict__ conv_out,
-                  int32_t *__restrict__ stats_out) {
+                  int32_t *__restrict__ stats_out, const int64_t *__restrict__ cell_list,
+                  const unsigned *__restrict__ list_len) {
   extern __shared__ double smem[];
-  const int64_t cell = blockIdx.x;
+  // with a cell list (cells handed over by the lane kernel), workgroup b
+  // solves cell_list[b] if b < *list_len
+  if (cell_list && blockIdx.x >= *list_len) return;
+  const int64_t cell = cell_list ? cell_list[blockIdx.x] : (int64_t)blockIdx.x;
   const int lane = threadIdx.x;
   const Lds L = carve(smem, N);
   const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
@@ -956,8 +960,11 @@ static int lane_solve(Network *net, int64_t n, double *y, const double *n_h, con
   // slot memory: one 64-lane block per resident wavefront (bounded to 16 GiB)
   const int64_t max_waves = ((int64_t)16 << 30) / (64 * (int64_t)lane_slot_doubles() * 8);
   n_waves = std::max<int64_t>(1, std::min(n_waves, max_waves));
-  const size_t slot_bytes = (size_t)n_waves * 64 * lane_slot_doubles() * sizeof(double);
-  const size_t need = slot_bytes + 256;
+  const int64_t n_lanes = n_waves * 64;
+  const size_t slot_bytes = (size_t)n_lanes * lane_slot_doubles() * sizeof(double);
+  // slot memory, then the counters (64-bit queue head, 32-bit exited lanes,
+  // 32-bit rescue count), then the rescue list (one int64 per lane)
+  const size_t need = slot_bytes + 256 + (size_t)n_lanes * sizeof(int64_t);
   if (net->lane_ws_bytes < need) {
     if (net->lane_ws) {
       P3D_CHECK_HIP(hipStreamSynchronize(as_stream(stream)));
@@ -978,10 +985,26 @@ static int lane_solve(Network *net, int64_t n, double *y, const double *n_h, con
   a.rtype = net->d.rtype; a.alpha = net->d.alpha; a.beta = net->d.beta; a.gamma = net->d.gamma;
   a.post_w = net->d.post_w; a.opts = o; a.stream = stream;
   double *ws = static_cast<double *>(net->lane_ws);
-  auto *ctr = reinterpret_cast<unsigned long long *>(static_cast<char *>(net->lane_ws) + slot_bytes);
+  char *tail = static_cast<char *>(net->lane_ws) + slot_bytes;
+  LaneQueue q;
+  q.head = reinterpret_cast<unsigned long long *>(tail);
+  q.exited = reinterpret_cast<unsigned *>(tail + 8);
+  q.n_rescue = reinterpret_cast<unsigned *>(tail + 12);
+  q.rescue = reinterpret_cast<int64_t *>(tail + 256);
+  // once this many lanes found the queue empty, cells still integrating are
+  // handed to the one-cell-per-workgroup kernel (long-tail cells)
+  q.exit_threshold = (unsigned)std::max<int64_t>(1, (int64_t)(kLaneRescueFrac * (double)n_lanes));
   // wavefronts per SIMD the kernel is compiled for: 4 SIMDs per CU
   const int occ = std::min(4, std::max(2, (wpc + 3) / 4));
-  return lane_launch(a, n_waves, occ, ws, ctr);
+  int rc = lane_launch(a, n_waves, occ, ws, q);
+  if (rc != P3D_OK) return rc;
+  const int64_t max_rescue = n_lanes - (int64_t)q.exit_threshold + 1;
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
+  P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
+  hipLaunchKernelGGL(chem_solve_kernel<LinReg<32>>, dim3((unsigned)max_rescue), dim3(64), sh,
+                     as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o,
+                     converged, stats, q.rescue, q.n_rescue);
+  return launch_status("chem_solve_kernel (lane rescue)");
 }
 
 }  // namespace p3d
@@ -1236,18 +1259,29 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
   int var = o.variant;
   if (var == 0) var = net->d.ns <= 32 ? 2 : 1;
+  // auto: the persistent lane kernel once the batch fills its resident lanes
+  // (below that the slowest cell's tail dominates and one cell per
+  // workgroup is faster; measured crossover ~1e5 cells on MI355X)
+  if (o.variant == 0 && net->lane_ok) {
+    int dev = 0, cus = 0;
+    P3D_CHECK_HIP(hipGetDevice(&dev));
+    P3D_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (n >= (int64_t)cus * 8 * 64)
+      return lane_solve(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o, converged,
+                        stats, stream);
+  }
   if (var == 3)
     hipLaunchKernelGGL(chem_solve_kernel<LinLdsU<32>>, dim3((unsigned)n), dim3(64), sh,
                        as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,
-                       o, converged, stats);
+                       o, converged, stats, nullptr, nullptr);
   else if (var == 2)
     hipLaunchKernelGGL(chem_solve_kernel<LinReg<32>>, dim3((unsigned)n), dim3(64), sh,
                        as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,
-                       o, converged, stats);
+                       o, converged, stats, nullptr, nullptr);
   else
     hipLaunchKernelGGL(chem_solve_kernel<LinLds>, dim3((unsigned)n), dim3(64), sh,
                        as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,
-                       o, converged, stats);
+                       o, converged, stats, nullptr, nullptr);
   return launch_status("chem_solve_kernel");
 }
 

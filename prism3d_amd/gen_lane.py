@@ -39,9 +39,11 @@ PAH_MINUS, NONE = -2, -1
 # compiler-fence spacing in the emitted code (0: none): bounds how many slot
 # loads the compiler may keep in flight at once, i.e. register pressure
 # against memory-latency overlap
-RHS_FENCE_EVERY = 8
-JAC_FENCE_EVERY = 0
-LU_PIVOT_FENCE = False
+RHS_FENCE_EVERY = 16
+JAC_ROWS_PER_CHUNK = 2
+JAC_TERMS_PER_GROUP = 8
+LU_ROWS_PER_GROUP = 5
+SOLVE_ROWS_PER_GROUP = 8
 
 
 def structure(net):
@@ -245,27 +247,33 @@ def generate(net=None):
     w("  const double ye = fmax((double)m.at(yoff + kE), 1e-30);")
     w("  double pm, dpm;")
     w("  pah_minus(ye, p, pm, dpm);")
-    for pi in range(ns):
-        i = perm[pi]
-        cols = [pj for pj in range(ns) if ppat[pi, pj]]
-        w(f"  {{  // row {net.species[i]}")
-        for pj in cols:
-            w(f"    double j{pj} = 0.0;")
-        by_r = {}
-        for pj in cols:
-            for (r, sl, v) in terms[(i, perm[pj])]:
-                by_r.setdefault((r, sl), []).append((pj, v))
-        for q, (r, sl) in enumerate(sorted(by_r)):
-            upd = " ".join(f"j{pj} += {v!r} * d;" for pj, v in by_r[(r, sl)])
-            used, expr = partial(r, sl)
-            loads = f"const double ya = fmax((double)m.at(yoff + {used[0]}), 1e-30); "
-            if len(used) > 1:
-                loads += f"const double yb = fmax((double)m.at(yoff + {used[1]}), 1e-30); "
-            w(f"    {{ const double k = m.k({r}); {loads}const double d = {expr}; {upd} }}")
-            if JAC_FENCE_EVERY and q % JAC_FENCE_EVERY == JAC_FENCE_EVERY - 1:
-                w("    P3D_FENCE();")
-        for pj in cols:
-            w(f"    m.J({jz[(pi, pj)]}) = j{pj};")
+    for c0 in range(0, ns, JAC_ROWS_PER_CHUNK):
+        rows = list(range(c0, min(ns, c0 + JAC_ROWS_PER_CHUNK)))
+        w(f"  {{  // rows {', '.join(net.species[perm[pi]] for pi in rows)}")
+        for pi in rows:
+            for pj in [pj for pj in range(ns) if ppat[pi, pj]]:
+                w(f"    double j{pi}_{pj} = 0.0;")
+        nterm = 0
+        for pi in rows:
+            i = perm[pi]
+            cols = [pj for pj in range(ns) if ppat[pi, pj]]
+            by_r = {}
+            for pj in cols:
+                for (r, sl, v) in terms[(i, perm[pj])]:
+                    by_r.setdefault((r, sl), []).append((pj, v))
+            for q, (r, sl) in enumerate(sorted(by_r)):
+                upd = " ".join(f"j{pi}_{pj} += {v!r} * d;" for pj, v in by_r[(r, sl)])
+                used, expr = partial(r, sl)
+                loads = f"const double ya = fmax((double)m.at(yoff + {used[0]}), 1e-30); "
+                if len(used) > 1:
+                    loads += f"const double yb = fmax((double)m.at(yoff + {used[1]}), 1e-30); "
+                w(f"    {{ const double k = m.k({r}); {loads}const double d = {expr}; {upd} }}")
+                nterm += 1
+                if nterm % JAC_TERMS_PER_GROUP == 0:
+                    w("    P3D_FENCE();")
+        for pi in rows:
+            for pj in [pj for pj in range(ns) if ppat[pi, pj]]:
+                w(f"    m.J({jz[(pi, pj)]}) = j{pi}_{pj};")
         w("  }")
         w("  P3D_FENCE();")
     w("}")
@@ -276,26 +284,39 @@ def generate(net=None):
     w("// on the diagonal, 1 / u_ii.")
     w("template <class Slot>")
     w("P3D_HD P3D_INLINE void lu_factor(const Slot &m, double c) {")
+    # rows grouped by dependency level: a level's rows read only rows of
+    # earlier levels, so each level loads, computes and stores as one group
+    lev = [0] * ns
     for pi in range(ns):
-        cols = [pj for pj in range(ns) if F[pi, pj]]
-        w(f"  {{  // row {net.species[perm[pi]]}")
-        for pj in cols:
-            if (pi, pj) in jz:
-                one = "1.0" if pi == pj else "0.0"
-                w(f"    double a{pj} = {one} - c * m.J({jz[(pi, pj)]});")
-            else:
-                w(f"    double a{pj} = 0.0;")
-        for pk in [pj for pj in cols if pj < pi]:
-            w(f"    a{pk} *= m.LU({lz[(pk, pk)]});")
-            for pj in [q for q in range(pk + 1, ns) if F[pk, q]]:
-                w(f"    a{pj} -= a{pk} * m.LU({lz[(pk, pj)]});")
-            if LU_PIVOT_FENCE:
-                w("    P3D_FENCE();")
-        for pj in cols:
-            if pj == pi:
-                w(f"    m.LU({lz[(pi, pj)]}) = 1.0 / a{pi};")
-            else:
-                w(f"    m.LU({lz[(pi, pj)]}) = a{pj};")
+        lev[pi] = 1 + max([lev[pk] for pk in range(pi) if F[pi, pk]], default=0)
+    groups = []
+    for L_ in range(1, max(lev) + 1):
+        rows_l = [pi for pi in range(ns) if lev[pi] == L_]
+        for g0 in range(0, len(rows_l), LU_ROWS_PER_GROUP):
+            groups.append((L_, rows_l[g0:g0 + LU_ROWS_PER_GROUP]))
+    for L_, rows in groups:
+        w(f"  {{  // level {L_}: {', '.join(net.species[perm[pi]] for pi in rows)}")
+        for pi in rows:
+            cols = [pj for pj in range(ns) if F[pi, pj]]
+            for pj in cols:
+                if (pi, pj) in jz:
+                    one = "1.0" if pi == pj else "0.0"
+                    w(f"    double a{pi}_{pj} = {one} - c * m.J({jz[(pi, pj)]});")
+                else:
+                    w(f"    double a{pi}_{pj} = 0.0;")
+        for pi in rows:
+            cols = [pj for pj in range(ns) if F[pi, pj]]
+            for pk in [pj for pj in cols if pj < pi]:
+                w(f"    a{pi}_{pk} *= m.LU({lz[(pk, pk)]});")
+                for pj in [q for q in range(pk + 1, ns) if F[pk, q]]:
+                    w(f"    a{pi}_{pj} -= a{pi}_{pk} * m.LU({lz[(pk, pj)]});")
+        for pi in rows:
+            cols = [pj for pj in range(ns) if F[pi, pj]]
+            for pj in cols:
+                if pj == pi:
+                    w(f"    m.LU({lz[(pi, pj)]}) = 1.0 / a{pi}_{pj};")
+                else:
+                    w(f"    m.LU({lz[(pi, pj)]}) = a{pi}_{pj};")
         w("  }")
         w("  P3D_FENCE();")
     w("}")
@@ -307,14 +328,14 @@ def generate(net=None):
     for pi in range(ns):
         for pk in [q for q in range(pi) if F[pi, q]]:
             w(f"  b[{perm[pi]}] -= m.LU({lz[(pi, pk)]}) * b[{perm[pk]}];")
-        if pi % 4 == 3:
+        if pi % SOLVE_ROWS_PER_GROUP == SOLVE_ROWS_PER_GROUP - 1:
             w("  P3D_FENCE();")
     w("  P3D_FENCE();")
     for pi in range(ns - 1, -1, -1):
         for pj in [q for q in range(ns - 1, pi, -1) if F[pi, q]]:
             w(f"  b[{perm[pi]}] -= m.LU({lz[(pi, pj)]}) * b[{perm[pj]}];")
         w(f"  b[{perm[pi]}] *= m.LU({lz[(pi, pi)]});")
-        if pi % 4 == 0:
+        if pi % SOLVE_ROWS_PER_GROUP == 0:
             w("  P3D_FENCE();")
     w("}")
     w("")

@@ -14,6 +14,8 @@ struct HostSrc {
   const double *nH, *T, *G0, *AV, *zeta, *fH2, *fCO;
   int *conv, *stats;
   int64_t fetch() { return next < n ? next++ : -1; }
+  bool should_abandon() const { return false; }
+  void abandon(int64_t) {}
   void load(int64_t c, Cell &p, double (&yv)[kNS]) {
     p.nH = nH[c]; p.T = T[c]; p.G0 = G0[c]; p.AV = AV[c];
     p.zeta = zeta[c]; p.fH2 = fH2[c]; p.fCO = fCO[c];
