@@ -70,6 +70,19 @@ def chem_form(variant, n_cells):
     return "plain drop-in: one cell per 64-lane workgroup"
 
 
+def profiled_traffic(args, lane):
+    """HBM bytes per chemistry launch of this workload as measured by rocprofv3
+    (FETCH_SIZE / WRITE_SIZE, calibrated; profiles/chem_64cube_traffic.json),
+    or None when no profile of this configuration is committed."""
+    path = os.path.join(ROOT, "profiles", "chem_64cube_traffic.json")
+    if not (lane and args.n == 64 and args.variant in (0, 4) and args.waves_per_cu in (0, 8)
+            and os.path.exists(path)):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    return float(sum(rec["hbm_bytes_per_launch"].values()))
+
+
 def solve(solver, max_iter):
     solver.reset()
     conv = solver.run(max_iterations=max_iter, convergence_tol=0.05, verbose=True)
@@ -169,10 +182,14 @@ def main():
     steps_mean = float(np.mean([st[:, 0].float().mean().item() for *_x, st in solver.chem_records]))
     avg_dur = float(np.mean(durs))
     achieved = float(np.mean(fl)) / avg_dur / 1e12
+    form = chem_form(args.variant, solver.n_cells)
+    lane = form.startswith("batched")
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-            "kernel": "chem_solve_kernel", "avg_launch_ms": avg_dur * 1e3,
-            "launches": len(durs), "bdf_steps_per_cell": steps_mean}
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": profiled_traffic(args, lane),
+            "kernel": ("chem_lane_kernel + hand-over chem_solve_kernel (one p3d_chem_solve call)"
+                       if lane else "chem_solve_kernel"),
+            "avg_launch_ms": avg_dur * 1e3, "launches": len(durs),
+            "bdf_steps_per_cell": steps_mean}
     out = {"metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -182,7 +199,7 @@ def main():
                                   "network, steady-state BDF (t_max 1e16-3e17 s) at rtol 1e-9, "
                                   "coupled thermal",
                       "cells": solver.n_cells, "fixed_point_iterations": its,
-                      "chem_form": chem_form(args.variant, solver.n_cells)},
+                      "chem_form": form},
            "adjoint_over_forward": None,
            "roofline": roof}
     if not args.no_cpu:

@@ -168,6 +168,10 @@ typedef struct p3d_solve_opts {
   int32_t waves_per_cu; /* variant 4: resident wavefronts per compute unit
                           (0: library default 8); the kernel is compiled for
                           waves_per_cu / 4 wavefronts per SIMD (2..4) */
+  double rescue_frac;  /* variant 4: once this fraction of lanes found the work
+                          queue empty, cells still integrating are solved by
+                          the one-cell-per-workgroup kernel instead (0: library
+                          default 0.95; 1: never) */
 } p3d_solve_opts;
 
 /* Integrate every cell to its steady state / end time with the

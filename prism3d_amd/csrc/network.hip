@@ -19,6 +19,7 @@
 //                          solver.py:195-201: the quasi-constant step size NDF
 //                          of Shampine & Reichelt (1997), orders 1-5.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -993,7 +994,8 @@ static int lane_solve(Network *net, int64_t n, double *y, const double *n_h, con
   q.rescue = reinterpret_cast<int64_t *>(tail + 256);
   // once this many lanes found the queue empty, cells still integrating are
   // handed to the one-cell-per-workgroup kernel (long-tail cells)
-  q.exit_threshold = (unsigned)std::max<int64_t>(1, (int64_t)(kLaneRescueFrac * (double)n_lanes));
+  const double frac = o.rescue_frac > 0.0 ? std::min(o.rescue_frac, 1.0) : kLaneRescueFrac;
+  q.exit_threshold = (unsigned)std::max<int64_t>(1, (int64_t)std::ceil(frac * (double)n_lanes));
   // wavefronts per SIMD the kernel is compiled for: 4 SIMDs per CU
   const int occ = std::min(4, std::max(2, (wpc + 3) / 4));
   int rc = lane_launch(a, n_waves, occ, ws, q);
@@ -1253,6 +1255,7 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
               "variant 4 (one cell per lane) is generated for the bundled network structure; "
               "this network differs");
   P3D_REQUIRE(o.waves_per_cu >= 0 && o.waves_per_cu <= 64, "waves_per_cu must be in [0, 64]");
+  P3D_REQUIRE(o.rescue_frac >= 0.0 && o.rescue_frac <= 1.0, "rescue_frac must be in [0, 1]");
   if (o.variant == 4) return lane_solve(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o,
                                         converged, stats, stream);
   const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);

@@ -43,7 +43,7 @@ class PDRSolverGPU:
                  fixed_T=None, chemistry='euler', x_c_total=GAS_C, x_o_total=GAS_O,
                  av_per_nh=grid.AV_PER_NH, t_chem=1e7 * YR, chem_rtol=1e-9, chem_atol=1e-20,
                  initial_abundances=None, T_init=50.0, chem_mode='steady', chem_variant=0,
-                 chem_waves_per_cu=0):
+                 chem_waves_per_cu=0, chem_rescue_frac=0.0):
         self.dev = device()
         dens = np.asarray(density, dtype=np.float64)
         self.shape = dens.shape
@@ -69,6 +69,7 @@ class PDRSolverGPU:
         self.chem_mode = chem_mode
         self.chem_variant = int(chem_variant)
         self.chem_waves_per_cu = int(chem_waves_per_cu)
+        self.chem_rescue_frac = float(chem_rescue_frac)
         self.record_inputs = None  # cell indices whose per-iteration inputs are kept
         self.recorded = []
         t = lambda v: torch.full(self.shape, float(v), dtype=torch.float64, device=self.dev)
@@ -159,11 +160,13 @@ class PDRSolverGPU:
                 o = solve_opts(t_max=self.t_chem, t_start=1e4, n_checkpoints=2,
                                early_stop=False, rtol=self.chem_rtol, atol=self.chem_atol,
                                post_mode=-1, variant=self.chem_variant,
-                               waves_per_cu=self.chem_waves_per_cu)
+                               waves_per_cu=self.chem_waves_per_cu,
+                               rescue_frac=self.chem_rescue_frac)
             else:
                 o = solve_opts(t_max=None, t_start=1e4, n_checkpoints=2, early_stop=False,
                                rtol=self.chem_rtol, atol=self.chem_atol, post_mode=-1,
-                               variant=self.chem_variant, waves_per_cu=self.chem_waves_per_cu)
+                               variant=self.chem_variant, waves_per_cu=self.chem_waves_per_cu,
+                               rescue_frac=self.chem_rescue_frac)
             if self.record_inputs is not None:
                 idx = self.record_inputs
                 flat = lambda a: a.reshape(-1)[idx].cpu().numpy()
