@@ -39,6 +39,10 @@
 // hoisting every slot load (or forwarding every stored value) across rows,
 // which would exhaust the register file.
 #define P3D_FENCE() __asm__ __volatile__("" ::: "memory")
+// true if the predicate holds on any lane of the wavefront (one lane on the host)
+#ifndef P3D_ANY
+#define P3D_ANY(x) (x)
+#endif
 
 namespace p3d {
 namespace lane {
@@ -535,51 +539,57 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
   }
   // ---- one Newton iteration ------------------------------------------------
   P3D_FENCE();
-  if (L.phase == kNewton) {
-    double f[kNS];
-    rhs(L.p, m, kOffYN, f);
-    L.st_rhs++;
-    double fs = 0.0;
+  if (L.phase == kNewton) L.nres = 0;
+  // Newton iterations of the step: lanes iterate until converged (nres 1) or
+  // failed (nres 2); the wavefront leaves the loop when no lane iterates, so
+  // the predictor, factorisation and acceptance blocks run once per step
+  for (int pass = 0; pass < 4 && P3D_ANY(L.phase == kNewton && L.nres == 0); ++pass) {
+    P3D_FENCE();
+    if (L.phase == kNewton && L.nres == 0) {
+      double f[kNS];
+      rhs(L.p, m, kOffYN, f);
+      L.st_rhs++;
+      double fs = 0.0;
 #pragma unroll
-    for (int s = 0; s < kNS; ++s) fs += f[s];
-    L.nres = 0;
-    if (!isfinite(fs)) {
-      L.nres = 2;
-    } else {
-#pragma unroll
-      for (int s = 0; s < kNS; ++s) f[s] = L.c * f[s] - m.psi(s) - m.dd(s);
-      lu_solve(m, f);
-      double sq = 0.0;
-#pragma unroll
-      for (int s = 0; s < kNS; ++s) {
-        const double sc = o.atol + o.rtol * fabs(m.yp(s));
-        sq += (f[s] / sc) * (f[s] / sc);
-      }
-      const double dyn = rms_sum(sq);
-      const double rate = L.dy_old >= 0.0 ? dyn / L.dy_old : -1.0;
-      if (rate >= 0.0 &&
-          (rate >= 1.0 || pow(rate, (double)(4 - L.it)) / (1.0 - rate) * dyn > o.newton_tol)) {
+      for (int s = 0; s < kNS; ++s) fs += f[s];
+      if (!isfinite(fs)) {
         L.nres = 2;
       } else {
 #pragma unroll
-        for (int c0 = 0; c0 < kNS; c0 += kChunk8) {
-          double yv[kChunk8], dv[kChunk8];
+        for (int s = 0; s < kNS; ++s) f[s] = L.c * f[s] - m.psi(s) - m.dd(s);
+        lu_solve(m, f);
+        double sq = 0.0;
 #pragma unroll
-          for (int q = 0; q < kChunk8; ++q)
-            if (c0 + q < kNS) { yv[q] = m.yn(c0 + q); dv[q] = m.dd(c0 + q); }
-#pragma unroll
-          for (int q = 0; q < kChunk8; ++q)
-            if (c0 + q < kNS) {
-              m.yn(c0 + q) = yv[q] + f[c0 + q];
-              m.dd(c0 + q) = dv[q] + f[c0 + q];
-            }
+        for (int s = 0; s < kNS; ++s) {
+          const double sc = o.atol + o.rtol * fabs(m.yp(s));
+          sq += (f[s] / sc) * (f[s] / sc);
         }
-        if (dyn == 0.0 || (rate >= 0.0 && rate / (1.0 - rate) * dyn < o.newton_tol)) {
-          L.nres = 1;
+        const double dyn = rms_sum(sq);
+        const double rate = L.dy_old >= 0.0 ? dyn / L.dy_old : -1.0;
+        if (rate >= 0.0 &&
+            (rate >= 1.0 || pow(rate, (double)(4 - L.it)) / (1.0 - rate) * dyn > o.newton_tol)) {
+          L.nres = 2;
         } else {
-          L.dy_old = dyn;
-          L.it++;
-          if (L.it == 4) L.nres = 2;
+#pragma unroll
+          for (int c0 = 0; c0 < kNS; c0 += kChunk8) {
+            double yv[kChunk8], dv[kChunk8];
+#pragma unroll
+            for (int q = 0; q < kChunk8; ++q)
+              if (c0 + q < kNS) { yv[q] = m.yn(c0 + q); dv[q] = m.dd(c0 + q); }
+#pragma unroll
+            for (int q = 0; q < kChunk8; ++q)
+              if (c0 + q < kNS) {
+                m.yn(c0 + q) = yv[q] + f[c0 + q];
+                m.dd(c0 + q) = dv[q] + f[c0 + q];
+              }
+          }
+          if (dyn == 0.0 || (rate >= 0.0 && rate / (1.0 - rate) * dyn < o.newton_tol)) {
+            L.nres = 1;
+          } else {
+            L.dy_old = dyn;
+            L.it++;
+            if (L.it == 4) L.nres = 2;
+          }
         }
       }
     }

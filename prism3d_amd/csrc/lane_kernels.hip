@@ -10,6 +10,7 @@
 // straight-line code of lane_net.h.
 #define P3D_HD __host__ __device__
 #define P3D_INLINE __forceinline__
+#define P3D_ANY(x) (__any((x) ? 1 : 0) != 0)
 #include "p3d_common.h"
 #include "lane_bdf.h"
 #include "lane_launch.h"
@@ -21,16 +22,16 @@ namespace {
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 // Slot memory through one buffer resource per wavefront.  Wavefront w owns
-// one contiguous block of kSlotSize x 64 doubles; inside it each lane's
-// entries are contiguous (entry e of lane l at byte l * kLaneBlock + e * 8).
-// Blocks that only a few lanes execute (factorisations, Jacobians, step-size
-// changes) then fetch only those lanes' lines, a wavefront's working set
-// spans a few pages, and entry offsets are compile-time constants.  The
-// resource's record count is the wavefront's block, so the hardware range
-// check confines every access to it.
-constexpr unsigned kEntryStride = sizeof(double);
-constexpr unsigned kLaneBlock = lane::kSlotSize * sizeof(double);
-constexpr unsigned kWaveBlock = kLaneBlock * 64;
+// one contiguous block of kSlotSize x 64 doubles, entry-major: entry e of
+// lane l at byte e * 512 + l * 8 of the block.  A wavefront's access to one
+// entry is one contiguous 512-byte segment (four cache lines), its working
+// set spans a few pages, and entry offsets are compile-time constants.  (A
+// lane-major block, each lane's entries contiguous, measured faster only
+// while most blocks ran with few active lanes.)  The resource's record count
+// is the wavefront's block, so the hardware range check confines every access
+// to it.
+constexpr unsigned kEntryStride = 64 * sizeof(double);
+constexpr unsigned kWaveBlock = lane::kSlotSize * kEntryStride;
 
 struct SlotBuf {
   __amdgpu_buffer_rsrc_t r;
@@ -136,7 +137,7 @@ chem_lane_kernel(lane::Tables tb, lane::Opts o, DevSrc src, double *__restrict__
   SlotBuf m;
   m.r = __builtin_amdgcn_make_buffer_rsrc(ws + (size_t)blockIdx.x * lane::kSlotSize * 64, (short)0,
                                           (int)kWaveBlock, 0x00020000);
-  m.voff = threadIdx.x * kLaneBlock;
+  m.voff = threadIdx.x * 8u;
   lane::Lane L;
   lane::lane_init(L);
   for (;;) {
