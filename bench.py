@@ -48,7 +48,7 @@ def flops_per_cell(stats):
             + jac * (2.0 * 300 + 4 * N_RX) + steps * 40.0 * n)
 
 
-def make_solver(n=64, variant=0, waves_per_cu=0):
+def make_solver(n=64, variant=0, waves_per_cu=0, autotune=False):
     from prism3d_amd.density import PC, lognormal_median
     from prism3d_amd.grid import directional_G0
     from prism3d_amd.solver3d import PDRSolverGPU
@@ -58,7 +58,7 @@ def make_solver(n=64, variant=0, waves_per_cu=0):
                         chemistry='network', x_c_total=1.4e-4, x_o_total=3.0e-4,
                         initial_abundances={'H': 1.0, 'C+': 1.4e-4, 'O': 3.0e-4, 'e-': 1.4e-4},
                         T_init=50.0, chem_atol=1e-20, chem_variant=variant,
-                        chem_waves_per_cu=waves_per_cu)
+                        chem_waves_per_cu=waves_per_cu, chem_autotune=autotune)
 
 
 def chem_form(variant, n_cells):
@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--max-iter", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="chemistry kernel variant (0 auto)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the chemistry kernel configuration online (prism3d_amd.autotune)")
     ap.add_argument("--waves-per-cu", type=int, default=0,
                     help="resident wavefronts per CU of the persistent kernel (0 default)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -150,7 +152,7 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    solver = make_solver(args.n, args.variant, args.waves_per_cu)
+    solver = make_solver(args.n, args.variant, args.waves_per_cu, args.autotune)
     print(f"[bench] {args.n}^3 = {solver.n_cells} cells; warmup {args.warmup}", file=sys.stderr,
           flush=True)
     for w in range(args.warmup):

@@ -152,8 +152,10 @@ class ChemistrySolver:
         self._check_state(y)
         n = y.shape[1]
         p = _cell_arrays(n, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO)
-        conv = torch.zeros(n, dtype=torch.int32, device=y.device)
-        stats = torch.zeros((n, 4), dtype=torch.int32, device=y.device)
+        # every cell's flag and counters are written by the solve (no fill, so
+        # the call can be captured in a graph)
+        conv = torch.empty(n, dtype=torch.int32, device=y.device)
+        stats = torch.empty((n, 4), dtype=torch.int32, device=y.device)
         o = opts if opts is not None else solve_opts()
         rc = _lib.lib().p3d_chem_solve(self.dnet.handle, n, y.data_ptr(),
                                        *[t.data_ptr() for t in p], o, conv.data_ptr(),

@@ -43,7 +43,7 @@ class PDRSolverGPU:
                  fixed_T=None, chemistry='euler', x_c_total=GAS_C, x_o_total=GAS_O,
                  av_per_nh=grid.AV_PER_NH, t_chem=1e7 * YR, chem_rtol=1e-9, chem_atol=1e-20,
                  initial_abundances=None, T_init=50.0, chem_mode='steady', chem_variant=0,
-                 chem_waves_per_cu=0, chem_rescue_frac=0.0):
+                 chem_waves_per_cu=0, chem_rescue_frac=0.0, chem_autotune=False):
         self.dev = device()
         dens = np.asarray(density, dtype=np.float64)
         self.shape = dens.shape
@@ -70,6 +70,12 @@ class PDRSolverGPU:
         self.chem_variant = int(chem_variant)
         self.chem_waves_per_cu = int(chem_waves_per_cu)
         self.chem_rescue_frac = float(chem_rescue_frac)
+        # optional online choice of (variant, waves_per_cu, rescue_frac) from
+        # the per-iteration chemistry timings
+        self.tuner = None
+        if chem_autotune and chemistry == 'network':
+            from .autotune import ChemTuner
+            self.tuner = ChemTuner(self.n_cells, torch.cuda.get_device_name(self.dev))
         self.record_inputs = None  # cell indices whose per-iteration inputs are kept
         self.recorded = []
         t = lambda v: torch.full(self.shape, float(v), dtype=torch.float64, device=self.dev)
@@ -155,18 +161,19 @@ class PDRSolverGPU:
 
     def chemistry_step(self):
         if self.chemistry == 'network':
+            cfg = (self.chem_variant, self.chem_waves_per_cu, self.chem_rescue_frac)
+            if self.tuner is not None:
+                cfg = self.tuner.next_config()
             if self.chem_mode == 'integrate':
                 self.y.copy_(self.y_init)
                 o = solve_opts(t_max=self.t_chem, t_start=1e4, n_checkpoints=2,
                                early_stop=False, rtol=self.chem_rtol, atol=self.chem_atol,
-                               post_mode=-1, variant=self.chem_variant,
-                               waves_per_cu=self.chem_waves_per_cu,
-                               rescue_frac=self.chem_rescue_frac)
+                               post_mode=-1, variant=cfg[0], waves_per_cu=cfg[1],
+                               rescue_frac=cfg[2])
             else:
                 o = solve_opts(t_max=None, t_start=1e4, n_checkpoints=2, early_stop=False,
                                rtol=self.chem_rtol, atol=self.chem_atol, post_mode=-1,
-                               variant=self.chem_variant, waves_per_cu=self.chem_waves_per_cu,
-                               rescue_frac=self.chem_rescue_frac)
+                               variant=cfg[0], waves_per_cu=cfg[1], rescue_frac=cfg[2])
             if self.record_inputs is not None:
                 idx = self.record_inputs
                 flat = lambda a: a.reshape(-1)[idx].cpu().numpy()
@@ -175,7 +182,8 @@ class PDRSolverGPU:
                     G0=flat(self.G0), A_V=flat(self.A_V), zeta=flat(self.zeta_CR),
                     fsh_H2=flat(self.fsh_H2), fsh_CO=flat(self.fsh_CO),
                     t_max=o.t_max))
-            if self.record_chem:
+            timed = self.record_chem or (self.tuner is not None and self.tuner.tuning)
+            if timed:
                 ev0 = torch.cuda.Event(enable_timing=True)
                 ev1 = torch.cuda.Event(enable_timing=True)
                 ev0.record()
@@ -183,9 +191,14 @@ class PDRSolverGPU:
                 self.y, self.density.reshape(-1), self.T_gas.reshape(-1), self.G0.reshape(-1),
                 self.A_V.reshape(-1), self.zeta_CR.reshape(-1), self.fsh_H2.reshape(-1),
                 self.fsh_CO.reshape(-1), opts=o, sync=False)
-            if self.record_chem:
+            if timed:
                 ev1.record()
+            if self.record_chem:
                 self.chem_records.append((ev0, ev1, self.chem_stats))
+            if self.tuner is not None and self.tuner.tuning:
+                ev1.synchronize()
+                self.tuner.record(cfg, ev0.elapsed_time(ev1) * 1e-3,
+                                  int(self.chem_stats[:, 0].sum().item()))
             return
         k = grid.euler_rates(self.density, self.G0, self.A_V, self.T_gas, self.zeta_CR,
                              self.fsh_H2, self.fsh_CO, self._R_H2)
@@ -233,8 +246,9 @@ class PDRSolverGPU:
         self._set_row('x_e', torch.clamp(self.x_Cp + self.x_HCOp, min=1e-10))
         self._set_row('x_OH', 1e-9 + 9e-8 * torch.clamp(f_h2, 0, 1))
 
-    def iterate(self, iteration, w_damp, first=False):
-        """One inner iteration; returns the convergence record."""
+    def _advance(self, iteration, w_damp, first):
+        """Device work of one inner iteration (solver_3d.py:262-380); returns the
+        previous T, x_H2, x_CO and the new damping weights."""
         old_T = self.T_gas.clone()
         old_H2 = self.x_H2.clone()
         old_CO = self.x_CO.clone()
@@ -259,8 +273,35 @@ class PDRSolverGPU:
                                                  torch.where(rel < 0.1, w_damp - 0.02, w_damp)),
                                      0.5, 0.98)
                 self.T_gas = w_damp * old_T + (1.0 - w_damp) * self.T_gas
-        rec = self._convergence(old_T, old_H2, old_CO)
-        return rec, w_damp
+        return old_T, old_H2, old_CO, w_damp
+
+    def iterate(self, iteration, w_damp, first=False):
+        """One inner iteration; returns the convergence record."""
+        old_T, old_H2, old_CO, w_damp = self._advance(iteration, w_damp, first)
+        return self._convergence(old_T, old_H2, old_CO), w_damp
+
+    def _capture(self, w_damp):
+        """Records one (non-first) inner iteration as a HIP graph.  The state
+        carried between iterations -- the abundances (updated in place), T_gas
+        and the damping weights -- lives in fixed buffers, so every replay
+        advances the solver by one iteration; the returned previous-state
+        tensors feed the host-side convergence test after each replay."""
+        if self.record_chem or self.record_inputs is not None:
+            raise ValueError("chemistry timing / input recording cannot run inside a graph")
+        if self.tuner is not None and self.tuner.tuning:
+            raise ValueError("finish autotuning before capturing the iteration")
+        self._T_state = self.T_gas.clone()
+        self._w_state = w_damp.clone()
+        self.T_gas = self._T_state
+        torch.cuda.synchronize(self.dev)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            old_T, old_H2, old_CO, w_new = self._advance(1, self._w_state, False)
+            self._T_state.copy_(self.T_gas)
+            self._w_state.copy_(w_new)
+        self.T_gas = self._T_state
+        self._graph = g
+        self._graph_prev = (old_T, old_H2, old_CO)
 
     def _convergence(self, old_T, old_H2, old_CO):
         """p95 of |dT|/T, p99 of |dH2| and |dCO| (solver_3d.py:381-406)."""
@@ -278,14 +319,26 @@ class PDRSolverGPU:
         rec['delta_max'] = max(rec['dT'], rec['dH2'], rec['dCO'])
         return rec
 
-    def run(self, max_iterations=20, convergence_tol=0.05, dust_steps=1, verbose=False):
-        """Inner fixed-point loop (solver_3d.py:197-463, one dust step)."""
+    def run(self, max_iterations=20, convergence_tol=0.05, dust_steps=1, verbose=False,
+            graph=False):
+        """Inner fixed-point loop (solver_3d.py:197-463, one dust step).  With
+        graph=True, iterations from the third on replay one captured HIP graph
+        of the iteration (the first two run eagerly, which also sizes every
+        library workspace before capture)."""
         if dust_steps != 1:
             raise NotImplementedError("the outer dust-evolution loop is not part of this build")
         w_damp = torch.full(self.shape, 0.5, dtype=torch.float64, device=self.dev)
+        self._graph = None
         converged = False
         for it in range(max_iterations):
-            rec, w_damp = self.iterate(it, w_damp, first=(it == 0))
+            if graph and it >= 2:
+                if self._graph is None:
+                    self._capture(w_damp)
+                self._graph.replay()
+                rec = self._convergence(*self._graph_prev)
+                w_damp = self._w_state
+            else:
+                rec, w_damp = self.iterate(it, w_damp, first=(it == 0))
             rec['iteration'] = it
             self.convergence_history.append(rec)
             if verbose:

@@ -104,3 +104,26 @@ def test_lane_matches_workgroup_kernel_on_mixed_grid(gpu):
     assert np.all(sb[:, 0] > 0)
     assert np.percentile(rel, 99.9) < 1e-6
     assert rel.max() < 1e-4
+
+
+def test_driver_autotune_picks_a_config_and_keeps_results(gpu, tmp_path, monkeypatch):
+    """16^3 slab, network chemistry, 7 fixed-point iterations with the online
+    tuner (warm-up + 5 candidates + tuned): a configuration is chosen, and the
+    state matches the untuned run to the solver tolerance."""
+    from oracle import oracle as O
+    from prism3d_amd.solver3d import PDRSolverGPU
+    monkeypatch.setenv("P3D_TUNE_CACHE", str(tmp_path / "tune.json"))
+    print("case: driver autotune on 16^3", flush=True)
+    dens, cell = O.uniform_slab(16, 1.0 * O.PC, 1e3)
+    runs = []
+    for tune in (False, True):
+        sv = PDRSolverGPU(dens, 16 * cell, G0_external=10.0, zeta_CR_0=5e-17, nside_rt=1,
+                          chemistry='network', chem_autotune=tune)
+        sv.run(max_iterations=7, convergence_tol=0.0)
+        runs.append(sv)
+    a, b = runs
+    assert b.tuner is not None and not b.tuner.tuning and len(b.tuner.scores) == 5
+    for k in ('T_gas', 'x_H2', 'x_CO', 'x_e'):
+        x, y = getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy()
+        m = np.abs(x) > 1e-14
+        assert np.max(np.abs(y[m] - x[m]) / np.abs(x[m])) < 1e-5, k
