@@ -24,9 +24,11 @@
  *   p3d_network_* / p3d_chem_* prism3d/chemistry/solver.py:69-225
  *                              (ChemistrySolver._rhs, solve_steady_state)
  *   p3d_fuv_columns            prism3d/radiative_transfer/fuv_rt_3d.py:215-392
+ *   p3d_fuv_point_source       prism3d/radiative_transfer/fuv_rt_3d.py:74-212
  *   p3d_shield_cr              prism3d/radiative_transfer/shielding.py:17-171,
  *                              prism3d/cosmic_rays/attenuation.py:18-113
  *   p3d_dust_state             prism3d/grains/themis_tables.py:278-354
+ *   p3d_dust_evolve            prism3d/solver_3d.py:465-507 (_evolve_dust_state)
  *   p3d_euler_rates            prism3d/solver_3d.py:549-557
  *   p3d_thermal_ctx            prism3d/solver_3d.py:693-731
  *   p3d_heating_cooling        prism3d/solver_3d.py:898-1050
@@ -221,6 +223,29 @@ int p3d_dust_state(int64_t n, const double *g0, const double *a_v,
                    const double *n_h, const double *t_gas, const double *f_nano,
                    double sigma_nano, double sigma_large_c, double sigma_sil,
                    double *t_dust, double *r_h2, void *stream);
+
+/* FUV field of an embedded point source (e.g. an O star) of luminosity l_fuv
+ * [erg/s] at host star_pos[3] [cm] on an (nx, ny, nz) C-order grid: columns
+ * summed from the star's plane along each cell's dominant axis (largest
+ * |displacement|, ties to x then y), scaled by the oblique-path factor
+ * min(r / max(|d_dom|, cell/2), 3) less half the cell's own layer;
+ * A_V = N_H * av_per_nh, G0 = l_fuv / (4 pi r^2 g0_to_flux) exp(-1.8 A_V)
+ * with r >= cell/2.  x_h2 / x_co may be NULL (zero abundances).  No
+ * workspace.  Replaces radiative_transfer/fuv_rt_3d.py:74-212
+ * (compute_fuv_field_point_source). */
+int p3d_fuv_point_source(int64_t nx, int64_t ny, int64_t nz, const double *density,
+                         const double *x_h2, const double *x_co,
+                         const double *star_pos, double l_fuv, double cell_size,
+                         double av_per_nh, double g0_to_flux, double *g0,
+                         double *a_v, double *n_h, double *n_h2, double *n_co,
+                         void *stream);
+
+/* One outer dust step (10 kyr) of the grain state, in place: nano-grain
+ * fraction f_nano (photo-destruction, replenishment, accretion; clipped to
+ * [0.01, 2]) and band gap e_g (relaxation toward its equilibrium; clipped to
+ * [0.01, 2.5]).  Replaces solver_3d.py:465-507 (_evolve_dust_state). */
+int p3d_dust_evolve(int64_t n, const double *g0, const double *n_h,
+                    double *f_nano, double *e_g, void *stream);
 
 /* Rate precompute handed to the explicit chemistry (solver_3d.py:549-557). */
 int p3d_euler_rates(int64_t n, const double *n_h, const double *g0,

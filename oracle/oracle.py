@@ -363,6 +363,77 @@ def fuv_field(density, G0_ext, cell, xH2, xCO, nside=1, av_per_nh=AV_per_NH):
     return G0, AVm, NHb, NH2b, NCOb
 
 
+G0_to_flux = 1.6e-3  # constants.py:45
+
+
+def _star_cell(pos, cell, shape):
+    """Per axis int(clip(pos / cell, 0, n - 1)) (fuv_rt_3d.py:124-126)."""
+    return [int(min(max(p / cell, 0.0), n - 1)) for p, n in zip(pos, shape)]
+
+
+def _columns_from_star_plane(layer, axis, s):
+    """Column of every cell along `axis`: the layers between the star's plane
+    s (included) and the cell (excluded), added one layer at a time outward
+    from the plane (the order of fuv_rt_3d.py:140-181's cumulative sums).
+    Cells in the star's plane get 0."""
+    lay = np.moveaxis(layer, axis, 0)
+    col = np.zeros_like(lay)
+    acc = np.zeros_like(lay[0])
+    for k in range(s + 1, lay.shape[0]):
+        acc = acc + lay[k - 1]
+        col[k] = acc
+    acc = np.zeros_like(lay[0])
+    for k in range(s - 1, -1, -1):
+        acc = acc + lay[k + 1]
+        col[k] = acc
+    return np.moveaxis(col, 0, axis)
+
+
+def fuv_point_source(density, star_pos, L_FUV, cell, xH2=None, xCO=None,
+                     av_per_nh=AV_per_NH):
+    """FUV field of an embedded point source, fuv_rt_3d.py:74-212: columns
+    from the star's plane along each cell's dominant axis (ties to x, then
+    y), oblique-path factor min(r / max(|d_dom|, cell/2), 3), half-cell
+    midpoint correction, G0 = L / (4 pi r^2 G0_to_flux) exp(-1.8 A_V)."""
+    density = np.asarray(density, dtype=np.float64)
+    xH2 = np.zeros_like(density) if xH2 is None else np.asarray(xH2, dtype=np.float64)
+    xCO = np.zeros_like(density) if xCO is None else np.asarray(xCO, dtype=np.float64)
+    pos = np.asarray(star_pos, dtype=np.float64)
+    s = _star_cell(pos, cell, density.shape)
+    idx = np.indices(density.shape, dtype=np.float64)
+    d = [(idx[a] + 0.5) * cell - pos[a] for a in range(3)]
+    r = np.maximum(np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2), 0.5 * cell)
+    ad = [np.abs(x) for x in d]
+    dom = np.where((ad[0] >= ad[1]) & (ad[0] >= ad[2]), 0,
+                   np.where((ad[1] > ad[0]) & (ad[1] >= ad[2]), 1, 2))
+    ddom = np.choose(dom, ad)
+    pc = np.minimum(r / np.maximum(ddom, 0.5 * cell), 3.0)
+    out = []
+    for lay in (density * cell, density * xH2 * 2.0 * cell, density * xCO * cell):
+        cols = [_columns_from_star_plane(lay, a, s[a]) for a in range(3)]
+        sel = np.choose(dom, cols)
+        out.append(np.maximum(sel * pc - 0.5 * lay * pc, 0.0))
+    NH, NH2, NCO = out
+    AV = NH * av_per_nh
+    G0 = L_FUV / (4.0 * np.pi * r ** 2 * G0_to_flux) * np.exp(-1.8 * AV)
+    return G0, AV, NH, NH2, NCO
+
+
+def evolve_dust_state(G0, nH, f_nano, E_g):
+    """One 10 kyr outer dust step of the grain state, solver_3d.py:465-507."""
+    yr = 3.15e7
+    dt = 1e4 * yr
+    G0, nH = np.asarray(G0, dtype=np.float64), np.asarray(nH, dtype=np.float64)
+    f_dest = np.exp(-dt / (1e4 * yr / np.maximum(G0 / 1e4, 0.01)))
+    f_rep = min(dt / (1e6 * yr), 0.1)
+    f_acc = np.where((G0 < 1) & (nH > 1e4), np.minimum(dt / (1e5 * yr * 1e4 / nH), 0.1), 0.0)
+    f_new = np.clip(f_nano * f_dest + (1.0 - f_nano) * f_rep + f_acc, 0.01, 2.0)
+    tau_ar = 1e3 * yr / np.maximum(G0, 0.01)
+    eg_eq = np.where(G0 > 10, 0.01, np.minimum(0.1 + 0.5 * np.log10(np.maximum(nH, 1.0)), 2.5))
+    e_new = np.clip(E_g + (eg_eq - E_g) * np.minimum(dt / tau_ar, 1.0), 0.01, 2.5)
+    return f_new, e_new
+
+
 def f_shield_H2(NH2, T=100.0):
     """shielding.py:17-59 (Draine & Bertoldi 1996 eq. 37)."""
     b = np.sqrt(2.0 * k_boltz * T / m_H2) / 1e5
@@ -608,10 +679,15 @@ def analytical_ic(AV, G0, x_c_total=GAS_C, x_o_total=GAS_O, x_HCOp=1e-10):
 
 
 def run_loop(density, box, G0_external, zeta0, nside, iterations, x_c_total=GAS_C,
-             x_o_total=GAS_O):
-    """The upstream inner loop (solver_3d.py:262-442) with the native-core
-    Euler chemistry, restated from the pinned pieces above; returns the final
-    state dict and the per-iteration convergence figures."""
+             x_o_total=GAS_O, dust_steps=1, star=None):
+    """The upstream run() (solver_3d.py:197-463): outer dust steps
+    (solver_3d.py:238-256; the grain state evolves before every step but the
+    first, and each step restarts the damping weights and the iteration
+    count) around the inner loop (solver_3d.py:262-442) with the native-core
+    Euler chemistry, restated from the pinned pieces above.  `star`
+    ({'position', 'L_FUV'}) replaces the distant field by a point source
+    (solver_3d.py:271-277).  Returns the final state dict and the
+    per-iteration convergence figures."""
     shape = density.shape
     full = lambda v: np.full(shape, float(v))
     st = dict(T_gas=full(50.0), x_HI=full(0.5), x_H2=full(0.25), x_Cp=full(x_c_total * 0.5),
@@ -621,15 +697,23 @@ def run_loop(density, box, G0_external, zeta0, nside, iterations, x_c_total=GAS_
               E_g=full(0.1))
     pe = pe_table()
     cell = box / shape[0]
-    w_damp = np.full(shape, 0.5)
     hist = []
     flat = lambda a: np.ascontiguousarray(a, dtype=np.float64).ravel()
-    for it in range(iterations):
-        old_T, old_H2, old_CO = st['T_gas'].copy(), st['x_H2'].copy(), st['x_CO'].copy()
-        G0, AV, NH, NH2, NCO = fuv_field(density, G0_external, cell, st['x_H2'], st['x_CO'],
-                                         nside=nside)
-        st.update(G0=G0, A_V=AV, N_H=NH, N_H2=NH2, N_CO=NCO)
+    for ds, it in ((ds, it) for ds in range(dust_steps) for it in range(iterations)):
         if it == 0:
+            if ds > 0:
+                st['f_nano'], st['E_g'] = evolve_dust_state(st['G0'], density, st['f_nano'],
+                                                            st['E_g'])
+            w_damp = np.full(shape, 0.5)
+        old_T, old_H2, old_CO = st['T_gas'].copy(), st['x_H2'].copy(), st['x_CO'].copy()
+        if star is not None:
+            G0, AV, NH, NH2, NCO = fuv_point_source(density, star['position'], star['L_FUV'],
+                                                    cell, st['x_H2'], st['x_CO'])
+        else:
+            G0, AV, NH, NH2, NCO = fuv_field(density, G0_external, cell, st['x_H2'],
+                                             st['x_CO'], nside=nside)
+        st.update(G0=G0, A_V=AV, N_H=NH, N_H2=NH2, N_CO=NCO)
+        if ds == 0 and it == 0:
             st.update(analytical_ic(AV, G0, x_c_total, x_o_total, st['x_HCOp']))
         fH2 = f_shield_H2(NH2, T=np.mean(st['T_gas']))
         fCO = f_shield_CO(NCO, NH2)
@@ -668,8 +752,10 @@ def run_loop(density, box, G0_external, zeta0, nside, iterations, x_c_total=GAS_
         relH = np.abs(st['x_H2'][mH] - old_H2[mH]) / np.maximum(old_H2[mH], 0.01)
         mC = old_CO > 1e-8
         relC = np.abs(st['x_CO'][mC] - old_CO[mC]) / np.maximum(old_CO[mC], 1e-6)
-        hist.append(dict(dT=float(np.percentile(relT, 95)) if relT.size else 0.0,
+        hist.append(dict(dust_step=ds, iteration=it,
+                         dT=float(np.percentile(relT, 95)) if relT.size else 0.0,
                          dH2=float(np.percentile(relH, 99)) if relH.size else 0.0,
-                         dCO=float(np.percentile(relC, 99)) if relC.size else 0.0))
+                         dCO=float(np.percentile(relC, 99)) if relC.size else 0.0,
+                         f_nano_mean=float(np.mean(st['f_nano']))))
         st.update(T_dust=Td, Gamma_PE=gpe, R_H2=RH2, fsh_H2=fH2, fsh_CO=fCO, zeta_CR=zeta)
     return st, hist

@@ -49,6 +49,9 @@ SIGNATURES = {
                                                         _VP]),
     "p3d_fuv_columns": (_i32, [_i64, _i64, _i64, _VP, _VP, _VP, _dbl, _i32, _D, _D, _dbl]
                         + [_VP] * 5 + [_VP, _VP]),
+    "p3d_fuv_point_source": (_i32, [_i64, _i64, _i64, _VP, _VP, _VP, _D, _dbl, _dbl, _dbl, _dbl]
+                             + [_VP] * 5 + [_VP]),
+    "p3d_dust_evolve": (_i32, [_i64, _VP, _VP, _VP, _VP, _VP]),
     "p3d_mean": (_i32, [_i64, _VP, _VP, _VP]),
     "p3d_shield_cr": (_i32, [_i64, _VP, _VP, _VP, _VP, _dbl, _VP, _VP, _VP, _VP]),
     "p3d_dust_state": (_i32, [_i64, _VP, _VP, _VP, _VP, _VP, _dbl, _dbl, _dbl, _VP, _VP, _VP]),

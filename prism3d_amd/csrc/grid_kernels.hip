@@ -1,13 +1,16 @@
-// Whole-grid kernels around the per-cell solve: multi-ray FUV columns,
-// self-shielding + cosmic-ray attenuation, dust state, the explicit-chemistry
-// rate precompute, the thermal context and the heating/cooling diagnostics.
+// Whole-grid kernels around the per-cell solve: multi-ray and point-source
+// FUV columns, self-shielding + cosmic-ray attenuation, dust state and its
+// slow evolution, the explicit-chemistry rate precompute, the thermal context
+// and the heating/cooling diagnostics.
 // All are HBM-streaming kernels over structure-of-arrays float64 buffers.
 //
 // Upstream (paths relative to the upstream repository):
 //   columns      prism3d/radiative_transfer/fuv_rt_3d.py:215-392, fuv_rt.py:19-57
+//   point source prism3d/radiative_transfer/fuv_rt_3d.py:74-212
 //   shielding    prism3d/radiative_transfer/shielding.py:17-59, 136-171
 //   cosmic rays  prism3d/cosmic_rays/attenuation.py:18-113 (model 'M')
 //   dust state   prism3d/grains/themis_tables.py:278-354
+//   dust evolve  prism3d/solver_3d.py:465-507
 //   euler rates  prism3d/solver_3d.py:549-557
 //   thermal ctx  prism3d/solver_3d.py:693-731
 //   diagnostics  prism3d/solver_3d.py:898-1050
@@ -368,6 +371,150 @@ heat_cool_kernel(int64_t n, const double *__restrict__ tg, const double *__restr
   lam[i] = L;
 }
 
+// ===========================================================================
+// Point-source columns (fuv_rt_3d.py:74-212).  The star's cell is
+// (position / cell) clipped to the grid and truncated, per axis.  Along an
+// axis, a cell's column is the sum of the layers between the star's plane
+// (included) and the cell (excluded), accumulated outward from the star's
+// plane, which is the upstream cumulative-sum order.  A cell uses only the
+// column along its dominant axis (largest |displacement| from the star; ties
+// go to x, then y), so one thread per (axis, line, direction) walks outward
+// from the star's plane carrying the running sums and writes the final G0,
+// A_V and columns of exactly the cells whose dominant axis it walks.  Every
+// cell is written once; every cell is read by three walks (one per axis).
+// Arithmetic is uncontracted so that sums and products round as upstream.
+// ===========================================================================
+struct PointSrc {
+  double sx, sy, sz;         // star position [cm]
+  double cell, l_fuv;        // cell edge [cm], FUV luminosity [erg/s]
+  double av_per_nh, g0_to_flux;
+  int64_t s[3];              // star cell per axis
+};
+
+// dominant axis of a displacement, upstream tie rule
+__device__ __forceinline__ int ps_dominant(double ax, double ay, double az) {
+  if (ax >= ay && ax >= az) return 0;
+  if (ay > ax && ay >= az) return 1;
+  return 2;
+}
+
+__device__ __forceinline__ void ps_finish(const PointSrc &P, double dx, double dy, double dz,
+                                          double ddom, double c0, double c1, double c2,
+                                          double l0, double l1, double l2, int64_t i,
+                                          double *__restrict__ g0, double *__restrict__ av,
+                                          double *__restrict__ nh, double *__restrict__ nh2,
+                                          double *__restrict__ nco) {
+#pragma clang fp contract(off)
+  const double half = 0.5 * P.cell;
+  const double r = fmax(sqrt(dx * dx + dy * dy + dz * dz), half);
+  const double pc = fmin(r / fmax(ddom, half), 3.0);
+  const double h = fmax(c0 * pc - 0.5 * l0 * pc, 0.0);
+  nh[i] = h;
+  nh2[i] = fmax(c1 * pc - 0.5 * l1 * pc, 0.0);
+  nco[i] = fmax(c2 * pc - 0.5 * l2 * pc, 0.0);
+  const double a = h * P.av_per_nh;
+  av[i] = a;
+  g0[i] = P.l_fuv / (4.0 * kPi * (r * r) * P.g0_to_flux) * exp(-1.8 * a);
+}
+
+__global__ void __launch_bounds__(256)
+point_source_kernel(int64_t nx, int64_t ny, int64_t nz, const double *__restrict__ dens,
+                    const double *__restrict__ xh2, const double *__restrict__ xco,
+                    const PointSrc P, double *__restrict__ g0, double *__restrict__ av,
+                    double *__restrict__ nh, double *__restrict__ nh2,
+                    double *__restrict__ nco) {
+#pragma clang fp contract(off)
+  // thread ranges: axis 0 lines (y, z), axis 1 lines (x, z), axis 2 lines (x, y), each twice
+  // (outward toward high indices, then toward low indices)
+  const int64_t n0 = ny * nz, n1 = nx * nz, n2 = nx * ny;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int axis;
+  int64_t nl;
+  if (t < 2 * n0) { axis = 0; nl = n0; }
+  else if ((t -= 2 * n0) < 2 * n1) { axis = 1; nl = n1; }
+  else if ((t -= 2 * n1) < 2 * n2) { axis = 2; nl = n2; }
+  else return;
+  const int up = t < nl ? 1 : 0;
+  const int64_t line = up ? t : t - nl;
+  int64_t len, stride, base, ix = 0, iy = 0, iz = 0;
+  if (axis == 0) {
+    len = nx; stride = ny * nz; base = line; iy = line / nz; iz = line - iy * nz;
+  } else if (axis == 1) {
+    ix = line / nz; iz = line - ix * nz;
+    len = ny; stride = nz; base = ix * ny * nz + iz;
+  } else {
+    ix = line / ny; iy = line - ix * ny;
+    len = nz; stride = 1; base = line * nz;
+  }
+  const int64_t s = P.s[axis];
+  // displacement components of the two fixed coordinates of this line
+  const double cell = P.cell;
+  const double dxf = ((double)ix + 0.5) * cell - P.sx;
+  const double dyf = ((double)iy + 0.5) * cell - P.sy;
+  const double dzf = ((double)iz + 0.5) * cell - P.sz;
+  const double sa = axis == 0 ? P.sx : (axis == 1 ? P.sy : P.sz);
+  double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+  auto layer = [&](int64_t i, double &l0, double &l1, double &l2) {
+#pragma clang fp contract(off)
+    const double d = dens[i];
+    l0 = d * cell;
+    l1 = xh2 ? d * xh2[i] * 2.0 * cell : 0.0 * cell;
+    l2 = xco ? d * xco[i] * cell : 0.0 * cell;
+  };
+  auto visit = [&](int64_t k, double l0, double l1, double l2) {
+#pragma clang fp contract(off)
+    const double dk = ((double)k + 0.5) * cell - sa;
+    const double dx = axis == 0 ? dk : dxf, dy = axis == 1 ? dk : dyf, dz = axis == 2 ? dk : dzf;
+    const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
+    if (ps_dominant(ax, ay, az) != axis) return;
+    ps_finish(P, dx, dy, dz, axis == 0 ? ax : (axis == 1 ? ay : az), c0, c1, c2, l0, l1, l2,
+              base + k * stride, g0, av, nh, nh2, nco);
+  };
+  if (up) {
+    for (int64_t k = s; k < len; ++k) {
+      double l0, l1, l2;
+      layer(base + k * stride, l0, l1, l2);
+      visit(k, l0, l1, l2);
+      c0 += l0; c1 += l1; c2 += l2;
+    }
+  } else if (s > 0) {
+    double p0, p1, p2;
+    layer(base + s * stride, p0, p1, p2);
+    for (int64_t k = s - 1; k >= 0; --k) {
+      c0 += p0; c1 += p1; c2 += p2;
+      double l0, l1, l2;
+      layer(base + k * stride, l0, l1, l2);
+      visit(k, l0, l1, l2);
+      p0 = l0; p1 = l1; p2 = l2;
+    }
+  }
+}
+
+// ===========================================================================
+// Slow grain evolution over one outer step of 10 kyr (solver_3d.py:465-507):
+// nano-grain photo-destruction, replenishment and accretion, and band-gap
+// relaxation toward its irradiation/density equilibrium.  In place on
+// f_nano and E_g.
+// ===========================================================================
+__global__ void __launch_bounds__(256)
+dust_evolve_kernel(int64_t n, const double *__restrict__ g0, const double *__restrict__ nh,
+                   double *__restrict__ f_nano, double *__restrict__ e_g) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double yr = 3.15e7;
+  const double dt = 1e4 * yr;
+  const double G = g0[i], n_h = nh[i], fn = f_nano[i], eg = e_g[i];
+  const double tau_dest = 1e4 * yr / fmax(G / 1e4, 0.01);
+  const double f_dest = exp(-dt / tau_dest);
+  const double f_rep = fmin(dt / (1e6 * yr), 0.1);
+  const double f_acc = (G < 1.0 && n_h > 1e4) ? fmin(dt / (1e5 * yr * 1e4 / n_h), 0.1) : 0.0;
+  f_nano[i] = fmin(fmax(fn * f_dest + (1.0 - fn) * f_rep + f_acc, 0.01), 2.0);
+  const double tau_ar = 1e3 * yr / fmax(G, 0.01);
+  const double eg_eq = G > 10.0 ? 0.01 : fmin(0.1 + 0.5 * log10(fmax(n_h, 1.0)), 2.5);
+  e_g[i] = fmin(fmax(eg + (eg_eq - eg) * fmin(dt / tau_ar, 1.0), 0.01), 2.5);
+}
+
 }  // namespace p3d
 
 using namespace p3d;
@@ -506,6 +653,43 @@ int p3d_heating_cooling(int64_t n, const double *t_gas, const double *n_h, const
                      n, t_gas, n_h, g0, a_v, zeta, fsh_h2, x_e, x_hi, x_h2, x_c, x_cp, x_o, x_co,
                      t_dust, gamma_pe, r_h2, Ls, gamma_out, lambda_out);
   return launch_status("heat_cool_kernel");
+}
+
+int p3d_fuv_point_source(int64_t nx, int64_t ny, int64_t nz, const double *density,
+                         const double *x_h2, const double *x_co, const double *star_pos,
+                         double l_fuv, double cell_size, double av_per_nh, double g0_to_flux,
+                         double *g0, double *a_v, double *n_h, double *n_h2, double *n_co,
+                         void *stream) {
+  P3D_REQUIRE(nx > 0 && ny > 0 && nz > 0, "grid extents must be > 0");
+  P3D_REQUIRE(star_pos, "star position (host, 3 doubles) is required");
+  P3D_REQUIRE(density && g0 && a_v && n_h && n_h2 && n_co, "null device array");
+  P3D_REQUIRE(cell_size > 0.0, "cell_size must be > 0");
+  P3D_REQUIRE(std::isfinite(star_pos[0]) && std::isfinite(star_pos[1]) &&
+                  std::isfinite(star_pos[2]),
+              "star position must be finite");
+  PointSrc P;
+  P.sx = star_pos[0]; P.sy = star_pos[1]; P.sz = star_pos[2];
+  P.cell = cell_size; P.l_fuv = l_fuv; P.av_per_nh = av_per_nh; P.g0_to_flux = g0_to_flux;
+  const int64_t ext[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a) {  // int(np.clip(pos / cell, 0, n - 1))
+    const double q = fmin(fmax(star_pos[a] / cell_size, 0.0), (double)(ext[a] - 1));
+    P.s[a] = (int64_t)q;
+  }
+  const int64_t threads = 2 * (ny * nz + nx * nz + nx * ny);
+  hipLaunchKernelGGL(point_source_kernel, dim3(grid_for(threads, 256)), dim3(256), 0,
+                     as_stream(stream), nx, ny, nz, density, x_h2, x_co, P, g0, a_v, n_h, n_h2,
+                     n_co);
+  return launch_status("point_source_kernel");
+}
+
+int p3d_dust_evolve(int64_t n, const double *g0, const double *n_h, double *f_nano, double *e_g,
+                    void *stream) {
+  P3D_REQUIRE(n >= 0, "n must be >= 0");
+  if (n == 0) return P3D_OK;
+  P3D_REQUIRE(g0 && n_h && f_nano && e_g, "null device array");
+  hipLaunchKernelGGL(dust_evolve_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream),
+                     n, g0, n_h, f_nano, e_g);
+  return launch_status("dust_evolve_kernel");
 }
 
 }  // extern "C"

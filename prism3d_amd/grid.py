@@ -12,6 +12,7 @@ from . import _lib
 from .device import device, stream_handle
 
 AV_PER_NH = 5.34e-22  # prism3d/utils/constants.py:52
+G0_TO_FLUX = 1.6e-3   # Habing flux [erg/cm^2/s], prism3d/utils/constants.py:45
 LINES = {  # (freq, A, T_ul, g_u, g_l) of prism3d/utils/constants.py:103-109
     "CII_158": (1.9005e12, 2.321e-6, 91.21, 4.0, 2.0),
     "OI_63": (4.7448e12, 8.910e-5, 227.71, 3.0, 5.0),
@@ -86,6 +87,48 @@ def fuv_columns(density, x_h2, x_co, cell_size, nside=1, G0_external=1.0,
                                     stream_handle())
     _lib.check(rc, "p3d_fuv_columns")
     return tuple(outs)
+
+
+def fuv_point_source(density, star_pos_cm, L_FUV, cell_size, x_h2=None, x_co=None,
+                     av_per_nh=AV_PER_NH, g0_to_flux=G0_TO_FLUX, out=None):
+    """G0, A_V, N_H, N_H2, N_CO of an embedded point source of FUV luminosity
+    L_FUV [erg/s] at star_pos_cm (fuv_rt_3d.py:74-212); x_h2 / x_co default
+    to zero abundances.  `out` may hold five preallocated result tensors."""
+    density = _t(density)
+    if density.ndim != 3:
+        raise ValueError("density must be a 3-D grid")
+    opt = []
+    for x in (x_h2, x_co):
+        if x is not None:
+            x = _t(x)
+            if x.shape != density.shape:
+                raise ValueError("x_h2 and x_co must match the density grid")
+        opt.append(x)
+    pos = np.ascontiguousarray(np.asarray(star_pos_cm, dtype=np.float64).reshape(-1))
+    if pos.size != 3:
+        raise ValueError("star position must have 3 components")
+    nx, ny, nz = density.shape
+    outs = list(out) if out is not None else _empty(density, 5)
+    rc = _lib.lib().p3d_fuv_point_source(
+        nx, ny, nz, density.data_ptr(), *[x.data_ptr() if x is not None else None for x in opt],
+        _lib.dptr(pos), float(L_FUV), float(cell_size), float(av_per_nh), float(g0_to_flux),
+        *[o.data_ptr() for o in outs], stream_handle())
+    _lib.check(rc, "p3d_fuv_point_source")
+    return tuple(outs)
+
+
+def dust_evolve(g0, n_h, f_nano, e_g):
+    """One outer dust step of f_nano and E_g, in place (solver_3d.py:465-507)."""
+    g0, n_h = _t(g0), _t(n_h)
+    for x in (f_nano, e_g):
+        if not (x.is_cuda and x.dtype == torch.float64 and x.is_contiguous()):
+            raise TypeError("f_nano and E_g must be contiguous float64 CUDA tensors")
+        if x.numel() != g0.numel():
+            raise ValueError("f_nano and E_g must match the grid")
+    rc = _lib.lib().p3d_dust_evolve(g0.numel(), g0.data_ptr(), n_h.data_ptr(), f_nano.data_ptr(),
+                                    e_g.data_ptr(), stream_handle())
+    _lib.check(rc, "p3d_dust_evolve")
+    return f_nano, e_g
 
 
 def mean(x, out=None):

@@ -1,15 +1,20 @@
 """Device-resident 3D driver: the RT -> chemistry -> thermal fixed point.
 
 ``PDRSolverGPU`` mirrors the call surface of the upstream
-``PDRSolver3D`` (prism3d/solver_3d.py:34-1190): constructor arguments,
-``run()``, ``refine()``, ``save()``/``load()`` and the per-quantity grid
-attributes.  All state lives on the GPU as float64 tensors; each inner
-iteration runs the gfx950 kernels in stream order:
+``PDRSolver3D`` (prism3d/solver_3d.py:34-1190): constructor arguments
+(including an embedded ``star``), ``run()`` with its outer dust steps,
+``save()``/``load()`` and the per-quantity grid attributes.  All state lives
+on the GPU as float64 tensors; each inner iteration runs the gfx950 kernels
+in stream order:
 
-  columns (fuv_rt_3d.py:215) -> shielding + cosmic rays (solver_3d.py:300-309)
+  columns (fuv_rt_3d.py:215, or the point source of fuv_rt_3d.py:74)
+  -> shielding + cosmic rays (solver_3d.py:300-309)
   -> PE heating, T_dust, R_H2 (solver_3d.py:509-530) -> chemistry
   -> thermal context + root-find (solver_3d.py:680-800) -> damping and the
   percentile convergence test (solver_3d.py:362-436).
+
+Between outer dust steps the grain state (f_nano, E_g) evolves in place on
+the device (solver_3d.py:238-256, 465-507).
 
 Chemistry: ``chemistry='euler'`` is the upstream default path (reduced
 7-variable explicit model, solver_3d.py:536-674); ``chemistry='network'``
@@ -37,13 +42,16 @@ YR = 3.15576e7
 
 
 class PDRSolverGPU:
-    """GPU twin of prism3d.solver_3d.PDRSolver3D (regular grid, distant source)."""
+    """GPU twin of prism3d.solver_3d.PDRSolver3D (regular grid; distant field
+    or an embedded point source given as star={'position': (x, y, z) [cm],
+    'L_FUV': erg/s})."""
 
     def __init__(self, density, box_size, G0_external=1.0, zeta_CR_0=2e-16, nside_rt=1,
                  fixed_T=None, chemistry='euler', x_c_total=GAS_C, x_o_total=GAS_O,
                  av_per_nh=grid.AV_PER_NH, t_chem=1e7 * YR, chem_rtol=1e-9, chem_atol=1e-20,
                  initial_abundances=None, T_init=50.0, chem_mode='steady', chem_variant=0,
-                 chem_waves_per_cu=0, chem_rescue_frac=0.0, chem_autotune=False):
+                 chem_waves_per_cu=0, chem_rescue_frac=0.0, chem_autotune=False, star=None,
+                 velocity_field=None):
         self.dev = device()
         dens = np.asarray(density, dtype=np.float64)
         self.shape = dens.shape
@@ -54,6 +62,14 @@ class PDRSolverGPU:
         self.G0_external = G0_external
         self.zeta_CR_0 = float(zeta_CR_0)
         self.nside_rt = int(nside_rt)
+        if star is not None:
+            pos = np.asarray(star['position'], dtype=np.float64).reshape(-1)
+            if pos.size != 3:
+                raise ValueError("star['position'] must have 3 components")
+            star = {'position': pos, 'L_FUV': float(star['L_FUV'])}
+        self.star = star
+        # carried through save()/load() as upstream; no kernel reads it
+        self.velocity_field = None if velocity_field is None else np.asarray(velocity_field)
         self.fixed_T = fixed_T
         self.chemistry = chemistry
         self.x_c_total, self.x_o_total = float(x_c_total), float(x_o_total)
@@ -134,10 +150,19 @@ class PDRSolverGPU:
 
     # --- one inner iteration ----------------------------------------------
     def rt_step(self):
-        G0, AV, NH, NH2, NCO = grid.fuv_columns(self.density, self.x_H2, self.x_CO,
-                                                self.cell_size, self.nside_rt, self.G0_external,
-                                                self.av_per_nh, workspace=self._ws)
-        self.G0, self.A_V, self.N_H, self.N_H2, self.N_CO = G0, AV, NH, NH2, NCO
+        if self.star is not None:  # solver_3d.py:271-277
+            out = grid.fuv_point_source(self.density, self.star['position'],
+                                        self.star['L_FUV'], self.cell_size, self.x_H2,
+                                        self.x_CO, self.av_per_nh)
+        else:
+            out = grid.fuv_columns(self.density, self.x_H2, self.x_CO, self.cell_size,
+                                   self.nside_rt, self.G0_external, self.av_per_nh,
+                                   workspace=self._ws)
+        self.G0, self.A_V, self.N_H, self.N_H2, self.N_CO = out
+
+    def evolve_dust(self):
+        """One outer dust step of f_nano and E_g, in place (solver_3d.py:465-507)."""
+        grid.dust_evolve(self.G0, self.density, self.f_nano, self.E_g)
 
     def shielding_cr_step(self):
         grid.mean(self.T_gas, out=self._tmean)
@@ -275,17 +300,18 @@ class PDRSolverGPU:
                 self.T_gas = w_damp * old_T + (1.0 - w_damp) * self.T_gas
         return old_T, old_H2, old_CO, w_damp
 
-    def iterate(self, iteration, w_damp, first=False):
+    def iterate(self, iteration, w_damp, first=False, convergence_tol=0.05):
         """One inner iteration; returns the convergence record."""
         old_T, old_H2, old_CO, w_damp = self._advance(iteration, w_damp, first)
-        return self._convergence(old_T, old_H2, old_CO), w_damp
+        return self._convergence(old_T, old_H2, old_CO, convergence_tol), w_damp
 
     def _capture(self, w_damp):
         """Records one (non-first) inner iteration as a HIP graph.  The state
-        carried between iterations -- the abundances (updated in place), T_gas
-        and the damping weights -- lives in fixed buffers, so every replay
-        advances the solver by one iteration; the returned previous-state
-        tensors feed the host-side convergence test after each replay."""
+        carried between iterations -- the abundances and the grain state
+        (updated in place), T_gas and the damping weights -- lives in fixed
+        buffers, so every replay advances the solver by one iteration; the
+        returned previous-state tensors feed the host-side convergence test
+        after each replay."""
         if self.record_chem or self.record_inputs is not None:
             raise ValueError("chemistry timing / input recording cannot run inside a graph")
         if self.tuner is not None and self.tuner.tuning:
@@ -302,51 +328,74 @@ class PDRSolverGPU:
         self.T_gas = self._T_state
         self._graph = g
         self._graph_prev = (old_T, old_H2, old_CO)
+        # the graph writes the tensors bound at capture; eager iterations
+        # rebind fresh ones, so every replay restores these bindings
+        self._graph_bind = {k: v for k, v in self.__dict__.items()
+                            if isinstance(v, torch.Tensor)}
 
-    def _convergence(self, old_T, old_H2, old_CO):
-        """p95 of |dT|/T, p99 of |dH2| and |dCO| (solver_3d.py:381-406)."""
+    def _convergence(self, old_T, old_H2, old_CO, convergence_tol=0.05):
+        """p95 (and p99, max) of |dT|/T, p99 of |dH2| and |dCO|, and the count
+        of cells whose |dT|/T exceeds the tolerance (solver_3d.py:381-432)."""
         def pct(x, q):
-            return float(np.percentile(x.cpu().numpy(), q)) if x.numel() else 0.0
+            return float(np.percentile(x, q)) if x.size else 0.0
         mT = old_T > 10
-        rel_T = (torch.abs(self.T_gas - old_T) / old_T)[mT]
+        rel_T = (torch.abs(self.T_gas - old_T) / old_T)[mT].cpu().numpy()
         mH = old_H2 > 1e-6
-        rel_H2 = (torch.abs(self.x_H2 - old_H2) / torch.clamp(old_H2, min=0.01))[mH]
+        rel_H2 = (torch.abs(self.x_H2 - old_H2) / torch.clamp(old_H2, min=0.01))[mH].cpu().numpy()
         mC = old_CO > 1e-8
-        rel_CO = (torch.abs(self.x_CO - old_CO) / torch.clamp(old_CO, min=1e-6))[mC]
-        dT = pct(rel_T, 95)
-        rec = {'dT': dT, 'dT_p99': pct(rel_T, 99), 'dH2': pct(rel_H2, 99),
+        rel_CO = (torch.abs(self.x_CO - old_CO) / torch.clamp(old_CO, min=1e-6))[mC].cpu().numpy()
+        rec = {'dT': pct(rel_T, 95), 'dT_p99': pct(rel_T, 99), 'dH2': pct(rel_H2, 99),
                'dCO': pct(rel_CO, 99)}
+        rec['dT_max'] = float(rel_T.max()) if rel_T.size else 0.0
+        rec['dH2_max'] = float(rel_H2.max()) if rel_H2.size else 0.0
+        rec['n_oscillating'] = int(np.sum(rel_T > convergence_tol))
         rec['delta_max'] = max(rec['dT'], rec['dH2'], rec['dCO'])
         return rec
 
     def run(self, max_iterations=20, convergence_tol=0.05, dust_steps=1, verbose=False,
             graph=False):
-        """Inner fixed-point loop (solver_3d.py:197-463, one dust step).  With
-        graph=True, iterations from the third on replay one captured HIP graph
-        of the iteration (the first two run eagerly, which also sizes every
-        library workspace before capture)."""
-        if dust_steps != 1:
-            raise NotImplementedError("the outer dust-evolution loop is not part of this build")
-        w_damp = torch.full(self.shape, 0.5, dtype=torch.float64, device=self.dev)
+        """The upstream run() (solver_3d.py:197-463): `dust_steps` outer steps,
+        the grain state evolving before each step but the first, each step an
+        inner fixed-point loop of at most `max_iterations` iterations that
+        restarts the damping weights and stops once the largest of the
+        convergence figures is below `convergence_tol` at the fourth
+        iteration or later.  Returns whether the last dust step converged.  With
+        graph=True, inner iterations from the third on replay one captured
+        HIP graph of the iteration (the first two of each dust step run
+        eagerly, which also sizes every library workspace before capture).
+        As upstream, dust_steps <= 0 runs nothing and returns False."""
         self._graph = None
         converged = False
-        for it in range(max_iterations):
-            if graph and it >= 2:
-                if self._graph is None:
-                    self._capture(w_damp)
-                self._graph.replay()
-                rec = self._convergence(*self._graph_prev)
-                w_damp = self._w_state
-            else:
-                rec, w_damp = self.iterate(it, w_damp, first=(it == 0))
-            rec['iteration'] = it
-            self.convergence_history.append(rec)
-            if verbose:
-                print(f"  iter {it:2d} dT={rec['dT']:.2e} dH2={rec['dH2']:.2e} "
-                      f"dCO={rec['dCO']:.2e}", file=sys.stderr, flush=True)
-            if rec['delta_max'] < convergence_tol and it > 2:
-                converged = True
-                break
+        for dust_step in range(int(dust_steps)):
+            if dust_step > 0:
+                self.evolve_dust()
+            converged = False
+            w_damp = torch.full(self.shape, 0.5, dtype=torch.float64, device=self.dev)
+            for it in range(max_iterations):
+                if graph and it >= 2:
+                    if self._graph is None:
+                        self._capture(w_damp)
+                    elif it == 2:  # a later dust step: load the graph's state buffers
+                        self._T_state.copy_(self.T_gas)
+                        self._w_state.copy_(w_damp)
+                        self.__dict__.update(self._graph_bind)
+                    self._graph.replay()
+                    rec = self._convergence(*self._graph_prev, convergence_tol)
+                    w_damp = self._w_state
+                else:
+                    rec, w_damp = self.iterate(it, w_damp, first=(dust_step == 0 and it == 0),
+                                               convergence_tol=convergence_tol)
+                rec['dust_step'] = dust_step
+                rec['iteration'] = it
+                rec['f_nano_mean'] = float(self.f_nano.mean())
+                self.convergence_history.append(rec)
+                if verbose:
+                    print(f"  {dust_step}.{it:2d} dT={rec['dT']:.2e} dH2={rec['dH2']:.2e} "
+                          f"dCO={rec['dCO']:.2e} n_osc={rec['n_oscillating']}",
+                          file=sys.stderr, flush=True)
+                if rec['delta_max'] < convergence_tol and it > 2:
+                    converged = True
+                    break
         return converged
 
     # --- I/O (solver_3d.py:1150-1190) ---------------------------------------
@@ -355,22 +404,31 @@ class PDRSolverGPU:
             'Gamma_PE', 'f_nano', 'E_g']
 
     def save(self, filepath):
+        """npz archive with the upstream keys (solver_3d.py:1150-1170)."""
         d = {k: getattr(self, k).cpu().numpy() for k in self.KEYS}
         d.update(density=self.density.cpu().numpy(), box_size=self.box_size,
                  G0_external=self.G0_external, zeta_CR_0=self.zeta_CR_0)
+        if self.velocity_field is not None:
+            d['velocity_field'] = self.velocity_field
         np.savez_compressed(filepath, **d)
 
     @classmethod
     def load(cls, filepath, **kw):
-        data = np.load(filepath, allow_pickle=False)
-        s = cls(data['density'], float(data['box_size']), G0_external=data['G0_external'],
-                zeta_CR_0=float(data['zeta_CR_0']), **kw)
-        for k in cls.KEYS:
-            if k not in data:
-                continue
-            v = torch.tensor(data[k].astype(np.float64), device=s.dev)
-            if k in _GRID_SPECIES:
-                s._set_row(k, v)
-            else:
-                setattr(s, k, v)
+        """Solver from an archive written by save() or by the upstream
+        PDRSolver3D.save (solver_3d.py:1172-1190); missing state keys keep
+        their initial values."""
+        with np.load(filepath, allow_pickle=False) as data:
+            vel = data['velocity_field'] if 'velocity_field' in data else None
+            g0e = data['G0_external']
+            s = cls(data['density'], float(data['box_size']),
+                    G0_external=float(g0e) if g0e.size == 1 else g0e,
+                    zeta_CR_0=float(data['zeta_CR_0']), velocity_field=vel, **kw)
+            for k in cls.KEYS:
+                if k not in data:
+                    continue
+                v = torch.tensor(data[k].astype(np.float64), device=s.dev).reshape(s.shape)
+                if k in _GRID_SPECIES:
+                    s._set_row(k, v)
+                else:
+                    setattr(s, k, v.contiguous())
         return s

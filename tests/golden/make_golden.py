@@ -23,10 +23,12 @@ Upstream functions exercised (paths relative to the upstream repository):
                                   _prepare_thermal_ctx, _net_heating_ctx,
                                   _solve_thermal_full (numpy fallback),
                                   _total_heating_vec, _total_cooling_vec,
-                                  _set_analytical_ic, run
+                                  _set_analytical_ic, _evolve_dust_state,
+                                  run (one and two dust steps), save
   prism3d/grains/themis_tables.py pe table, pe_heating_vec (numpy fallback),
                                   h2_formation_rate_vec, dust_temperature_vec
   prism3d/radiative_transfer/*    compute_fuv_field_3d, directional_G0,
+                                  compute_fuv_field_point_source,
                                   healpix_directions, f_shield_H2, f_shield_CO
   prism3d/cosmic_rays/attenuation.py cr_ionization_rate
   prism3d/density_fields.py       fractal_turbulent, uniform_slab
@@ -51,7 +53,7 @@ from prism3d.cosmic_rays.attenuation import cr_ionization_rate  # noqa: E402
 from prism3d.density_fields import fractal_turbulent, uniform_slab  # noqa: E402
 from prism3d.radiative_transfer.fuv_rt import healpix_directions  # noqa: E402
 from prism3d.radiative_transfer.fuv_rt_3d import (  # noqa: E402
-    compute_fuv_field_3d, directional_G0)
+    compute_fuv_field_3d, compute_fuv_field_point_source, directional_G0)
 from prism3d.radiative_transfer.shielding import f_shield_CO, f_shield_H2  # noqa: E402
 from prism3d.utils import constants as C  # noqa: E402
 
@@ -446,6 +448,53 @@ def gen_run():
              'Lambda', 'Gamma_PE', '_R_H2']})
 
 
+def gen_next():
+    print("point-source RT, dust evolution, two-dust-step run, state archive")
+    out = {}
+    dens, cs = fractal_turbulent(16, 0.5 * PC, n_mean=1e4 * np.exp(0.5), sigma_ln=1.0,
+                                 spectral_index=-11.0 / 3.0, seed=7)
+    dens = np.ascontiguousarray(dens[:, :12, :10])  # non-cubic grid
+    rng = np.random.RandomState(11)
+    xH2 = rng.uniform(0, 0.5, dens.shape)
+    xCO = 10.0 ** rng.uniform(-10, -4, dens.shape)
+    out.update(ps_density=dens, ps_cell=cs, ps_xH2=xH2, ps_xCO=xCO, ps_L=1e38)
+    stars = {"in": [0.31, 0.55, 0.72], "edge": [0.02, 0.97, 0.5], "out": [-0.1, 0.4, 1.3]}
+    ext = np.array(dens.shape) * cs
+    for name, frac in stars.items():
+        pos = np.array(frac) * ext
+        r = compute_fuv_field_point_source(dens, pos, 1e38, cs, x_H2=xH2, x_CO=xCO)
+        out[f"ps_{name}_pos"] = pos
+        for k, v in zip(("G0", "AV", "NH", "NH2", "NCO"), r):
+            out[f"ps_{name}_{k}"] = v
+    # one outer dust step on fields spanning every branch
+    sv = solver_8cube()
+    n = sv.density.shape
+    G0 = 10.0 ** rng.uniform(-3, 5, n)
+    nH = 10.0 ** rng.uniform(1, 6, n)
+    f_n = rng.uniform(0.005, 2.5, n)
+    eg = rng.uniform(0.0, 3.0, n)
+    sv.G0, sv.density, sv.f_nano, sv.E_g = G0, nH, f_n.copy(), eg.copy()
+    sv._evolve_dust_state()
+    out.update(de_G0=G0, de_nH=nH, de_f_nano=f_n, de_E_g=eg, de_f_nano_out=sv.f_nano,
+               de_E_g_out=sv.E_g)
+    save("next_golden.npz", **out)
+
+    print("  two-dust-step run on the 8^3 fixture (3 inner iterations per step)")
+    box = 2.0 * PC
+    density, _ = fractal_turbulent(8, box, n_mean=500.0, sigma_ln=1.5, seed=42)
+    sv = s3d.PDRSolver3D(density, box, G0_external=100.0, zeta_CR_0=2e-16, nside_rt=1)
+    sv.run(max_iterations=3, convergence_tol=1e-9, dust_steps=2, verbose=False)
+    hist = sv.convergence_history
+    save("run2_golden.npz", density=density, box=box,
+         dT=[h['dT'] for h in hist], dH2=[h['dH2'] for h in hist],
+         dCO=[h['dCO'] for h in hist],
+         **{k: getattr(sv, k).ravel() for k in
+            ['T_gas', 'G0', 'A_V', 'x_H2', 'x_CO', 'x_e', 'f_nano', 'E_g', 'Gamma_PE']})
+    path = os.path.join(OUT, "state_archive_golden.npz")
+    sv.save(path)
+    print(f"  wrote state_archive_golden.npz: {os.path.getsize(path) / 1024:.1f} KiB")
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["rhs", "thermal", "rt", "run", "steady"]
     if "rhs" in which:
@@ -460,3 +509,5 @@ if __name__ == "__main__":
         gen_steady()
     elif "tight" in which:
         gen_tight()
+    if "next" in which:
+        gen_next()

@@ -309,3 +309,134 @@ def test_run_loop_against_upstream_trace():
     for k in ('T_gas', 'x_H2', 'x_CO'):
         rel = np.abs(st[k].ravel() - r[k]) / np.abs(r[k])
         assert np.median(rel) < 0.02, k
+
+
+# --- point-source RT, grain evolution, outer dust loop, state archive -------
+@pytest.mark.parametrize("star", ["in", "edge", "out"])
+def test_point_source_matches_upstream(star):
+    """compute_fuv_field_point_source on a non-cubic log-normal grid with the
+    star inside, near a corner face, and outside the grid (clipped cell)."""
+    g = golden("next_golden.npz")
+    out = O.fuv_point_source(g['ps_density'], g[f'ps_{star}_pos'], float(g['ps_L']),
+                             float(g['ps_cell']), g['ps_xH2'], g['ps_xCO'])
+    for k, name in enumerate(['G0', 'AV', 'NH', 'NH2', 'NCO']):
+        assert np.array_equal(out[k], g[f'ps_{star}_{name}']), name
+
+
+def test_point_source_closed_forms():
+    """Uniform density n, star at the centre of cell (4, 5, 6) of a 9x11x13
+    grid: on an axis through the star the column m cells away is
+    n cell (m - 1/2) and G0 = L / (4 pi (m cell)^2 F) exp(-1.8 A_V); the
+    star's own cell sees r = cell/2 and no column; off-axis cells whose
+    dominant axis is x have the column (|i - 4| - 1/2) n cell r / |dx|
+    (capped at 3x); mirror cells about the star agree."""
+    n, cell, L = 300.0, 1e17, 1e37
+    dens = np.full((9, 11, 13), n)
+    pos = (np.array([4, 5, 6]) + 0.5) * cell
+    G0, AV, NH, NH2, NCO = O.fuv_point_source(dens, pos, L, cell)
+    assert NH[4, 5, 6] == 0.0 and np.all(NH2 == 0.0) and np.all(NCO == 0.0)
+    assert G0[4, 5, 6] == pytest.approx(L / (4 * np.pi * (0.5 * cell) ** 2 * O.G0_to_flux),
+                                        rel=1e-15)
+    for m in range(1, 5):
+        for idx in ((4 + m, 5, 6), (4 - m, 5, 6), (4, 5 + m, 6), (4, 5, 6 - m)):
+            assert NH[idx] == pytest.approx(n * cell * (m - 0.5), rel=1e-14)
+            assert G0[idx] == pytest.approx(
+                L / (4 * np.pi * (m * cell) ** 2 * O.G0_to_flux)
+                * np.exp(-1.8 * n * cell * (m - 0.5) * O.AV_per_NH), rel=1e-13)
+    for (i, j, k) in ((7, 6, 5), (1, 3, 7), (8, 2, 3)):
+        dx, dy, dz = i - 4, j - 5, k - 6
+        r = np.sqrt(dx * dx + dy * dy + dz * dz)
+        pc = min(r / abs(dx), 3.0)
+        assert NH[i, j, k] == pytest.approx((abs(dx) - 0.5) * n * cell * pc, rel=1e-14)
+    assert np.allclose(NH[4 - 3, 5, 6], NH[4 + 3, 5, 6], rtol=1e-15)
+    assert np.allclose(G0[4, 5 - 4, 6], G0[4, 5 + 4, 6], rtol=1e-15)
+
+
+def test_point_source_tie_rule():
+    """|dx| = |dy| goes to x, |dy| = |dz| > |dx| to y: a dense cell planted on
+    one candidate path only tells the chosen axis apart."""
+    cell = 1.0
+    dens = np.full((7, 7, 7), 1.0)
+    dens[4, 5, 3] = 1000.0  # on the x path of cell (6, 5, 3), not on its y path
+    dens[3, 5, 4] = 1000.0  # on the z path of cell (3, 5, 6), not on its y path
+    pos = np.array([3.5, 2.5, 3.5])  # star cell (3, 2, 3)
+    _, _, NH, _, _ = O.fuv_point_source(dens, pos, 1.0, cell)
+    pc = np.sqrt(18.0) / 3.0
+    # cell (6, 5, 3): |dx| = |dy| = 3 -> x: layers (3..5, 5, 3)
+    assert NH[6, 5, 3] == pytest.approx(1002.0 * pc - 0.5 * pc, rel=1e-14)
+    # cell (3, 5, 6): |dy| = |dz| = 3 > |dx| = 0 -> y: layers (3, 2..4, 6)
+    assert NH[3, 5, 6] == pytest.approx(3.0 * pc - 0.5 * pc, rel=1e-14)
+
+
+def test_dust_evolution_matches_upstream():
+    g = golden("next_golden.npz")
+    f, e = O.evolve_dust_state(g['de_G0'], g['de_nH'], g['de_f_nano'], g['de_E_g'])
+    assert np.array_equal(f, g['de_f_nano_out'])
+    assert np.array_equal(e, g['de_E_g_out'])
+
+
+def test_dust_evolution_known_answers():
+    """Each branch by hand: dt = 10 kyr (3.15e11 s); destruction time
+    3.15e11 s / max(G0/1e4, 0.01); replenishment 0.01; accretion
+    min(dt nH / 3.15e16, 0.1) only for G0 < 1 and nH > 1e4; band gap relaxes
+    by the fraction min(dt max(G0, 0.01) / 3.15e10, 1) toward 0.01 (G0 > 10)
+    or min(0.1 + 0.5 log10 nH, 2.5)."""
+    G0 = np.array([1e4, 0.5, 0.5, 20.0, 1e-4])
+    nH = np.array([1e2, 1e5, 1e3, 1e2, 1e12])
+    fn = np.array([1.0, 0.5, 0.5, 1.0, 1.9])
+    eg = np.array([0.1, 0.3, 0.3, 2.0, 0.3])
+    f, e = O.evolve_dust_state(G0, nH, fn, eg)
+    assert f[0] == pytest.approx(np.exp(-1.0), rel=1e-14)
+    assert f[1] == pytest.approx(0.5 * np.exp(-0.01) + 0.5 * 0.01 + 0.1, rel=1e-14)
+    assert f[2] == pytest.approx(0.5 * np.exp(-0.01) + 0.5 * 0.01, rel=1e-14)
+    assert f[4] == pytest.approx(min(1.9 * np.exp(-0.01) - 0.9 * 0.01 + 0.1, 2.0), rel=1e-14)
+    # G0 > 10: full relaxation to 0.01
+    assert e[0] == pytest.approx(0.01, rel=1e-13) and e[3] == pytest.approx(0.01, rel=1e-13)
+    assert e[1] == pytest.approx(2.5, rel=1e-14)   # dt / tau = 5 -> equilibrium, capped 2.5
+    assert e[2] == pytest.approx(0.3 + (0.1 + 0.5 * 3.0 - 0.3), rel=1e-14)
+    assert e[4] == pytest.approx(0.3 + (2.5 - 0.3) * 0.1, rel=1e-14)  # G0 floor 0.01
+    f_lo, _ = O.evolve_dust_state(np.array([1e9]), np.array([10.0]), np.array([2.0]),
+                                  np.array([0.1]))
+    assert f_lo[0] == 0.01  # destroyed and clipped (1 - 2) * 0.01 < 0.01
+
+
+def test_two_dust_step_run_matches_upstream():
+    """run(dust_steps=2, max_iterations=3) on the 8^3 fixture: the grain state
+    after the outer step and the fields that depend only on it and on the
+    density match the upstream run exactly; the first iteration of each dust
+    step restarts the damping (dT of iteration 0 is large again); the
+    abundance / temperature state matches in distribution as for one dust
+    step (the upstream trace is its numpy explicit chemistry)."""
+    r = golden("run2_golden.npz")
+    st, hist = O.run_loop(r['density'].reshape(8, 8, 8), float(r['box']), 100.0, 2e-16, 1, 3,
+                          dust_steps=2)
+    assert [(h['dust_step'], h['iteration']) for h in hist] == \
+        [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (1, 2)]
+    for k in ('G0', 'A_V', 'f_nano', 'E_g'):
+        assert np.array_equal(st[k].ravel(), r[k]), k
+    assert hist[0]['dT'] == pytest.approx(r['dT'][0], rel=1e-12)
+    assert hist[0]['dH2'] == pytest.approx(r['dH2'][0], rel=1e-12)
+    for h, dT in zip(hist, r['dT']):
+        assert h['dT'] == pytest.approx(dT, rel=0.05)
+    assert hist[3]['dT'] > 2.0 * hist[2]['dT']  # the new dust step moves T again
+    for k in ('T_gas', 'x_H2', 'x_CO', 'x_e'):
+        rel = np.abs(st[k].ravel() - r[k]) / np.abs(r[k])
+        assert np.median(rel) < 0.02, k
+
+
+def test_state_archive_keys_match_upstream():
+    """The archive written by the upstream save() holds exactly the state
+    keys the GPU driver writes and reads, plus the grid metadata."""
+    import ast
+    import os
+    src = open(os.path.join(os.path.dirname(os.path.abspath(__file__)), os.pardir,
+                            "prism3d_amd", "solver3d.py")).read()
+    tree = ast.parse(src)
+    keys = None
+    for node in ast.walk(tree):
+        if isinstance(node, ast.Assign) and any(getattr(t, 'id', None) == 'KEYS'
+                                                for t in node.targets):
+            keys = [e.value for e in node.value.elts]
+    a = golden("state_archive_golden.npz")
+    assert set(a.files) == set(keys) | {'density', 'box_size', 'G0_external', 'zeta_CR_0'}
+    assert a['density'].shape == (8, 8, 8) and a['box_size'].shape == ()
