@@ -52,6 +52,11 @@ SIGNATURES = {
     "p3d_fuv_point_source": (_i32, [_i64, _i64, _i64, _VP, _VP, _VP, _D, _dbl, _dbl, _dbl, _dbl]
                              + [_VP] * 5 + [_VP]),
     "p3d_dust_evolve": (_i32, [_i64, _VP, _VP, _VP, _VP, _VP]),
+    "p3d_surrogate_create": (_i32, [_i32, _i32, _i32, _I32P, _D, _D, _D, _I32P,
+                                     ctypes.POINTER(ctypes.c_float), _D, _D, _D, _D, _D, _I64P]),
+    "p3d_surrogate_destroy": (_i32, [_i64]),
+    "p3d_surrogate_predict": (_i32, [_i64, _i64] + [_VP] * 7 + [_dbl, _dbl,
+                                     ctypes.POINTER(_VP), _VP]),
     "p3d_mean": (_i32, [_i64, _VP, _VP, _VP]),
     "p3d_shield_cr": (_i32, [_i64, _VP, _VP, _VP, _VP, _dbl, _VP, _VP, _VP, _VP]),
     "p3d_dust_state": (_i32, [_i64, _VP, _VP, _VP, _VP, _VP, _dbl, _dbl, _dbl, _VP, _VP, _VP]),

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libp3d_hip.so")
-SOURCES = ["core_kernels.hip", "network.hip", "grid_kernels.hip", "lane_kernels.hip"]
+SOURCES = ["core_kernels.hip", "network.hip", "grid_kernels.hip", "lane_kernels.hip",
+           "surrogate.hip"]
 ARCH = os.environ.get("P3D_OFFLOAD_ARCH", "gfx950")
 
 

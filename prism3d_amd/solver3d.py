@@ -2,8 +2,9 @@
 
 ``PDRSolverGPU`` mirrors the call surface of the upstream
 ``PDRSolver3D`` (prism3d/solver_3d.py:34-1190): constructor arguments
-(including an embedded ``star``), ``run()`` with its outer dust steps,
-``save()``/``load()`` and the per-quantity grid attributes.  All state lives
+(including an embedded ``star``), ``run()`` with its outer dust steps, the
+``refine()`` BDF pass, ``save()``/``load()`` and the per-quantity grid
+attributes.  All state lives
 on the GPU as float64 tensors; each inner iteration runs the gfx950 kernels
 in stream order:
 
@@ -127,6 +128,9 @@ class PDRSolverGPU:
         self._ws = torch.empty(18 * self.n_cells, dtype=torch.float64, device=self.dev)
         self._tmean = torch.empty(1, dtype=torch.float64, device=self.dev)
         self.chem = ChemistrySolver() if chemistry == 'network' else None
+        # an ML surrogate (surrogate.ChemistrySurrogate) set here replaces the
+        # chemistry of every inner iteration, as upstream (solver_3d.py:142-143)
+        self.accelerator = None
         self.chem_stats = None
         self.record_chem = False  # keep (start, end, stats) of each chemistry launch
         self.chem_records = []
@@ -185,6 +189,13 @@ class PDRSolverGPU:
                                                   self.pe.sigma_largeC, self.pe.sigma_Sil)
 
     def chemistry_step(self):
+        if self.accelerator is not None:  # solver_3d.py:319-322, accelerator.py:472-510
+            from .surrogate import SPECIES
+            self.accelerator.predict_into(
+                [self.y[self.sidx[s]] for s in SPECIES], self.density.reshape(-1),
+                self.T_gas.reshape(-1), self.G0.reshape(-1), self.A_V.reshape(-1),
+                self.zeta_CR.reshape(-1), self.fsh_H2.reshape(-1), self.fsh_CO.reshape(-1))
+            return
         if self.chemistry == 'network':
             cfg = (self.chem_variant, self.chem_waves_per_cu, self.chem_rescue_frac)
             if self.tuner is not None:
@@ -278,12 +289,24 @@ class PDRSolverGPU:
         old_H2 = self.x_H2.clone()
         old_CO = self.x_CO.clone()
         self.rt_step()
-        if first and self.chemistry == 'euler':
+        if first and (self.chemistry == 'euler' or self.accelerator is not None):
             self._set_analytical_ic()
         self.shielding_cr_step()
         self.dust_step()
         self.chemistry_step()
-        if iteration > 0 and self.chemistry == 'euler':
+        if iteration > 0 and self.accelerator is not None:
+            # the surrogate damps only oscillating cells (solver_3d.py:330-345)
+            osc = w_damp > 0.55
+            w = w_damp
+            x_h2 = torch.where(osc, w * old_H2 + (1 - w) * self.x_H2, self.x_H2)
+            self._set_row('x_H2', x_h2)
+            self._set_row('x_HI', torch.where(osc, torch.clamp(1.0 - 2.0 * x_h2, min=1e-30),
+                                              self.x_HI))
+            x_co = torch.where(osc, w * old_CO + (1 - w) * self.x_CO, self.x_CO)
+            self._set_row('x_CO', x_co)
+            self._set_row('x_Cp', torch.where(
+                osc, torch.clamp(self.x_c_total - x_co - self.x_C, min=1e-30), self.x_Cp))
+        elif iteration > 0 and self.chemistry == 'euler':
             # Euler mode damps the abundances 50/50 (solver_3d.py:345-354)
             self._set_row('x_H2', 0.5 * old_H2 + 0.5 * self.x_H2)
             self._set_row('x_HI', torch.clamp(1.0 - 2.0 * self.x_H2, min=1e-30))
@@ -397,6 +420,39 @@ class PDRSolverGPU:
                     converged = True
                     break
         return converged
+
+    # --- BDF refinement (solver_3d.py:1056-1144) -----------------------------
+    def refine(self, verbose=True, opts=None):
+        """BDF refinement pass on the current fixed point: every cell's
+        31-species network solved to steady state at the upstream
+        solve_steady_state settings, starting from the ten grid species (the
+        rest at the 1e-30 floor), with shielding from the current columns and
+        the mean T; then the thermal balance is solved again at the refined
+        abundances.  Returns the number of early-stopped (converged) cells."""
+        if self.chem is None:
+            self.chem = ChemistrySolver()
+        grid.mean(self.T_gas, out=self._tmean)
+        fsh_H2, fsh_CO, _ = grid.shield_cr(self.N_H2, self.N_CO, self.N_H, self._tmean,
+                                           self.zeta_CR_0)
+        y0 = torch.full_like(self.y, 1e-30)
+        rows = [self.sidx[s] for s in _GRID_SPECIES.values()]
+        y0[rows] = self.y[rows]
+        if opts is None:
+            opts = solve_opts(variant=self.chem_variant, waves_per_cu=self.chem_waves_per_cu,
+                              rescue_frac=self.chem_rescue_frac)
+        conv, self.chem_stats = self.chem.solve_steady_batch(
+            y0, self.density.reshape(-1), self.T_gas.reshape(-1), self.G0.reshape(-1),
+            self.A_V.reshape(-1), self.zeta_CR.reshape(-1), fsh_H2.reshape(-1),
+            fsh_CO.reshape(-1), opts=opts)
+        self.y.copy_(y0)
+        self.fsh_H2, self.fsh_CO = fsh_H2, fsh_CO
+        n_conv = int(conv.sum().item())
+        if verbose:
+            print(f"  refine: {n_conv}/{self.n_cells} cells converged", file=sys.stderr,
+                  flush=True)
+        if self.fixed_T is None:
+            self.thermal_step()
+        return n_conv
 
     # --- I/O (solver_3d.py:1150-1190) ---------------------------------------
     KEYS = ['T_gas', 'T_dust', 'G0', 'A_V', 'N_H', 'N_H2', 'N_CO', 'zeta_CR', 'x_HI', 'x_H2',

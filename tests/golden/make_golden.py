@@ -31,7 +31,9 @@ Upstream functions exercised (paths relative to the upstream repository):
                                   compute_fuv_field_point_source,
                                   healpix_directions, f_shield_H2, f_shield_CO
   prism3d/cosmic_rays/attenuation.py cr_ionization_rate
-  prism3d/density_fields.py       fractal_turbulent, uniform_slab
+  prism3d/density_fields.py       fractal_turbulent, uniform_slab, embedded_star_cloud
+  prism3d/chemistry/accelerator.py generate_training_data, ChemistryAccelerator.predict_batch
+                                  (with small GBRT / RF models fitted here), run() with it
 """
 
 import copy
@@ -495,6 +497,115 @@ def gen_next():
     print(f"  wrote state_archive_golden.npz: {os.path.getsize(path) / 1024:.1f} KiB")
 
 
+def _raw_ensemble(model):
+    """Node arrays of a fitted sklearn tree ensemble: per tree the
+    children/feature/threshold/leaf-value arrays, plus the ensemble's
+    prediction = (base + scale * sum of leaf values) / divisor."""
+    if hasattr(model, "init_"):  # gradient boosting
+        base = float(np.ravel(model.init_.predict(np.zeros((1, model.n_features_in_))))[0])
+        scale, div, trees = float(model.learning_rate), 1.0, list(model.estimators_[:, 0])
+    else:  # random forest
+        base, scale, div, trees = 0.0, 1.0, float(len(model.estimators_)), list(model.estimators_)
+    return base, scale, div, [t.tree_ for t in trees]
+
+
+def _pack(models):
+    left, right, feat, thr, val, off, meta = [], [], [], [], [], [0], []
+    for m in models:
+        base, scale, div, trees = _raw_ensemble(m)
+        meta.append((base, scale, div, len(trees)))
+        for t in trees:
+            left.append(t.children_left.astype(np.int32))
+            right.append(t.children_right.astype(np.int32))
+            feat.append(t.feature.astype(np.int32))
+            thr.append(t.threshold.astype(np.float64))
+            val.append(t.value[:, 0, 0].astype(np.float64))
+            off.append(off[-1] + t.node_count)
+    meta = np.array(meta)
+    return dict(left=np.concatenate(left), right=np.concatenate(right),
+                feature=np.concatenate(feat), threshold=np.concatenate(thr),
+                value=np.concatenate(val), tree_offset=np.array(off, dtype=np.int64),
+                base=meta[:, 0], scale=meta[:, 1], divisor=meta[:, 2],
+                n_trees=meta[:, 3].astype(np.int32))
+
+
+def gen_surrogate():
+    print("ML surrogate: ChemistryAccelerator.predict_batch with small GBRT / RF models")
+    from sklearn.ensemble import GradientBoostingRegressor, RandomForestRegressor
+    from sklearn.preprocessing import StandardScaler
+    from prism3d.chemistry import accelerator as A
+    X, Y, species = A.generate_training_data(n_samples=400, seed=5, verbose=False)
+    print(f"  {len(X)} training samples from the upstream generator")
+    sx, sy = StandardScaler().fit(X), StandardScaler().fit(Y)
+    Xs, Ys = sx.transform(X), sy.transform(Y)
+    out = dict(species=np.array(species), x_mean=sx.mean_, x_scale=sx.scale_, y_mean=sy.mean_,
+               y_scale=sy.scale_)
+    # inputs: the 8^3 fixture after one RT pass plus cells below every floor
+    base = solver_8cube()
+    fH2 = f_shield_H2(base.N_H2, T=np.mean(base.T_gas)).ravel()
+    fCO = f_shield_CO(base.N_CO, base.N_H2).ravel()
+    rng = np.random.RandomState(17)
+    n_extra = 64
+    ins = [base.density.ravel(), base.T_gas.ravel() * rng.uniform(0.2, 40, 512),
+           base.G0.ravel(), base.A_V.ravel(), base.zeta_CR.ravel(), fH2, fCO]
+    extra = [10 ** rng.uniform(-1, 7, n_extra), 10 ** rng.uniform(0, 4.5, n_extra),
+             10 ** rng.uniform(-4, 6, n_extra), rng.uniform(0, 30, n_extra),
+             10 ** rng.uniform(-19, -14, n_extra), 10 ** rng.uniform(-12, 0, n_extra),
+             10 ** rng.uniform(-12, 0, n_extra)]
+    extra[5][:8] = 0.0
+    extra[6][8:16] = 0.0
+    ins = [np.concatenate([a, b]) for a, b in zip(ins, extra)]
+    names = ['n_H', 'T', 'G0', 'A_V', 'zeta', 'fsh_H2', 'fsh_CO']
+    out.update({f"in_{k}": v for k, v in zip(names, ins)})
+    for kind in ("gbrt", "rf"):
+        acc = A.ChemistryAccelerator()
+        acc.scaler_X, acc.scaler_Y, acc.is_trained = sx, sy, True
+        for j, sp in enumerate(species):
+            if kind == "gbrt":
+                m = GradientBoostingRegressor(n_estimators=20, max_depth=4, learning_rate=0.1,
+                                              subsample=0.8, random_state=42)
+            else:
+                m = RandomForestRegressor(n_estimators=10, max_depth=5, random_state=42,
+                                          n_jobs=1)
+            acc.models[sp] = m.fit(Xs, Ys[:, j])
+        pk = _pack([acc.models[sp] for sp in species])
+        out.update({f"{kind}_{k}": v for k, v in pk.items()})
+        feats = np.column_stack([np.log10(np.maximum(ins[0], 1)), np.log10(np.maximum(ins[1], 5)),
+                                 np.log10(np.maximum(ins[2], 1e-2)), ins[3],
+                                 np.log10(np.maximum(ins[4], 1e-18)),
+                                 np.log10(np.maximum(ins[5], 1e-10)),
+                                 np.log10(np.maximum(ins[6], 1e-10))])
+        out[f"{kind}_raw"] = np.stack([acc.models[sp].predict(sx.transform(feats))
+                                       for sp in species])
+        res = acc.predict_batch(*ins)
+        out[f"{kind}_out"] = np.stack([res[sp] for sp in species])
+        # the upstream run() with this accelerator on the 8^3 fixture (3 iterations)
+        if kind == "gbrt":
+            box = 2.0 * PC
+            density, _ = fractal_turbulent(8, box, n_mean=500.0, sigma_ln=1.5, seed=42)
+            sv = s3d.PDRSolver3D(density, box, G0_external=100.0, zeta_CR_0=2e-16, nside_rt=1)
+            sv.accelerator = acc
+            sv.run(max_iterations=3, convergence_tol=1e-9, dust_steps=1, verbose=False)
+            h = sv.convergence_history
+            out.update(run_dT=[r['dT'] for r in h], run_dH2=[r['dH2'] for r in h],
+                       run_dCO=[r['dCO'] for r in h])
+            for k in ('T_gas', 'x_HI', 'x_H2', 'x_Cp', 'x_C', 'x_CO', 'x_O', 'x_e', 'x_OH',
+                      'x_HCOp', 'G0', 'A_V'):
+                out[f"run_{k}"] = getattr(sv, k).ravel()
+    save("surrogate_golden.npz", **out)
+
+
+def gen_preset():
+    print("blowout preset input: embedded_star_cloud on 16^3 and 12^3")
+    from prism3d.density_fields import embedded_star_cloud
+    out = {}
+    for n, seed in ((16, 42), (12, 7)):
+        d, cs, pos = embedded_star_cloud(n, 3.0 * PC, n_cloud=3000, sigma_ln=1.2, seed=seed)
+        out.update({f"d{n}": d, f"cell{n}": cs, f"star{n}": pos})
+    out["L_FUV"] = 0.3 * 2e5 * C.Lsun_erg
+    save("preset_golden.npz", **out)
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["rhs", "thermal", "rt", "run", "steady"]
     if "rhs" in which:
@@ -511,3 +622,7 @@ if __name__ == "__main__":
         gen_tight()
     if "next" in which:
         gen_next()
+    if "surrogate" in which:
+        gen_surrogate()
+    if "preset" in which:
+        gen_preset()

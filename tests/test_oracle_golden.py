@@ -440,3 +440,83 @@ def test_state_archive_keys_match_upstream():
     a = golden("state_archive_golden.npz")
     assert set(a.files) == set(keys) | {'density', 'box_size', 'G0_external', 'zeta_CR_0'}
     assert a['density'].shape == (8, 8, 8) and a['box_size'].shape == ()
+
+
+# --- ML surrogate inference ---------------------------------------------------
+SURROGATE_INPUTS = ['n_H', 'T', 'G0', 'A_V', 'zeta', 'fsh_H2', 'fsh_CO']
+PACKED = ('left', 'right', 'feature', 'threshold', 'value', 'tree_offset', 'base', 'scale',
+          'divisor', 'n_trees')
+
+
+def _surrogate_model(g, kind):
+    return {k: g[f'{kind}_{k}'] for k in PACKED}
+
+
+@pytest.mark.parametrize("kind", ["gbrt", "rf"])
+def test_surrogate_matches_upstream_predict_batch(kind):
+    """ChemistryAccelerator.predict_batch with gradient-boosting and random-
+    forest models on the 8^3 fixture state plus cells below every floor:
+    per-species ensemble outputs and the conserved abundances, bit-exact."""
+    g = golden("surrogate_golden.npz")
+    m = _surrogate_model(g, kind)
+    ins = [g['in_' + k] for k in SURROGATE_INPUTS]
+    out = O.surrogate_predict_batch(m, g['x_mean'], g['x_scale'], g['y_mean'], g['y_scale'], *ins)
+    assert np.array_equal(out, g[f'{kind}_out'])
+    X = np.column_stack([np.log10(np.maximum(ins[0], 1)), np.log10(np.maximum(ins[1], 5)),
+                         np.log10(np.maximum(ins[2], 1e-2)), ins[3],
+                         np.log10(np.maximum(ins[4], 1e-18)), np.log10(np.maximum(ins[5], 1e-10)),
+                         np.log10(np.maximum(ins[6], 1e-10))])
+    Xs = (X - g['x_mean']) / g['x_scale']
+    for k in range(9):
+        assert np.array_equal(O.tree_ensemble_predict(m, k, Xs), g[f'{kind}_raw'][k])
+
+
+def test_tree_traversal_matches_the_tree_library():
+    """The restated traversal against the library's own predict at the
+    upstream training settings (200 stages of depth 6, subsample 0.8) and for
+    a forest with unbalanced trees, on inputs that include values exactly at
+    float32-rounded thresholds."""
+    sk = pytest.importorskip("sklearn.ensemble")
+    rng = np.random.RandomState(3)
+    X = rng.normal(size=(600, 7))
+    y = np.sin(X[:, 0]) + X[:, 1] * X[:, 2] - np.abs(X[:, 3])
+    models = [sk.GradientBoostingRegressor(n_estimators=200, max_depth=6, learning_rate=0.1,
+                                           subsample=0.8, random_state=42).fit(X, y),
+              sk.RandomForestRegressor(n_estimators=15, max_depth=9, min_samples_leaf=3,
+                                       random_state=1, n_jobs=1).fit(X, y)]
+    Xt = rng.normal(size=(400, 7))
+    Xt[:50, 0] = models[0].estimators_[0, 0].tree_.threshold[0]  # on a split value
+    for mdl in models:
+        if hasattr(mdl, 'init_'):
+            base, scale, div = float(mdl.init_.predict(X[:1])[0]), mdl.learning_rate, 1.0
+            trees = [e.tree_ for e in mdl.estimators_[:, 0]]
+        else:
+            base, scale, div, trees = 0.0, 1.0, float(len(mdl.estimators_)), \
+                [e.tree_ for e in mdl.estimators_]
+        off = np.cumsum([0] + [t.node_count for t in trees])
+        m = dict(left=np.concatenate([t.children_left for t in trees]),
+                 right=np.concatenate([t.children_right for t in trees]),
+                 feature=np.concatenate([t.feature for t in trees]),
+                 threshold=np.concatenate([t.threshold for t in trees]),
+                 value=np.concatenate([t.value[:, 0, 0] for t in trees]), tree_offset=off,
+                 base=np.array([base]), scale=np.array([scale]), divisor=np.array([div]),
+                 n_trees=np.array([len(trees)]))
+        assert np.array_equal(O.tree_ensemble_predict(m, 0, Xt), mdl.predict(Xt))
+
+
+def test_surrogate_run_loop_matches_upstream():
+    """run() with the accelerator on the 8^3 fixture (3 iterations): the
+    surrogate replaces the explicit chemistry and only oscillating cells are
+    damped; no explicit-chemistry exit rule is involved, so the whole
+    trajectory matches the upstream run to rounding."""
+    g = golden("surrogate_golden.npz")
+    acc = dict(model=_surrogate_model(g, 'gbrt'), x_mean=g['x_mean'], x_scale=g['x_scale'],
+               y_mean=g['y_mean'], y_scale=g['y_scale'])
+    r = golden("run_golden.npz")
+    st, hist = O.run_loop(r['density'].reshape(8, 8, 8), float(r['box']), 100.0, 2e-16, 1, 3,
+                          accelerator=acc)
+    for k, key in (('dT', 'run_dT'), ('dH2', 'run_dH2'), ('dCO', 'run_dCO')):
+        assert [h[k] for h in hist] == pytest.approx(list(g[key]), rel=1e-12), k
+    for k in ('T_gas', 'x_HI', 'x_H2', 'x_Cp', 'x_C', 'x_CO', 'x_O', 'x_e', 'x_OH', 'x_HCOp',
+              'G0', 'A_V'):
+        assert np.allclose(st[k].ravel(), g['run_' + k], rtol=1e-13, atol=0), k
