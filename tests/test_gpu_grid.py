@@ -242,3 +242,48 @@ def test_graph_replay_matches_eager_loop(gpu, chem):
             assert ra[k] == rb[k], (ra['iteration'], k)
     for k in ('T_gas', 'x_H2', 'x_CO', 'x_e', 'G0', 'Gamma', 'Lambda'):
         assert np.array_equal(getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy()), k
+
+
+def test_refine_matches_oracle_steady_solve(gpu):
+    """refine() after three Euler iterations on the 8^3 fixture: every cell's
+    network solve at the upstream solve_steady_state settings from the ten
+    grid species (others at the floor), shielding from the columns and the
+    mean T, against the oracle on the same inputs; then the thermal balance
+    at the refined abundances against the oracle thermal solve."""
+    from oracle import oracle as O
+    from prism3d_amd.solver3d import PDRSolverGPU
+    r = golden("run_golden.npz")
+    dens = r['density'].reshape(8, 8, 8)
+    sv = PDRSolverGPU(dens, float(r['box']), G0_external=100.0, zeta_CR_0=2e-16, nside_rt=1)
+    sv.run(max_iterations=3, convergence_tol=1e-9)
+    f = lambda a: a.cpu().numpy().ravel()
+    grid_sp = ['H', 'H2', 'C+', 'C', 'CO', 'O', 'e-', 'OH', 'H2O', 'HCO+']
+    y0 = np.full((O.NSP, 512), 1e-30)
+    ycur = sv.y.cpu().numpy()
+    for s in grid_sp:
+        y0[O.IDX[s]] = ycur[sv.sidx[s]]
+    T0 = f(sv.T_gas)
+    fH2 = O.f_shield_H2(f(sv.N_H2), T=np.mean(T0))
+    fCO = O.f_shield_CO(f(sv.N_CO), f(sv.N_H2))
+    ref, conv_ref, _ = O.solve_steady_batch(y0, f(sv.density), T0, f(sv.G0), f(sv.A_V),
+                                            f(sv.zeta_CR), fH2, fCO)
+    n_conv = sv.refine(verbose=False)
+    got = sv.y.cpu().numpy()
+    m = ref > 1e-14
+    rel = np.abs(got[m] - ref[m]) / ref[m]
+    print(f"refine: frac < 1e-3 {np.mean(rel < 1e-3):.4f}, median {np.median(rel):.2e}, "
+          f"converged {n_conv} (oracle {int(conv_ref.sum())})", flush=True)
+    assert np.mean(rel < 1e-3) > 0.99
+    assert abs(n_conv - int(conv_ref.sum())) <= 0.05 * 512
+    assert np.allclose(f(sv.fsh_H2), fH2, rtol=1e-13)
+    # thermal at the refined state
+    Td, gpe, rh2, zeta = f(sv.T_dust), f(sv.Gamma_PE), f(sv._R_H2), f(sv.zeta_CR)
+    xs = {k: got[sv.sidx[s]] for k, s in (('x_HI', 'H'), ('x_H2', 'H2'), ('x_e', 'e-'),
+                                           ('x_CO', 'CO'), ('x_Cp', 'C+'), ('x_O', 'O'),
+                                           ('x_C', 'C'))}
+    ctx = O.thermal_context(f(sv.density), f(sv.G0), f(sv.A_V), fH2, zeta, Td, gpe, rh2,
+                            xs['x_HI'], xs['x_H2'], xs['x_e'], xs['x_CO'], xs['x_Cp'], xs['x_O'],
+                            xs['x_C'])
+    Tref = O.solve_thermal_vec(*ctx, *O.line_params())
+    relT = np.abs(f(sv.T_gas) - Tref) / Tref
+    assert np.median(relT) < 1e-12 and np.mean(relT > 1e-4) <= 0.02
