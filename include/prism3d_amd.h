@@ -208,8 +208,11 @@ int p3d_fuv_columns(int64_t nx, int64_t ny, int64_t nz, const double *density,
                     double av_per_nh, double *g0, double *a_v, double *n_h,
                     double *n_h2, double *n_co, double *workspace, void *stream);
 
-/* out[0] = mean of x[0..n) (device scalar; np.mean of solver_3d.py:302). */
-int p3d_mean(int64_t n, const double *x, double *out, void *stream);
+/* out[0] = mean of x[0..n) (device scalar; np.mean of solver_3d.py:302),
+ * summed over a fixed partition (deterministic for a given n).  workspace:
+ * device buffer of 1024 doubles for the partial sums. */
+int p3d_mean(int64_t n, const double *x, double *out, double *workspace,
+             void *stream);
 
 /* H2 self-shielding (Draine & Bertoldi 1996, b from the device scalar
  * t_mean), CO shielding fit and the CR ionisation rate (model M scaled to

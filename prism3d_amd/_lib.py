@@ -57,7 +57,7 @@ SIGNATURES = {
     "p3d_surrogate_destroy": (_i32, [_i64]),
     "p3d_surrogate_predict": (_i32, [_i64, _i64] + [_VP] * 7 + [_dbl, _dbl,
                                      ctypes.POINTER(_VP), _VP]),
-    "p3d_mean": (_i32, [_i64, _VP, _VP, _VP]),
+    "p3d_mean": (_i32, [_i64, _VP, _VP, _VP, _VP]),
     "p3d_shield_cr": (_i32, [_i64, _VP, _VP, _VP, _VP, _dbl, _VP, _VP, _VP, _VP]),
     "p3d_dust_state": (_i32, [_i64, _VP, _VP, _VP, _VP, _VP, _dbl, _dbl, _dbl, _VP, _VP, _VP]),
     "p3d_euler_rates": (_i32, [_i64] + [_VP] * 8 + [_VP] * 6 + [_VP]),

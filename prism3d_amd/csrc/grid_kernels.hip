@@ -14,6 +14,8 @@
 //   euler rates  prism3d/solver_3d.py:549-557
 //   thermal ctx  prism3d/solver_3d.py:693-731
 //   diagnostics  prism3d/solver_3d.py:898-1050
+#include <algorithm>
+
 #include "p3d_common.h"
 
 namespace p3d {
@@ -155,16 +157,37 @@ ray_combine(int64_t nx, int64_t ny, int64_t nz, const double *__restrict__ dens,
   g0[i] = G; av[i] = avm; nh[i] = bh; nh2[i] = bh2; nco[i] = bco;
 }
 
-// Deterministic mean of n values by one workgroup (grid-stride partial sums,
-// then a fixed-shape tree).  Writes out[0].
-__global__ void __launch_bounds__(1024) mean_kernel(int64_t n, const double *__restrict__ x,
-                                                    double *__restrict__ out) {
-  __shared__ double s[1024];
+// Deterministic mean of n values in two passes over a fixed partition: block
+// b of mean_partial sums the contiguous chunk [b*chunk, (b+1)*chunk) (strided
+// per thread, then a fixed-shape tree) into part[b]; mean_final reduces the
+// at most kMeanParts partials with a fixed tree and writes out[0].  The
+// result depends only on n, never on scheduling.
+constexpr int kMeanParts = 1024;
+
+__global__ void __launch_bounds__(256) mean_partial(int64_t n, int64_t chunk,
+                                                    const double *__restrict__ x,
+                                                    double *__restrict__ part) {
+  __shared__ double s[256];
+  const int64_t b0 = (int64_t)blockIdx.x * chunk;
+  const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += 1024) acc += x[i];
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) acc += x[i];
   s[threadIdx.x] = acc;
   __syncthreads();
-  for (int w = 512; w >= 1; w >>= 1) {
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void __launch_bounds__(1024) mean_final(int nb, int64_t n,
+                                                   const double *__restrict__ part,
+                                                   double *__restrict__ out) {
+  __shared__ double s[kMeanParts];
+  s[threadIdx.x] = (int)threadIdx.x < nb ? part[threadIdx.x] : 0.0;
+  __syncthreads();
+  for (int w = kMeanParts / 2; w >= 1; w >>= 1) {
     if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
@@ -559,9 +582,14 @@ int p3d_fuv_columns(int64_t nx, int64_t ny, int64_t nz, const double *density,
   return launch_status("fuv_columns");
 }
 
-int p3d_mean(int64_t n, const double *x, double *out, void *stream) {
-  P3D_REQUIRE(n > 0 && x && out, "p3d_mean: n > 0 and device pointers required");
-  hipLaunchKernelGGL(mean_kernel, dim3(1), dim3(1024), 0, as_stream(stream), n, x, out);
+int p3d_mean(int64_t n, const double *x, double *out, double *workspace, void *stream) {
+  P3D_REQUIRE(n > 0 && x && out && workspace,
+              "p3d_mean: n > 0 and device pointers (x, out, workspace) required");
+  const int64_t nb = std::min<int64_t>(kMeanParts, std::max<int64_t>(1, (n + 8191) / 8192));
+  const int64_t chunk = (n + nb - 1) / nb;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(mean_partial, dim3((unsigned)nb), dim3(256), 0, s, n, chunk, x, workspace);
+  hipLaunchKernelGGL(mean_final, dim3(1), dim3(kMeanParts), 0, s, (int)nb, n, workspace, out);
   return launch_status("mean_kernel");
 }
 

@@ -131,12 +131,18 @@ def dust_evolve(g0, n_h, f_nano, e_g):
     return f_nano, e_g
 
 
-def mean(x, out=None):
+MEAN_WORKSPACE = 1024  # doubles of partial sums p3d_mean needs
+
+
+def mean(x, out=None, workspace=None):
+    """Deterministic device mean of x into out[0]."""
     x = _t(x)
     if out is None:
         out = torch.empty(1, dtype=torch.float64, device=x.device)
-    _lib.check(_lib.lib().p3d_mean(x.numel(), x.data_ptr(), out.data_ptr(), stream_handle()),
-               "p3d_mean")
+    if workspace is None:
+        workspace = torch.empty(MEAN_WORKSPACE, dtype=torch.float64, device=x.device)
+    _lib.check(_lib.lib().p3d_mean(x.numel(), x.data_ptr(), out.data_ptr(),
+                                   workspace.data_ptr(), stream_handle()), "p3d_mean")
     return out
 
 

@@ -127,6 +127,7 @@ class PDRSolverGPU:
                          (self.pe.sigma_abs, self.pe.dn_da_da, self.pe.is_nano)]
         self._ws = torch.empty(18 * self.n_cells, dtype=torch.float64, device=self.dev)
         self._tmean = torch.empty(1, dtype=torch.float64, device=self.dev)
+        self._mean_ws = torch.empty(grid.MEAN_WORKSPACE, dtype=torch.float64, device=self.dev)
         self.chem = ChemistrySolver() if chemistry == 'network' else None
         # an ML surrogate (surrogate.ChemistrySurrogate) set here replaces the
         # chemistry of every inner iteration, as upstream (solver_3d.py:142-143)
@@ -169,7 +170,7 @@ class PDRSolverGPU:
         grid.dust_evolve(self.G0, self.density, self.f_nano, self.E_g)
 
     def shielding_cr_step(self):
-        grid.mean(self.T_gas, out=self._tmean)
+        grid.mean(self.T_gas, out=self._tmean, workspace=self._mean_ws)
         self.fsh_H2, self.fsh_CO, self.zeta_CR = grid.shield_cr(self.N_H2, self.N_CO, self.N_H,
                                                                 self._tmean, self.zeta_CR_0)
 
@@ -431,7 +432,7 @@ class PDRSolverGPU:
         abundances.  Returns the number of early-stopped (converged) cells."""
         if self.chem is None:
             self.chem = ChemistrySolver()
-        grid.mean(self.T_gas, out=self._tmean)
+        grid.mean(self.T_gas, out=self._tmean, workspace=self._mean_ws)
         fsh_H2, fsh_CO, _ = grid.shield_cr(self.N_H2, self.N_CO, self.N_H, self._tmean,
                                            self.zeta_CR_0)
         y0 = torch.full_like(self.y, 1e-30)

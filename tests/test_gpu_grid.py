@@ -59,12 +59,16 @@ def test_shielding_and_cosmic_rays_match_upstream(gpu):
     assert close(z5, g['cr_zeta_5e17'], 0) < 1e-13
 
 
-def test_mean_kernel(gpu):
-    import torch
+@pytest.mark.parametrize("n", [1, 5, 8192, 8193, 100003, 9000011])
+def test_mean_kernel(gpu, n):
+    """Single- and multi-block partitions (up to the 1024-partial cap), against
+    numpy, and bit-identical on a second call."""
     from prism3d_amd import grid
-    x = np.random.RandomState(0).uniform(10, 1000, 100003)
-    m = grid.mean(T(x, gpu))
-    assert abs(m.item() - np.mean(x)) / np.mean(x) < 1e-14
+    x = np.random.RandomState(n).uniform(10, 1000, n)
+    xt = T(x, gpu)
+    m = grid.mean(xt).item()
+    assert abs(m - np.mean(x)) / np.mean(x) < 1e-13
+    assert grid.mean(xt).item() == m
 
 
 def test_dust_state_matches_upstream(gpu):
