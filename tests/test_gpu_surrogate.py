@@ -39,22 +39,18 @@ def test_surrogate_matches_upstream_predict_batch(gpu, kind):
         assert _rel(got, ref[k]).max() < 1e-13, sp
 
 
-def test_surrogate_full_size_ensemble_matches_oracle(gpu):
-    """Nine outputs of 200 depth-6 trees (the upstream training settings) with
-    unbalanced random trees, on 20k random cells, against the oracle."""
-    import torch
-    from oracle import oracle as O
-    from prism3d_amd.surrogate import SPECIES, ChemistrySurrogate
-    rng = np.random.RandomState(11)
+def _random_ensemble(rng, n_trees, depth, p_split=0.85):
+    """Nine outputs of n_trees random unbalanced trees of depth <= `depth`
+    (node ids in creation order, as the tree library stores them)."""
     left, right, feat, thr, val, off = [], [], [], [], [], [0]
-    for _ in range(9 * 200):
-        # grow a random tree to depth <= 6: node ids in creation order
+    for _ in range(9 * n_trees):
         L, R, F, TH, V = [], [], [], [], []
+
         def grow(d):
             k = len(L)
             L.append(-1); R.append(-1); F.append(-2); TH.append(-2.0)
             V.append(rng.normal())
-            if d < 6 and rng.rand() < 0.85:
+            if d < depth and (d == 0 or rng.rand() < p_split):
                 F[k] = rng.randint(7)
                 TH[k] = rng.normal() * 1.2
                 L[k] = grow(d + 1)
@@ -63,14 +59,27 @@ def test_surrogate_full_size_ensemble_matches_oracle(gpu):
         grow(0)
         left += L; right += R; feat += F; thr += TH; val += V
         off.append(off[-1] + len(L))
-    packed = dict(left=np.array(left), right=np.array(right), feature=np.array(feat),
-                  threshold=np.array(thr), value=np.array(val), tree_offset=np.array(off),
-                  base=rng.normal(size=9), scale=np.full(9, 0.1), divisor=np.ones(9),
-                  n_trees=np.full(9, 200, dtype=np.int32))
+    return dict(left=np.array(left), right=np.array(right), feature=np.array(feat),
+                threshold=np.array(thr), value=np.array(val), tree_offset=np.array(off),
+                base=rng.normal(size=9), scale=np.full(9, 0.1), divisor=np.ones(9),
+                n_trees=np.full(9, n_trees, dtype=np.int32))
+
+
+@pytest.mark.parametrize("n_trees,depth", [(200, 6), (7, 2), (13, 3), (9, 8), (5, 10)])
+def test_surrogate_ensembles_match_oracle(gpu, n_trees, depth):
+    """Random unbalanced ensembles: 200 stages of depth 6 (the upstream
+    training settings) and the depths either side of the LDS-staged kernel's
+    range (4-8) that run the direct kernel, on 20k random cells."""
+    import torch
+    from oracle import oracle as O
+    from prism3d_amd.surrogate import SPECIES, ChemistrySurrogate
+    print(f"case: ensemble {n_trees} trees, depth {depth}", flush=True)
+    rng = np.random.RandomState(11 + depth)
+    packed = _random_ensemble(rng, n_trees, depth)
     xm, xs = rng.normal(size=7), rng.uniform(0.5, 2, 7)
     ym, ys = rng.uniform(-8, -2, 9), rng.uniform(0.2, 1.0, 9)
     s = ChemistrySurrogate(packed, xm, xs, ym, ys)
-    assert s.depth == 6
+    assert s.depth == depth
     n = 20000
     ins = [10 ** rng.uniform(0, 7, n), 10 ** rng.uniform(0.5, 4, n), 10 ** rng.uniform(-3, 5, n),
            rng.uniform(0, 20, n), 10 ** rng.uniform(-19, -15, n), 10 ** rng.uniform(-11, 0, n),
@@ -85,6 +94,7 @@ def test_surrogate_full_size_ensemble_matches_oracle(gpu):
     dev_in = [torch.tensor(v, device=gpu) for v in ins]
     s.predict_into([y[k] for k in range(9)], *dev_in)
     assert torch.equal(y[2], out['C+'])
+    s.close()
 
 
 def test_surrogate_save_load_and_errors(gpu, tmp_path):
