@@ -128,7 +128,7 @@ def cpu_baseline(solver, recorded, budget_s=20.0):
 # (columns: density, x_H2, x_CO in; G0, A_V, N_H, N_H2, N_CO out).
 KERNEL_BYTES = {
     "columns": 64, "point_source": 64, "shield_cr": 48, "pe_heating": 56, "dust_state": 56,
-    "euler_rates": 112, "euler_chemistry": 176, "thermal_ctx": 192, "thermal": 88,
+    "euler_rates": 112, "euler_chemistry": 160, "thermal_ctx": 184, "thermal": 88,
     "heating_cooling": 144, "dust_evolve": 48, "surrogate": 128,
 }
 KERNEL_NAMES = {  # the device kernels each operation launches
