@@ -122,6 +122,64 @@ def cpu_baseline(solver, recorded, budget_s=20.0):
                       f"{solver.chem_rtol:g}) replayed by the C oracle in {t_chem:.2f} s"}
 
 
+# --- differentiation: adjoint / forward time ----------------------------------
+def adjoint_measure(n=32, steps=2):
+    """The differentiation workload: an n^3 log-normal cube (median n_H 1e4,
+    sigma_ln 1.0, 0.5 pc), G0 1e3 from -x over 48 rays, zeta 2e-16.  Its
+    fields (T, G0, A_V, zeta, shielding) come from the coupled fixed point
+    (explicit chemistry, converged); the objective is the summed [CII] 158 um
+    cooling plus the CO column along +x summed over the exit face, at the
+    charge-neutral steady state of every cell.  Forward = tight BDF
+    integration of every cell from the initial abundances + polish; adjoint
+    = the gradient with respect to log10 of every rate coefficient, ln G0,
+    ln zeta and ln density scale (p3d_chem_adjoint + the sum over cells).
+    Both timed with HIP events on the launch stream."""
+    import torch
+    from prism3d_amd.adjoint import SteadyStateProblem
+    from prism3d_amd.chemistry import CII_158
+    from prism3d_amd.density import PC, lognormal_median
+    from prism3d_amd.grid import directional_G0
+    from prism3d_amd.solver3d import PDRSolverGPU
+    box = 0.5 * PC
+    dens, cell = lognormal_median(n, box, 1e4, 1.0, seed=42)
+    g0r = directional_G0(1e3, [-1.0, 0.0, 0.0], nside=2)
+    sv = PDRSolverGPU(dens, box, G0_external=g0r, zeta_CR_0=2e-16, nside_rt=2,
+                      chemistry='euler')
+    sv.run(max_iterations=20, convergence_tol=0.05)
+    sv.shielding_cr_step()
+    f = lambda t: t.reshape(-1)
+    y0 = {'H': 1.0, 'C+': 1.4e-4, 'O': 3.0e-4, 'e-': 1.4e-4, 'He': 0.0851}
+    y0v = np.full(31, 1e-30)
+    from prism3d_amd.network import TRACKED
+    for s_, v in y0.items():
+        y0v[TRACKED.index(s_)] = v
+    prob = SteadyStateProblem(f(sv.density), f(sv.T_gas), f(sv.G0), f(sv.A_V), f(sv.zeta_CR),
+                              f(sv.fsh_H2), f(sv.fsh_CO), y0v, w_cii=1.0, w_co=cell)
+    t_f, t_a = [], []
+    for _ in range(steps):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        y, cs, ins = prob.state()
+        e1.record()
+        obj, grad = cs.steady_adjoint_batch(y, *ins, w_cii=prob.w_cii, w_co=prob.w_co,
+                                            line=CII_158)
+        g = grad.sum(dim=0)
+        e2.record()
+        e2.synchronize()
+        t_f.append(e0.elapsed_time(e1) * 1e-3)
+        t_a.append(e1.elapsed_time(e2) * 1e-3)
+    iters = prob.last_iters.cpu().numpy()
+    J = float(obj[:, 0].sum() + cell * obj[:, 1].sum())
+    return {"workload": f"{n}^3 log-normal cube (median n_H 1e4, sigma_ln 1.0, 0.5 pc), G0 1e3 "
+                        "from -x over 48 rays, zeta 2e-16; objective sum Lambda_CII + CO column "
+                        "along +x at the charge-neutral steady state",
+            "cells": prob.n, "parameters": prob.n_params,
+            "forward_s": float(np.median(t_f)), "adjoint_s": float(np.median(t_a)),
+            "ratio": float(np.median(t_a) / np.median(t_f)),
+            "polish_iterations_max": int(iters.max()), "polish_unconverged": int((iters < 0).sum()),
+            "objective": J, "grad_norm": float(g.abs().max())}
+
+
 # --- per-kernel lines (--kernel) ---------------------------------------------
 # Algorithmic HBM bytes per cell of one launch: every per-cell array the
 # operation must read plus every array it writes, 8 B each, read once
@@ -387,6 +445,8 @@ def main():
                     help="per-kernel lines instead of the headline: a comma list of "
                          + ", ".join(KERNEL_BYTES) + " or 'all'")
     ap.add_argument("--kn", type=int, default=256, help="grid cells per side of --kernel runs")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint / forward time of the differentiation workload")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (a known-traffic "
                          "kernel that calibrates profiler byte counters)")
@@ -462,6 +522,10 @@ def main():
                       "chem_form": form},
            "adjoint_over_forward": None,
            "roofline": roof}
+    if not args.no_adjoint:
+        adj = adjoint_measure()
+        out["adjoint_over_forward"] = adj["ratio"]
+        out["adjoint"] = adj
     if not args.no_cpu:
         rec = solver.recorded[:its[0]]  # the first timed solve's iterations
         out["cpu_baseline"] = cpu_baseline(solver, rec)

@@ -191,6 +191,53 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h,
                    const double *fsh_co, const p3d_solve_opts *opts,
                    int32_t *converged, int32_t *stats, void *stream);
 
+/* Conservation structure of a network for its charge-neutral steady state:
+ * nuclei[e][s] of element e in species s (n_elem <= 8), and cation[s] = 1
+ * for the species whose sum closes the electron abundance. */
+typedef struct p3d_steady_cons {
+  int32_t n_elem;
+  int8_t nuclei[8][64];
+  int8_t cation[64];
+} p3d_steady_cons;
+
+/* Polish every cell of y (n_species, n) in place onto its charge-neutral
+ * steady state from the given (integrated) state, by pseudo-transient
+ * continuation (implicit Euler steps of the balances with a pseudo time
+ * step from 1e10 s controlled by the relative change per step, ending in
+ * plain Newton at 1e30 s; about 35 iterations for most cells, up to ~220
+ * for cold dense cells near the bistable regime).  At that
+ * state every species balance is zero except, per element, that of the most
+ * abundant species carrying that element alone (replaced by the element's
+ * nuclei total of y_totals, a (n_species, n) device array such as the
+ * initial abundances, or of the input state when y_totals is NULL), the
+ * electron's (replaced by e- = sum of
+ * the cations, the closure of chemistry/solver.py:382-386), and species at or
+ * below 1e-28 (at the rate equations' 1e-30 floor; held).  iters[c] = Newton iterations used, -1 when max_iter
+ * were not enough (may be NULL).  No upstream counterpart: the upstream
+ * integration has no fixed point because the network does not conserve
+ * charge; this is the state its electron closure describes. */
+int p3d_chem_polish(int64_t handle, int64_t n, double *y, const double *n_h,
+                    const double *t_gas, const double *g0, const double *a_v,
+                    const double *zeta, const double *fsh_h2,
+                    const double *fsh_co, const p3d_steady_cons *cons,
+                    const double *y_totals, int32_t max_iter, int32_t *iters,
+                    void *stream);
+
+/* Gradient at the charge-neutral steady state y (from p3d_chem_polish) of
+ * J_c = w_cii * Lambda_CII(T, n_H, x) + w_co * n_H * x_CO per cell, with T,
+ * G0, A_V, zeta and the shielding held fixed ([CII] 158 um cooling of
+ * solver_3d.py:956-965; cii5 = host {freq, A, T_ul, g_u, g_l}).  obj (n x 2)
+ * = {Lambda_CII, n_H x_CO}; grad (n x (n_reactions + 3)) = dJ_c/dlog10 k_r for
+ * every reaction, then dJ_c/dln G0, dJ_c/dln zeta, dJ_c/dln n_H.  One
+ * transposed solve per cell (implicit function theorem).  Differentiation of
+ * the chemistry-thermal step (north-star objective; no upstream counterpart). */
+int p3d_chem_adjoint(int64_t handle, int64_t n, const double *y,
+                     const double *n_h, const double *t_gas, const double *g0,
+                     const double *a_v, const double *zeta, const double *fsh_h2,
+                     const double *fsh_co, const p3d_steady_cons *cons,
+                     const double *cii5, double w_cii, double w_co, double *obj,
+                     double *grad, void *stream);
+
 /* ---------------------------------------------------------------------- */
 /* Whole-grid kernels around the per-cell step                              */
 /* ---------------------------------------------------------------------- */

@@ -110,6 +110,9 @@ def lib():
             [i64, D] + [D] * 7 + [dbl, dbl, dbl, dbl,
                                   ctypes.POINTER(ctypes.c_int32),
                                   ctypes.POINTER(ctypes.c_int64)])
+        L.orc_reaction_rates.argtypes = [D] + [dbl] * 7 + [D, D]
+        L.orc_reaction_table.argtypes = [ctypes.POINTER(ctypes.c_int)] * 3
+        L.orc_set_k_scale.argtypes = [D]
         L.orc_n_species.restype = i32
         L.orc_n_reactions.restype = i32
         _lib = L
@@ -285,6 +288,220 @@ def integrate_batch(y_soa, nH, T, G0, AV, zeta, fH2, fCO, t0, t1, rtol, atol):
         status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
         steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
     return y, status, steps
+
+
+# ---------------------------------------------------------------------------
+# Steady-state sensitivities of the per-cell chemistry (adjoint and forward)
+# ---------------------------------------------------------------------------
+T_2B, T_PH, T_PH_H2, T_PH_CO, T_CR, T_GS = range(6)
+H_PLANCK = 6.62607015e-27
+
+
+def set_k_scale(scale=None):
+    """Multiply every rate coefficient k_r by scale[r] in all later oracle
+    calls (None restores 1)."""
+    if scale is None:
+        lib().orc_set_k_scale(None)
+    else:
+        sc = np.ascontiguousarray(scale, dtype=np.float64)
+        assert sc.size == 75
+        lib().orc_set_k_scale(_p(sc))
+
+
+def reaction_table():
+    """(type[75], reactants[75, 2], products[75, 3]) of the bundled network;
+    reactant -2 is the PAH- pseudo-species, -1 none."""
+    ci = ctypes.c_int
+    t, r, p = (np.zeros(n, dtype=np.int32) for n in (75, 150, 225))
+    lib().orc_reaction_table(t.ctypes.data_as(ctypes.POINTER(ci)),
+                             r.ctypes.data_as(ctypes.POINTER(ci)),
+                             p.ctypes.data_as(ctypes.POINTER(ci)))
+    return t, r.reshape(75, 2), p.reshape(75, 3)
+
+
+def reaction_rates(y, nH, T, G0, AV, zeta, fH2=1.0, fCO=1.0):
+    rate = np.zeros(75)
+    psi = ctypes.c_double(0.0)
+    lib().orc_reaction_rates(_p(np.ascontiguousarray(y, dtype=np.float64)), nH, T, G0, AV, zeta,
+                             fH2, fCO, _p(rate), ctypes.byref(psi))
+    return rate, psi.value
+
+
+def element_matrix():
+    """Nuclei per species, (elements, C[n_elem, NSP]), parsed from the species
+    formulas (charge signs and the electron carry none)."""
+    import re
+    elems = ['H', 'He', 'C', 'O', 'S', 'Si', 'Fe']
+    C = np.zeros((len(elems), NSP))
+    for s, name in enumerate(SPECIES):
+        for el, cnt in re.findall(r'([A-Z][a-z]?)(\d*)', name):
+            C[elems.index(el), s] += int(cnt) if cnt else 1
+    return elems, C
+
+
+def stoichiometry():
+    """nu[NSP, 75]: net change of each species per unit rate."""
+    t, re_, pr = reaction_table()
+    nu = np.zeros((NSP, 75))
+    for r in range(75):
+        for a in re_[r]:
+            if a >= 0:
+                nu[a, r] -= 1
+        for b in pr[r]:
+            if b >= 0:
+                nu[b, r] += 1
+    return nu
+
+
+def cii_cooling(T, nH, x_cp, x_e, x_h, x_h2, line=LINES["CII_158"]):
+    """[CII] 158 um cooling (solver_3d.py:956-965) and its partial derivatives
+    with respect to x_C+, x_e, x_H, x_H2 and ln n_H (at fixed abundances)."""
+    freq, A, Tul, gu, gl = line
+    ge = 8.63e-6 / (gl * np.sqrt(T)) * 2.15 * np.exp(-Tul / T)
+    gh = 8.0e-10 * (T / 100.0) ** 0.07
+    gh2 = 4.9e-10 * (T / 100.0) ** 0.12
+    q = ge * x_e * nH + gh * x_h * nH + gh2 * x_h2 * nH
+    a = (gu / gl) * np.exp(-Tul / T)
+    frac = a * q / (a * q + A + q)
+    e = A * H_PLANCK * freq
+    lam = x_cp * nH * frac * e
+    dfrac_dq = a * A / ((1.0 + a) * q + A) ** 2
+    base = x_cp * nH * e * dfrac_dq
+    d = dict(x_cp=nH * frac * e, x_e=base * ge * nH, x_h=base * gh * nH, x_h2=base * gh2 * nH,
+             ln_nh=lam + base * q)
+    return lam, d
+
+
+CATIONS = ['H+', 'C+', 'S+', 'Si+', 'Fe+', 'O+', 'H2+', 'H3+', 'OH+', 'H2O+', 'H3O+', 'CH+',
+           'CH2+', 'CO+', 'HCO+', 'He+']  # the electron closure, chemistry/solver.py:382-386
+
+
+FROZEN_BELOW = 1e-28  # species at or below this are held in the steady state
+
+
+def _steady_roles(x):
+    """Row roles at state x: held species (at or below 1e-28, i.e. at the
+    rate equations' 1e-30 floor, with a margin that keeps the choice the
+    same for integrations that differ by rounding) and, per
+    element, the pivot species (the most abundant one carrying that element
+    alone) whose balance row is replaced by the element total."""
+    _, C = element_matrix()
+    frozen = ~(x > FROZEN_BELOW)
+    pure = (C > 0).sum(axis=0) == 1
+    piv = []
+    for e in range(C.shape[0]):
+        cand = [s for s in range(NSP) if pure[s] and C[e, s] > 0 and not frozen[s]]
+        piv.append(max(cand, key=lambda q: C[e, q] * x[q]) if cand else -1)
+    return C, frozen, piv
+
+
+def _steady_system(x, tot, C, frozen, piv, par):
+    """Residual and Jacobian of the charge-neutral steady-state equations."""
+    cat = np.zeros(NSP)
+    cat[[IDX[s] for s in CATIONS]] = 1.0
+    F = rhs(x, *par)
+    M = jacobian(x, *par)
+    ie = IDX['e-']
+    F[ie] = x[ie] - cat @ x
+    M[ie] = -cat
+    M[ie, ie] = 1.0
+    for e, s in enumerate(piv):
+        if s >= 0:
+            F[s] = C[e] @ x - tot[e]
+            M[s] = C[e]
+    for s in np.where(frozen)[0]:
+        F[s] = 0.0
+        M[s] = 0.0
+        M[s, s] = 1.0
+    return F, M
+
+
+def polish_steady(y, nH, T, G0, AV, zeta, fH2=1.0, fCO=1.0, max_iter=600, y_totals=None):
+    """Charge-neutral steady state of one cell from y: every species balance
+    zero except the electron (e- = the 16 cations of chemistry/solver.py:
+    382-386), one pivot species per element (its nuclei total of y_totals,
+    or of y) and
+    species at or below 1e-28 (held).  Pseudo-transient continuation:
+    implicit Euler steps of the balances (constraint rows solved exactly)
+    with the pseudo time step starting at 1e10 s and scaled after each step
+    by clamp(target / (largest relative change), 0.1, 10), or by 0.1 when
+    the step had to be damped to keep abundances positive, up to 1e30 s
+    (plain Newton); target 0.5, and 0.1 after 100 iterations (cold dense
+    cells near the bistable regime need the gentler path).  Converged once a
+    full step at 1e30 s changes nothing by more than 1e-10 relative.
+    Returns (x, iterations or -1)."""
+    par = (nH, T, G0, AV, zeta, fH2, fCO)
+    x = np.array(y, dtype=np.float64)
+    _, C = element_matrix()
+    tot = C @ (x if y_totals is None else np.asarray(y_totals, dtype=np.float64))
+    cat = np.zeros(NSP)
+    cat[[IDX[s] for s in CATIONS]] = 1.0
+    x[IDX['e-']] = cat @ x
+    C, frozen, piv = _steady_roles(x)
+    dyn = ~frozen
+    dyn[IDX['e-']] = False
+    dyn[[s for s in piv if s >= 0]] = False
+    rows = np.where(dyn)[0]
+    dt = 1e10
+    for it in range(max_iter):
+        F, M = _steady_system(x, tot, C, frozen, piv, par)
+        M[rows, rows] -= 1.0 / dt
+        d = np.linalg.solve(M, -F)
+        with np.errstate(divide='ignore'):
+            lim = np.where(d < 0, -0.9 * x / np.where(d < 0, d, -1.0), np.inf)
+        a = min(1.0, float(lim.min()))
+        x = x + a * d
+        rel = np.max(np.abs(d) / np.maximum(np.abs(x), 1e-30))
+        if a == 1.0 and dt >= 1e30 and rel < 1e-10:
+            return x, it + 1
+        target = 0.5 if it < 100 else 0.1
+        f = 0.1 if a < 1.0 else min(max(target / max(rel, 1e-300), 0.1), 10.0)
+        dt = min(max(dt * f, 1.0), 1e30)
+    return x, -1
+
+
+def steady_sensitivities(x, nH, T, G0, AV, zeta, fH2, fCO, w_cii=1.0, w_co=1.0, forward=True):
+    """Gradient of J = w_cii Lambda_CII(T, n_H, x*) + w_co n_H x*_CO at the
+    charge-neutral steady state x* of one cell (polish_steady; radiation,
+    shielding and T held fixed) with respect to log10 k_r (75), ln G0, ln zeta
+    and ln n_H, by the implicit function theorem on its equations: the
+    adjoint form solves M^T lambda = dJ/dx once; the forward form solves
+    M s_p = -dF/dp for every parameter.  Returns (obj[2], grad_adjoint[78],
+    grad_forward[78] or None)."""
+    I = IDX
+    par = (nH, T, G0, AV, zeta, fH2, fCO)
+    x = np.asarray(x, dtype=np.float64)
+    C, frozen, piv = _steady_roles(x)
+    _, M = _steady_system(x, C @ x, C, frozen, piv, par)
+    rate, psi = reaction_rates(x, *par)
+    types, react, _ = reaction_table()
+    nu = stoichiometry()
+    moving = ~frozen
+    moving[I['e-']] = False
+    for s in piv:
+        if s >= 0:
+            moving[s] = False
+    lam, dl = cii_cooling(T, nH, x[I['C+']], x[I['e-']], x[I['H']], x[I['H2']])
+    g = np.zeros(NSP)
+    g[I['C+']] += w_cii * dl['x_cp']
+    g[I['e-']] += w_cii * dl['x_e']
+    g[I['H']] += w_cii * dl['x_h']
+    g[I['H2']] += w_cii * dl['x_h2']
+    g[I['CO']] += w_co * nH
+    fpah = np.where(react[:, 1] == -2, psi / (psi + 500.0), 0.0) if psi > 0 else np.zeros(75)
+    dr = np.zeros((75, 78))
+    dr[np.arange(75), np.arange(75)] = np.log(10.0) * rate
+    photo = np.isin(types, [T_PH, T_PH_H2, T_PH_CO])
+    dr[:, 75] = rate * photo - rate * fpah
+    dr[:, 76] = rate * (types == T_CR)
+    dr[:, 77] = rate * np.where(types == T_2B, 1.0 + fpah, np.where(types == T_GS, 1.0, 0.0))
+    Fp = nu @ dr
+    Fp[~moving] = 0.0
+    explicit = np.zeros(78)
+    explicit[77] = w_cii * dl['ln_nh'] + w_co * nH * x[I['CO']]
+    grad = explicit - np.linalg.solve(M.T, g) @ Fp
+    gf = explicit + g @ np.linalg.solve(M, -Fp) if forward else None
+    return np.array([lam, nH * x[I['CO']]]), grad, gf
 
 
 # ---------------------------------------------------------------------------

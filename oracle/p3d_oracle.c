@@ -394,6 +394,7 @@ static const orc_rxn RX[] = {
     R2(8.6e-8, -0.5, 0.0, H2Op_, E_, OH_, H_, NONE),
 };
 #define NRX ((int)(sizeof(RX) / sizeof(RX[0])))
+#define NRX_MAX 75
 
 int orc_n_species(void) { return NSP; }
 int orc_n_reactions(void) { return NRX; }
@@ -401,6 +402,16 @@ int orc_n_reactions(void) { return NRX; }
 typedef struct {
   double nH, T, G0, AV, zeta, fH2, fCO;
 } cellp;
+
+/* Per-reaction multipliers of the rate coefficients (1 unless set): the
+ * finite-difference checks of the sensitivities perturb one k_r at a time. */
+static double g_kscale[NRX_MAX];
+static int g_kscale_on = 0;
+
+void orc_set_k_scale(const double *scale) {
+  g_kscale_on = scale != NULL;
+  for (int r = 0; r < NRX_MAX; ++r) g_kscale[r] = scale ? scale[r] : 1.0;
+}
 
 /* Rate coefficients k_r (solver.py:100-133); grain_surface keeps its
  * n_H*0.5 factor outside (applied in the rate). */
@@ -416,6 +427,7 @@ static void rate_coeffs(const cellp *p, double *k) {
       case T_CR: k[r] = x->a * p->zeta; break;
       case T_GS: k[r] = x->a * pow(p->T / 300.0, x->b); break;
     }
+    if (g_kscale_on) k[r] *= g_kscale[r];
   }
 }
 
@@ -486,6 +498,31 @@ void orc_rhs(const double *y, double nH, double T, double G0, double AV,
   double k[NRX];
   rate_coeffs(&p, k);
   rhs_k(y, &p, k, dydt);
+}
+
+/* Per-reaction rates at max(y, 1e-30) (solver.py:89-133), and the PAH
+ * charge parameter psi = G0 sqrt(T) / n_e of the PAH- closed form (-1 when
+ * its constant branch applies).  Used by the steady-state sensitivities. */
+void orc_reaction_rates(const double *y, double nH, double T, double G0, double AV,
+                        double zeta, double fH2, double fCO, double *rate, double *psi) {
+  cellp p = {nH, T, G0, AV, zeta, fH2, fCO};
+  double k[NRX], yf[NSP], pm, dpm;
+  rate_coeffs(&p, k);
+  for (int i = 0; i < NSP; ++i) yf[i] = y[i] > 1e-30 ? y[i] : 1e-30;
+  pah_minus(yf, &p, &pm, &dpm);
+  reaction_rates(yf, &p, k, pm, rate);
+  double ye = yf[E_] > 1e-10 ? yf[E_] : 1e-10;
+  *psi = (G0 > 0.0 && ye * nH > 0.0) ? G0 * sqrt(T) / (ye * nH) : -1.0;
+}
+
+/* The reaction table: type (T_2B, T_PH, T_PH_H2, T_PH_CO, T_CR, T_GS),
+ * reactants (2 per reaction; -1 none, -2 PAH-) and products (3; -1 none). */
+void orc_reaction_table(int *type, int *react, int *prod) {
+  for (int r = 0; r < NRX; ++r) {
+    type[r] = RX[r].type;
+    for (int m = 0; m < 2; ++m) react[2 * r + m] = RX[r].r[m];
+    for (int m = 0; m < 3; ++m) prod[3 * r + m] = RX[r].p[m];
+  }
 }
 
 /* Analytic Jacobian d(dy/dt)/dy (row-major NSP x NSP).  The floor

@@ -31,6 +31,12 @@ class SolveOpts(ctypes.Structure):
                 ("waves_per_cu", _i32), ("rescue_frac", _dbl)]
 
 
+class SteadyCons(ctypes.Structure):
+    """p3d_steady_cons."""
+    _fields_ = [("n_elem", _i32), ("nuclei", (ctypes.c_int8 * 64) * 8),
+                ("cation", ctypes.c_int8 * 64)]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = {
     "p3d_version": (ctypes.c_char_p, []),
@@ -47,6 +53,10 @@ SIGNATURES = {
     "p3d_chem_post": (_i32, [_i64, _i64, _VP, _VP, _VP, _VP, _i32, _dbl, _dbl, _VP]),
     "p3d_chem_solve": (_i32, [_i64, _i64] + [_VP] * 8 + [ctypes.POINTER(SolveOpts), _VP, _VP,
                                                         _VP]),
+    "p3d_chem_polish": (_i32, [_i64, _i64] + [_VP] * 8 + [ctypes.POINTER(SteadyCons), _VP, _i32,
+                                                        _VP, _VP]),
+    "p3d_chem_adjoint": (_i32, [_i64, _i64] + [_VP] * 8 + [ctypes.POINTER(SteadyCons), _D, _dbl,
+                                                         _dbl, _VP, _VP, _VP]),
     "p3d_fuv_columns": (_i32, [_i64, _i64, _i64, _VP, _VP, _VP, _dbl, _i32, _D, _D, _dbl]
                         + [_VP] * 5 + [_VP, _VP]),
     "p3d_fuv_point_source": (_i32, [_i64, _i64, _i64, _VP, _VP, _VP, _D, _dbl, _dbl, _dbl, _dbl]

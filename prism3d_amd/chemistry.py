@@ -7,6 +7,8 @@ post-solve corrections.  Every solve runs the gfx950 BDF kernel
 structure-of-arrays device tensors and solve all cells in one launch.
 """
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -15,6 +17,34 @@ from .device import device, stream_handle
 from .network import DeviceNetwork, Network, default_device_network
 
 X_FLOOR = 1e-30
+CII_158 = (1.9005e12, 2.321e-6, 91.21, 4.0, 2.0)  # freq, A, T_ul, g_u, g_l (constants.py:103)
+ELEMENTS = ('H', 'He', 'C', 'O', 'S', 'Si', 'Fe')
+
+
+def element_counts(species):
+    """(len(ELEMENTS), len(species)) nuclei per species, from the formulas."""
+    import re
+    C = np.zeros((len(ELEMENTS), len(species)), dtype=np.int64)
+    for s, name in enumerate(species):
+        for el, cnt in re.findall(r'([A-Z][a-z]?)(\d*)', name):
+            C[ELEMENTS.index(el), s] += int(cnt) if cnt else 1
+    return C
+
+
+def steady_constraints(network):
+    """p3d_steady_cons of a network: element nuclei and the closure cations."""
+    from .network import CATIONS
+    if network.n_species > 64:
+        raise ValueError("at most 64 species")
+    C = element_counts(network.species)
+    k = _lib.SteadyCons()
+    k.n_elem = len(ELEMENTS)
+    for e in range(len(ELEMENTS)):
+        for s in range(network.n_species):
+            k.nuclei[e][s] = int(C[e, s])
+    for s, name in enumerate(network.species):
+        k.cation[s] = 1 if name in CATIONS else 0
+    return k
 GAS_C = 2.69e-4 * 0.4  # gas-phase C/H (prism3d/utils/constants.py:90-99)
 GAS_O = 4.90e-4 * 0.6
 GAS_S = 1.32e-5 * 0.4
@@ -177,6 +207,59 @@ class ChemistrySolver:
         _lib.check(rc, "p3d_chem_post")
         torch.cuda.synchronize(y.device)
         return y
+
+    # --- charge-neutral steady state and its sensitivities -------------------
+    def steady_constraints(self):
+        """p3d_steady_cons of this network: nuclei of each element per species
+        (from the species formulas) and the cations of the electron closure."""
+        if getattr(self, "_cons", None) is None:
+            self._cons = steady_constraints(self.network)
+        return self._cons
+
+    def polish_steady_batch(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0, f_shield_CO=1.0,
+                            max_iter=600, sync=True, y_totals=None):
+        """Polish every cell of y in place onto its charge-neutral steady
+        state (p3d_chem_polish), with the element totals of y_totals (same
+        shape, e.g. the initial abundances) or of y.  Returns the iterations
+        per cell (-1: not converged in max_iter)."""
+        self._check_state(y)
+        if y_totals is not None:
+            self._check_state(y_totals)
+            if y_totals.shape != y.shape:
+                raise ValueError("y_totals must have the shape of y")
+        n = y.shape[1]
+        p = _cell_arrays(n, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO)
+        iters = torch.empty(n, dtype=torch.int32, device=y.device)
+        cons = self.steady_constraints()
+        rc = _lib.lib().p3d_chem_polish(self.dnet.handle, n, y.data_ptr(),
+                                        *[t.data_ptr() for t in p], ctypes.byref(cons),
+                                        None if y_totals is None else y_totals.data_ptr(),
+                                        int(max_iter), iters.data_ptr(), stream_handle())
+        _lib.check(rc, "p3d_chem_polish")
+        if sync:
+            torch.cuda.synchronize(y.device)
+        return iters
+
+    def steady_adjoint_batch(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0,
+                             f_shield_CO=1.0, w_cii=1.0, w_co=1.0, line=CII_158):
+        """Per-cell objective {Lambda_CII, n_H x_CO} (n, 2) and gradient (n,
+        n_reactions + 3) of w_cii Lambda_CII + w_co n_H x_CO at the
+        charge-neutral steady state y, with respect to log10 k_r, ln G0,
+        ln zeta, ln n_H (p3d_chem_adjoint)."""
+        self._check_state(y)
+        n = y.shape[1]
+        p = _cell_arrays(n, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO)
+        obj = torch.empty((n, 2), dtype=torch.float64, device=y.device)
+        grad = torch.empty((n, self.network.n_reactions + 3), dtype=torch.float64,
+                           device=y.device)
+        cons = self.steady_constraints()
+        lp = np.ascontiguousarray(line, dtype=np.float64)
+        rc = _lib.lib().p3d_chem_adjoint(self.dnet.handle, n, y.contiguous().data_ptr(),
+                                         *[t.data_ptr() for t in p], ctypes.byref(cons),
+                                         _lib.dptr(lp), float(w_cii), float(w_co),
+                                         obj.data_ptr(), grad.data_ptr(), stream_handle())
+        _lib.check(rc, "p3d_chem_adjoint")
+        return obj, grad
 
     def rhs_batch(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0, f_shield_CO=1.0):
         self._check_state(y)

@@ -304,6 +304,8 @@ __device__ void jacobian(const NetDev &N, const CellParams &p, const Lds &L, int
   __syncthreads();
 }
 
+__device__ void lu_factor_inplace(const NetDev &N, const Lds &L, int lane);
+
 // M = I - c J, factorised in place with partial pivoting.
 __device__ void lu_factor(const NetDev &N, const Lds &L, double c, int lane) {
   const int n = N.ns, ld = L.ld;
@@ -312,6 +314,12 @@ __device__ void lu_factor(const NetDev &N, const Lds &L, double c, int lane) {
     L.M[idx] = (i == j ? 1.0 : 0.0) - c * L.J[idx];
   }
   __syncthreads();
+  lu_factor_inplace(N, L, lane);
+}
+
+// L.M factorised in place with partial pivoting (row interchanges in L.piv).
+__device__ void lu_factor_inplace(const NetDev &N, const Lds &L, int lane) {
+  const int n = N.ns, ld = L.ld;
   const bool half = n <= 32;
   const int col = half ? (lane & 31) : lane;
   const int r0 = half ? (lane >> 5) : 0, rs = half ? 2 : 1;
@@ -947,6 +955,239 @@ chem_jac_kernel(NetDev N, int64_t n, const double *__restrict__ y, const double 
   }
 }
 
+// ===========================================================================
+// Charge-neutral steady state and its adjoint.
+//
+// The network conserves the nuclei of every element but not charge (PAH
+// recombinations move charge into the untracked PAH- pseudo-species), so the
+// raw rate equations have no fixed point: free electrons accumulate.  The
+// upstream solve closes charge after integrating, with e- = sum of the
+// cations (chemistry/solver.py:382-386).  The steady state here is the
+// algebraic state that the closure makes exact: every species balance is
+// zero except
+//   - per element, the row of its pivot species (the most abundant species
+//     carrying that element alone), replaced by the element's nuclei total;
+//   - the electron row, replaced by e- = sum of the cations;
+//   - species at or below 1e-28 (at the 1e-30 floor of the rate equations),
+//     held fixed.
+// chem_polish_kernel reaches it from an integrated state by pseudo-transient
+// continuation (implicit Euler steps of the balances with a growing pseudo
+// time step, ending in plain Newton); chem_adjoint_kernel
+// differentiates an objective at it with respect to every log10 k_r, ln G0,
+// ln zeta and ln n_H by one transposed solve (the implicit function theorem).
+// ===========================================================================
+constexpr int kMaxElem = 8;
+// Species at or below this are held: they sit at the rate equations' 1e-30
+// floor (their own balance has no root there); the margin keeps the choice
+// the same for integrations that differ by rounding.
+constexpr double kFrozenBelow = 1e-28;
+
+struct SteadyCons {  // kernel argument
+  int n_elem;
+  signed char nuclei[kMaxElem][kMaxSpecies];
+  signed char cation[kMaxSpecies];
+};
+
+struct SteadyRows {
+  int pivot_of;  // element whose constraint replaces this lane's row, or -1
+  bool frozen;   // species at the floor: row held fixed
+};
+
+// Row roles of lane `lane`'s species at abundance x (all lanes participate).
+__device__ SteadyRows steady_rows(const NetDev &N, const SteadyCons &K, double x, int lane) {
+  SteadyRows r;
+  r.frozen = lane < N.ns && !(x > kFrozenBelow);
+  r.pivot_of = -1;
+  int carried = 0;
+  for (int e = 0; e < K.n_elem; ++e) carried += (lane < N.ns && K.nuclei[e][lane] > 0) ? 1 : 0;
+  for (int e = 0; e < K.n_elem; ++e) {
+    const bool cand = lane < N.ns && carried == 1 && K.nuclei[e][lane] > 0 && !r.frozen &&
+                      lane != N.e_idx;
+    const double v = cand ? (double)K.nuclei[e][lane] * x : -1.0;
+    const int best = wave_argmax(v, lane);
+    const double bv = bcast(v, best);
+    if (bv > 0.0 && lane == best) r.pivot_of = e;
+  }
+  return r;
+}
+
+// Writes row `lane` of the constrained Jacobian into L.M, transposed when
+// `transpose` (L.J holds the network Jacobian); the balance rows carry the
+// pseudo-time term -1/dt on the diagonal (0 for the plain Jacobian).
+__device__ void steady_matrix(const NetDev &N, const SteadyCons &K, const Lds &L,
+                              const SteadyRows &r, bool transpose, int lane,
+                              double inv_dt = 0.0) {
+  const int n = N.ns, ld = L.ld;
+  if (lane < n) {
+    for (int j = 0; j < n; ++j) {
+      double v;
+      if (r.frozen) v = j == lane ? 1.0 : 0.0;
+      else if (lane == N.e_idx) v = (j == lane ? 1.0 : 0.0) - (double)K.cation[j];
+      else if (r.pivot_of >= 0) v = (double)K.nuclei[r.pivot_of][j];
+      else v = L.J[lane * ld + j] - (j == lane ? inv_dt : 0.0);
+      if (transpose) L.M[j * ld + lane] = v;
+      else L.M[lane * ld + j] = v;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double lane_sum(const NetDev &N, double v, int lane) {
+  return wave_sum(lane < N.ns ? v : 0.0);
+}
+
+__global__ void __launch_bounds__(64)
+chem_polish_kernel(NetDev N, SteadyCons K, int64_t n, double *__restrict__ y,
+                   const double *__restrict__ nH, const double *__restrict__ T,
+                   const double *__restrict__ G0, const double *__restrict__ AV,
+                   const double *__restrict__ zeta, const double *__restrict__ fH2,
+                   const double *__restrict__ fCO, const double *__restrict__ y_tot,
+                   int max_iter, int32_t *__restrict__ iters) {
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  const Lds L = carve(smem, N);
+  const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
+  rate_coeffs(N, p, L, lane);
+  __syncthreads();
+  double x = lane < N.ns ? y[(int64_t)lane * n + cell] : 0.0;
+  // element totals (of the reference state when given), and the electron closure
+  const double xt = (y_tot && lane < N.ns) ? y_tot[(int64_t)lane * n + cell] : x;
+  double tot[kMaxElem];
+  for (int e = 0; e < K.n_elem; ++e)
+    tot[e] = lane_sum(N, lane < N.ns ? (double)K.nuclei[e][lane] * xt : 0.0, lane);
+  const double cat0 = lane_sum(N, lane < N.ns ? (double)K.cation[lane] * x : 0.0, lane);
+  if (lane == N.e_idx) x = cat0;
+  const SteadyRows rows = steady_rows(N, K, x, lane);
+  // pseudo-transient continuation: implicit Euler steps of the balances with
+  // a pseudo time step from 1e10 s, scaled after each step so the largest
+  // relative change aims at 0.5, or 0.1 after 100 iterations (x0.1 ... x10;
+  // x0.1 when positivity damps the step), up to 1e30 s where the iteration
+  // is plain Newton
+  double dt = 1e10;
+  int it = 0, done = 0;
+  for (; it < max_iter && !done; ++it) {
+    publish_y(N, L, x, lane);
+    const double f = rhs_lane(N, p, L, lane);
+    jacobian(N, p, L, lane);
+    // wave reductions run on every lane, outside the per-lane selection
+    const double cat = lane_sum(N, lane < N.ns ? (double)K.cation[lane] * x : 0.0, lane);
+    double F;
+    if (rows.frozen) F = 0.0;
+    else if (lane == N.e_idx) F = x - cat;
+    else F = f;
+    // element rows (every lane takes part in the sums)
+    for (int e = 0; e < K.n_elem; ++e) {
+      const double c = lane_sum(N, lane < N.ns ? (double)K.nuclei[e][lane] * x : 0.0, lane) - tot[e];
+      if (rows.pivot_of == e) F = c;
+    }
+    steady_matrix(N, K, L, rows, false, lane, 1.0 / dt);
+    lu_factor_inplace(N, L, lane);
+    const double d = lu_solve(N, L, lane < N.ns ? -F : 0.0, lane);
+    const double lim = (lane < N.ns && d < 0.0) ? -0.9 * x / d : INFINITY;
+    const double a = fmin(1.0, -wave_max(-lim));
+    if (lane < N.ns) x += a * d;
+    const double rel = wave_max(lane < N.ns ? fabs(d) / fmax(fabs(x), 1e-30) : 0.0);
+    done = (a == 1.0 && dt >= 1e30 && rel < 1e-10) ? 1 : 0;
+    const double target = it < 100 ? 0.5 : 0.1;  // gentler for cold dense cells
+    const double grow = a < 1.0 ? 0.1 : fmin(fmax(target / fmax(rel, 1e-300), 0.1), 10.0);
+    dt = fmin(fmax(dt * grow, 1.0), 1e30);
+  }
+  if (lane < N.ns) y[(int64_t)lane * n + cell] = x;
+  if (lane == 0 && iters) iters[cell] = done ? it : -1;
+}
+
+struct CiiLine {
+  double freq, A, Tul, gu, gl;
+};
+
+__global__ void __launch_bounds__(64)
+chem_adjoint_kernel(NetDev N, SteadyCons K, int64_t n, const double *__restrict__ y,
+                    const double *__restrict__ nH, const double *__restrict__ T,
+                    const double *__restrict__ G0, const double *__restrict__ AV,
+                    const double *__restrict__ zeta, const double *__restrict__ fH2,
+                    const double *__restrict__ fCO, CiiLine cl, double w_cii, double w_co,
+                    double *__restrict__ obj, double *__restrict__ grad) {
+#pragma clang fp contract(off)
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  const Lds L = carve(smem, N);
+  const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
+  rate_coeffs(N, p, L, lane);
+  __syncthreads();
+  const double x = lane < N.ns ? y[(int64_t)lane * n + cell] : 0.0;
+  publish_y(N, L, x, lane);
+  jacobian(N, p, L, lane);
+  const SteadyRows rows = steady_rows(N, K, x, lane);
+  steady_matrix(N, K, L, rows, true, lane);
+  // objective: [CII] 158 um cooling (solver_3d.py:956-965) + the CO column
+  const int iH = N.roles[0], iH2 = N.roles[1], iCp = N.roles[3], iCO = N.roles[5];
+  const double xcp = bcast(x, iCp), xe = bcast(x, N.e_idx), xh = bcast(x, iH),
+               xh2 = bcast(x, iH2), xco = bcast(x, iCO);
+  const double Tk = p.T, nh = p.nH;
+  const double ge = 8.63e-6 / (cl.gl * sqrt(Tk)) * 2.15 * exp(-cl.Tul / Tk);
+  const double gh = 8.0e-10 * pow(Tk / 100.0, 0.07);
+  const double gh2 = 4.9e-10 * pow(Tk / 100.0, 0.12);
+  const double q = ge * (xe * nh) + gh * (xh * nh) + gh2 * (xh2 * nh);
+  const double ar = (cl.gu / cl.gl) * exp(-cl.Tul / Tk);
+  const double frac = ar * q / (ar * q + cl.A + q);
+  const double eph = cl.A * 6.62607015e-27 * cl.freq;
+  const double lam = xcp * nh * frac * eph;
+  const double dfq = ar * cl.A / (((1.0 + ar) * q + cl.A) * ((1.0 + ar) * q + cl.A));
+  const double base = xcp * nh * eph * dfq;
+  double g = 0.0;
+  if (lane == iCp) g += w_cii * (nh * frac * eph);
+  if (lane == N.e_idx) g += w_cii * (base * ge * nh);
+  if (lane == iH) g += w_cii * (base * gh * nh);
+  if (lane == iH2) g += w_cii * (base * gh2 * nh);
+  if (lane == iCO) g += w_co * nh;
+  lu_factor_inplace(N, L, lane);
+  const double lmb = lu_solve(N, L, lane < N.ns ? g : 0.0, lane);
+  // multipliers of the rows that move with the parameters (the species
+  // balances); the constraint and frozen rows do not
+  const bool dyn = lane < N.ns && !rows.frozen && rows.pivot_of < 0 && lane != N.e_idx;
+  __syncthreads();  // the factors are no longer read
+  if (lane < N.ns) L.M[lane] = dyn ? lmb : 0.0;
+  eval_reactions(N, p, L, lane, false);  // rates into part[2r] (syncs)
+  // PAH charge parameter of the closed form (solver.py:74-87)
+  const double yef = fmax(L.ys[N.e_idx], 1e-10);
+  const double ne = yef * p.nH;
+  const double psi = (p.G0 > 0.0 && ne > 0.0) ? p.G0 * sqrt(p.T) / ne : -1.0;
+  const double fp = psi > 0.0 ? psi / (psi + 500.0) : 0.0;
+  double sG = 0.0, sZ = 0.0, sN = 0.0;
+  double *gout = grad + cell * (int64_t)(N.nr + 3);
+  for (int r = lane; r < N.nr; r += 64) {
+    double w = 0.0;  // sum over species of multiplier x net stoichiometry
+    for (int s2 = 0; s2 < N.ns; ++s2) {
+      const double l2 = L.M[s2];
+      if (l2 == 0.0) continue;
+      for (int e = N.rhs_off[s2]; e < N.rhs_off[s2 + 1]; ++e)
+        if (N.rhs_rxn[e] == r) w += N.rhs_sgn[e] * l2;
+    }
+    const double rate = L.part[2 * r];
+    const int t = N.rtype[r];
+    const bool pah = N.react[2 * r + 1] == P3D_SP_PAH_MINUS;
+    gout[r] = -(2.302585092994046 * rate) * w;
+    const double wr = w * rate;
+    const bool photo = t == P3D_RX_PHOTO || t == P3D_RX_PHOTO_FH2 || t == P3D_RX_PHOTO_FCO;
+    sG += wr * ((photo ? 1.0 : 0.0) - (pah && t == P3D_RX_TWO_BODY ? fp : 0.0));
+    sZ += wr * (t == P3D_RX_CR ? 1.0 : 0.0);
+    sN += wr * (t == P3D_RX_TWO_BODY ? 1.0 + (pah ? fp : 0.0) : (t == P3D_RX_GRAIN ? 1.0 : 0.0));
+  }
+  sG = wave_sum(sG);
+  sZ = wave_sum(sZ);
+  sN = wave_sum(sN);
+  if (lane == 0) {
+    const double explicit_n = w_cii * (lam + base * q) + w_co * (nh * xco);
+    gout[N.nr] = -sG;
+    gout[N.nr + 1] = -sZ;
+    gout[N.nr + 2] = explicit_n - sN;
+    obj[2 * cell] = lam;
+    obj[2 * cell + 1] = nh * xco;
+  }
+}
+
 // Persistent one-cell-per-lane solve (variant 4): sizes the resident grid,
 // grows the handle's slot memory and launches.
 static int lane_solve(Network *net, int64_t n, double *y, const double *n_h, const double *t_gas,
@@ -1211,6 +1452,60 @@ int p3d_chem_jacobian(int64_t handle, int64_t n, const double *y, const double *
   hipLaunchKernelGGL(chem_jac_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream), net->d,
                      n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, jac);
   return launch_status("chem_jac_kernel");
+}
+
+static int steady_cons_from(const p3d_steady_cons *c, const NetDev &d, SteadyCons &K) {
+  P3D_REQUIRE(c, "constraint description (host) is required");
+  P3D_REQUIRE(c->n_elem >= 0 && c->n_elem <= kMaxElem, "n_elem must be in [0, %d]", kMaxElem);
+  K.n_elem = c->n_elem;
+  for (int e = 0; e < kMaxElem; ++e)
+    for (int s = 0; s < kMaxSpecies; ++s) K.nuclei[e][s] = e < c->n_elem ? c->nuclei[e][s] : 0;
+  for (int s = 0; s < kMaxSpecies; ++s) K.cation[s] = s < d.ns ? c->cation[s] : 0;
+  P3D_REQUIRE(d.e_idx >= 0 && d.e_idx < d.ns, "the network has no electron species");
+  return P3D_OK;
+}
+
+int p3d_chem_polish(int64_t handle, int64_t n, double *y, const double *n_h, const double *t_gas,
+                    const double *g0, const double *a_v, const double *zeta,
+                    const double *fsh_h2, const double *fsh_co, const p3d_steady_cons *cons,
+                    const double *y_totals, int32_t max_iter, int32_t *iters, void *stream) {
+  Network *net = find_net(handle);
+  int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
+  if (rc != P3D_OK || n == 0) return rc;
+  SteadyCons K;
+  if ((rc = steady_cons_from(cons, net->d, K)) != P3D_OK) return rc;
+  P3D_REQUIRE(max_iter >= 1 && max_iter <= 1000, "max_iter must be in [1, 1000]");
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
+  P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
+  hipLaunchKernelGGL(chem_polish_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream),
+                     net->d, K, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, y_totals,
+                     (int)max_iter, iters);
+  return launch_status("chem_polish_kernel");
+}
+
+int p3d_chem_adjoint(int64_t handle, int64_t n, const double *y, const double *n_h,
+                     const double *t_gas, const double *g0, const double *a_v,
+                     const double *zeta, const double *fsh_h2, const double *fsh_co,
+                     const p3d_steady_cons *cons, const double *cii5, double w_cii, double w_co,
+                     double *obj, double *grad, void *stream) {
+  Network *net = find_net(handle);
+  int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
+  if (rc != P3D_OK || n == 0) return rc;
+  SteadyCons K;
+  if ((rc = steady_cons_from(cons, net->d, K)) != P3D_OK) return rc;
+  P3D_REQUIRE(cii5, "[CII] line parameters (host, 5 doubles) are required");
+  P3D_REQUIRE(obj && grad, "null output");
+  for (int k = 0; k < 7; ++k)
+    P3D_REQUIRE(k == 2 || k == 4 || k == 6 || net->d.roles[k] >= 0,
+                "the network lacks a species the objective needs (H, H2, C+, CO)");
+  CiiLine cl;
+  cl.freq = cii5[0]; cl.A = cii5[1]; cl.Tul = cii5[2]; cl.gu = cii5[3]; cl.gl = cii5[4];
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
+  P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
+  hipLaunchKernelGGL(chem_adjoint_kernel, dim3((unsigned)n), dim3(64), sh, as_stream(stream),
+                     net->d, K, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, cl, w_cii, w_co,
+                     obj, grad);
+  return launch_status("chem_adjoint_kernel");
 }
 
 int p3d_chem_post(int64_t handle, int64_t n, double *y, const double *n_h,
