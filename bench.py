@@ -428,6 +428,72 @@ def kernel_lines(args):
     sur.close()
 
 
+def clouds_line(args):
+    """One JSON line for the ensemble workload: args.clouds independent
+    log-normal clouds of args.cn^3 cells (G0 log-uniform 1e1-1e5, median n_H
+    log-uniform 1e2-1e6, sigma_ln uniform 0.5-1.5; prism3d_amd.density.
+    cloud_ensemble), args.clouds_iter fixed-point iterations, once with the
+    chemistry of all clouds in one launch per iteration (CloudBatch) and once
+    cloud by cloud, each with its own launch."""
+    import torch
+    from prism3d_amd.batch import CloudBatch
+    from prism3d_amd.density import cloud_ensemble
+    from prism3d_amd.grid import directional_G0
+    from prism3d_amd.solver3d import PDRSolverGPU
+    ens = cloud_ensemble(args.clouds, args.cn)
+
+    def make():
+        return [PDRSolverGPU(e['density'], e['box_size'],
+                             G0_external=directional_G0(e['G0'], [-1.0, 0.0, 0.0], nside=2),
+                             zeta_CR_0=2e-16, nside_rt=2, chemistry='network',
+                             x_c_total=1.4e-4, x_o_total=3.0e-4,
+                             initial_abundances={'H': 1.0, 'C+': 1.4e-4, 'O': 3.0e-4,
+                                                 'e-': 1.4e-4},
+                             T_init=50.0, chem_atol=1e-20, chem_variant=args.variant)
+                for e in ens]
+
+    n_cells = args.clouds * args.cn ** 3
+    clouds = make()
+    batch = CloudBatch(clouds, chem_variant=args.variant, chem_waves_per_cu=args.waves_per_cu)
+    print(f"[bench] ensemble: {args.clouds} clouds of {args.cn}^3, {n_cells} cells, one launch",
+          file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.run(max_iterations=args.clouds_iter, convergence_tol=0.0)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    print(f"[bench] one launch per iteration: {t_batch:.2f} s; now cloud by cloud",
+          file=sys.stderr, flush=True)
+    solo = make()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in solo:
+        c.run(max_iterations=args.clouds_iter, convergence_tol=0.0)
+    torch.cuda.synchronize()
+    t_solo = time.perf_counter() - t0
+    worst = 0.0
+    for a, b in zip(solo, clouds):
+        for k in ("T_gas", "x_H2", "x_CO", "x_e", "x_Cp"):
+            x, y = getattr(a, k).cpu().numpy().ravel(), getattr(b, k).cpu().numpy().ravel()
+            m = np.abs(x) > 1e-14
+            if m.any():
+                worst = max(worst, float(np.percentile(np.abs(y[m] - x[m]) / np.abs(x[m]),
+                                                       99.9)))
+    out = {"metric": "cells/s per fixed-point iteration, independent-cloud ensemble",
+           "value": n_cells * args.clouds_iter / t_batch, "unit": "cells/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{args.clouds} independent {args.cn}^3 log-normal clouds, "
+                                  "48 rays from -x, 31-species network, steady-state BDF at "
+                                  "rtol 1e-9, coupled thermal",
+                      "cells": n_cells, "iterations": args.clouds_iter,
+                      "chem_form_one_launch": chem_form(args.variant, n_cells),
+                      "chem_form_per_cloud": chem_form(args.variant, args.cn ** 3)},
+           "s_one_launch": t_batch, "s_cloud_by_cloud": t_solo,
+           "speedup_one_launch": t_solo / t_batch,
+           "p999_rel_diff_between_the_two": worst}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -450,6 +516,11 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (a known-traffic "
                          "kernel that calibrates profiler byte counters)")
+    ap.add_argument("--clouds", type=int, default=0,
+                    help="ensemble line instead of the headline: this many independent clouds")
+    ap.add_argument("--cn", type=int, default=32, help="cells per side of each --clouds cloud")
+    ap.add_argument("--clouds-iter", type=int, default=1,
+                    help="fixed-point iterations of the --clouds runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the per-cell "
@@ -465,6 +536,9 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.clouds > 0:
+        clouds_line(args)
+        return
     if args.kernel:
         if args.steps < 3:
             args.steps = 20

@@ -176,6 +176,9 @@ typedef struct p3d_solve_opts {
                           queue empty, cells still integrating are solved by
                           the one-cell-per-workgroup kernel instead (0: library
                           default 0.95; 1: never) */
+  int32_t rate_fp32;   /* 1: evaluate the exp/log of the rate coefficients in
+                          FP32 (once per cell; the integration stays FP64);
+                          0: FP64 as upstream (solver.py:100-133)           */
 } p3d_solve_opts;
 
 /* Integrate every cell to its steady state / end time with the

@@ -55,12 +55,14 @@ GAS_HE = 8.51e-2
 
 def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=1e-4,
                atol=1e-22, stop_rtol=1e-3, post_mode=0, max_steps=100000, x_c_total=GAS_C,
-               x_o_total=GAS_O, variant=0, waves_per_cu=0, rescue_frac=0.0):
+               x_o_total=GAS_O, variant=0, waves_per_cu=0, rescue_frac=0.0, rate_fp32=False):
     """p3d_solve_opts; the defaults are the upstream refine-path settings
     (solver.py:150-223).  variant: 0 auto, 1-3 one cell per 64-lane
     workgroup (LU in LDS / registers / unrolled LDS), 4 persistent
     one-cell-per-lane kernel; waves_per_cu sizes variant 4's resident grid and
-    rescue_frac sets when its long-tail cells move to the workgroup kernel."""
+    rescue_frac sets when its long-tail cells move to the workgroup kernel;
+    rate_fp32 evaluates the rate coefficients' exp/log in FP32 (the solve
+    itself stays FP64)."""
     o = _lib.SolveOpts()
     o.t_max = -1.0 if t_max is None else float(t_max)
     o.t_start = float(t_start)
@@ -73,6 +75,7 @@ def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=
     o.variant = int(variant)
     o.waves_per_cu = int(waves_per_cu)
     o.rescue_frac = float(rescue_frac)
+    o.rate_fp32 = int(bool(rate_fp32))
     return o
 
 

@@ -119,7 +119,7 @@ struct Tables {
 
 struct Opts {
   double t_max, t_start, rtol, atol, stop_rtol, x_c_total, x_o_total, newton_tol;
-  int nk, early_stop, post_mode, max_steps;
+  int nk, early_stop, post_mode, max_steps, rate_fp32;
 };
 
 // Rate coefficient k_r (solver.py:100-133; network.py:139-176).
@@ -132,6 +132,20 @@ P3D_HD P3D_INLINE double rate_coeff(int t, double a, double b, double g, const C
     case 3: return a * p.G0 * exp(-g * p.AV) * p.fCO;       // photo, CO shielded
     case 4: return a * p.zeta;                               // cosmic ray
     default: return a * exp(b * lT);                         // grain surface
+  }
+}
+
+// The same with the exponentials in FP32 (one expf of the combined exponent).
+P3D_HD P3D_INLINE double rate_coeff_f32(int t, double a, double b, double g, const Cell &p) {
+  const float Tc = fminf(fmaxf((float)p.T, 10.0f), 41000.0f);
+  const float bf = (float)b, gf = (float)g;
+  switch (t) {
+    case 0: return a * (double)expf(bf * logf(Tc / 300.0f) - gf / Tc);
+    case 1: return a * p.G0 * (double)expf(-gf * (float)p.AV);
+    case 2: return a * p.G0 * (double)expf(-gf * (float)p.AV) * p.fH2;
+    case 3: return a * p.G0 * (double)expf(-gf * (float)p.AV) * p.fCO;
+    case 4: return a * p.zeta;
+    default: return a * (double)expf(bf * logf((float)p.T / 300.0f));
   }
 }
 
@@ -330,7 +344,10 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
       const double Tc = fmin(fmax(L.p.T, 10.0), 41000.0);
       const double lTc = log(Tc / 300.0), lT = log(L.p.T / 300.0);
       for (int r = 0; r < kNR; ++r)
-        m.k(r) = rate_coeff(tb.rtype[r], tb.alpha[r], tb.beta[r], tb.gamma[r], L.p, Tc, lTc, lT);
+        m.k(r) = o.rate_fp32
+                     ? rate_coeff_f32(tb.rtype[r], tb.alpha[r], tb.beta[r], tb.gamma[r], L.p)
+                     : rate_coeff(tb.rtype[r], tb.alpha[r], tb.beta[r], tb.gamma[r], L.p, Tc,
+                                  lTc, lT);
       L.tmax = o.t_max > 0.0 ? o.t_max : fmin(3e17, fmax(1e16, 3e17 / fmax(L.p.nH, 1.0)));
       L.l0 = log10(o.t_start);
       L.l1 = log10(L.tmax);

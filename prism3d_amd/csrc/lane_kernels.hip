@@ -184,6 +184,7 @@ int lane_launch(const LaneArgs &a, int64_t n_waves, int occupancy, double *ws,
   o.newton_tol = fmax(10.0 * 2.220446049250313e-16 / q.rtol, fmin(0.03, sqrt(q.rtol)));
   o.nk = q.n_checkpoints; o.early_stop = q.early_stop; o.post_mode = q.post_mode;
   o.max_steps = q.max_steps;
+  o.rate_fp32 = q.rate_fp32;
   lane::Tables tb{a.rtype, a.alpha, a.beta, a.gamma, a.post_w};
   DevSrc src{a.n,     wq.head, wq.exited, wq.n_rescue, wq.rescue, wq.exit_threshold,
              a.y,     a.nH,    a.T,      a.G0,       a.AV,     a.zeta,

@@ -139,12 +139,34 @@ struct RxCache {
   }
 };
 
-// Rate coefficients k_r (solver.py:100-133).
-__device__ void rate_coeffs(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
+// Rate coefficient with its exponentials in FP32: one expf of the combined
+// exponent per reaction, the prefactors in FP64 (relative error about
+// |exponent| * 2^-24 against the FP64 form).
+__device__ __forceinline__ double rate_coeff_f32(int t, double a, double b, double g,
+                                                 const CellParams &p) {
+  const float Tc = fminf(fmaxf((float)p.T, 10.0f), 41000.0f);
+  const float bf = (float)b, gf = (float)g;
+  switch (t) {
+    case P3D_RX_TWO_BODY: return a * (double)expf(bf * logf(Tc / 300.0f) - gf / Tc);
+    case P3D_RX_PHOTO: return a * p.G0 * (double)expf(-gf * (float)p.AV);
+    case P3D_RX_PHOTO_FH2: return a * p.G0 * (double)expf(-gf * (float)p.AV) * p.fH2;
+    case P3D_RX_PHOTO_FCO: return a * p.G0 * (double)expf(-gf * (float)p.AV) * p.fCO;
+    case P3D_RX_CR: return a * p.zeta;
+    default: return a * (double)expf(bf * logf((float)p.T / 300.0f));  // grain surface
+  }
+}
+
+// Rate coefficients k_r (solver.py:100-133); f32: exponentials in FP32.
+__device__ void rate_coeffs(const NetDev &N, const CellParams &p, const Lds &L, int lane,
+                            bool f32 = false) {
   const double Tc = fmin(fmax(p.T, 10.0), 41000.0);
   const double lTc = log(Tc / 300.0), lT = log(p.T / 300.0);
   for (int r = lane; r < N.nr; r += 64) {
     const double a = N.alpha[r], b = N.beta[r], g = N.gamma[r];
+    if (f32) {
+      L.kco[r] = rate_coeff_f32(N.rtype[r], a, b, g, p);
+      continue;
+    }
     double k;
     switch (N.rtype[r]) {
       case P3D_RX_TWO_BODY: k = a * exp(b * lTc) * exp(-g / Tc); break;
@@ -848,7 +870,7 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
   const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
   const bool act = lane < N.ns;
   double yc = act ? y[(int64_t)lane * n + cell] : 0.0;
-  rate_coeffs(N, p, L, lane);
+  rate_coeffs(N, p, L, lane, o.rate_fp32 != 0);
   stage_tables(N, L, lane);
   __syncthreads();
   RxCache rc;

@@ -113,3 +113,27 @@ def blowout(n_cells=64, seed=42, n_mean=3000.0, sigma_ln=1.2, box_pc=3.0):
                                                seed=seed)
     star = {'position': star_pos, 'L_FUV': 0.3 * 2e5 * LSUN_ERG}
     return density, box, {'star': star, 'G0_external': 1.0, 'nside_rt': 1}
+
+
+def cloud_ensemble(n_clouds=64, n_cells=32, box_size=0.5 * PC, param_seed=2026, field_seeds=None,
+                   g0_range=(1e1, 1e5), median_range=(1e2, 1e6), sigma_range=(0.5, 1.5)):
+    """Independent log-normal clouds for batched runs: per cloud a field
+    G0 drawn log-uniform in g0_range, a median n_H log-uniform in
+    median_range and sigma_ln uniform in sigma_range (one
+    RandomState(param_seed) draw of the three per cloud, in that order); cloud
+    i's field uses field_seeds[i] (default i).  Returns a list of dicts
+    (density, box_size, G0, median, sigma_ln, seed)."""
+    rng = np.random.RandomState(param_seed)
+    seeds = list(range(int(n_clouds))) if field_seeds is None else list(field_seeds)
+    if len(seeds) != int(n_clouds):
+        raise ValueError("one field seed per cloud")
+    lg, lm = np.log10(g0_range), np.log10(median_range)
+    out = []
+    for i in range(int(n_clouds)):
+        g0 = 10.0 ** rng.uniform(*lg)
+        med = 10.0 ** rng.uniform(*lm)
+        sig = rng.uniform(*sigma_range)
+        d, _ = lognormal_median(n_cells, box_size, med, sig, seed=int(seeds[i]))
+        out.append(dict(density=d, box_size=float(box_size), G0=float(g0), median=float(med),
+                        sigma_ln=float(sig), seed=int(seeds[i])))
+    return out

@@ -51,6 +51,7 @@ int64_t lane_solve_host(int64_t n, double *y, const double *nH, const double *T,
   o.x_c_total = x_c_total; o.x_o_total = x_o_total;
   o.newton_tol = fmax(10.0 * 2.220446049250313e-16 / rtol, fmin(0.03, sqrt(rtol)));
   o.nk = nk; o.early_stop = early_stop; o.post_mode = post_mode; o.max_steps = max_steps;
+  o.rate_fp32 = 0;
   std::vector<double> mem(kSlotSize, 0.0);
   Slot m{mem.data(), 1};
   Lane L;

@@ -127,3 +127,42 @@ def test_driver_autotune_picks_a_config_and_keeps_results(gpu, tmp_path, monkeyp
         x, y = getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy()
         m = np.abs(x) > 1e-14
         assert np.max(np.abs(y[m] - x[m]) / np.abs(x[m])) < 1e-5, k
+
+
+@pytest.mark.parametrize("variant", [2, 4])
+def test_fp32_rate_evaluation_error_against_fp64_oracle(gpu, variant):
+    """Rate coefficients with FP32 exponentials (rate_fp32), the solve in
+    FP64: the steady states of the tight fixture against the FP64 oracle.
+    The rate error is ~|exponent| * 2^-24 (up to ~1e-5 for the steepest
+    Arrhenius terms at 10 K); the abundances above 1e-14 move by less than
+    1e-4 relative, and by far less at the median."""
+    import torch
+    from oracle import oracle as O
+    from prism3d_amd.chemistry import ChemistrySolver, solve_opts
+    print(f"case: FP32 rates, variant {variant}", flush=True)
+    g = golden("tight_golden.npz")
+    n = g['n_H'].size
+    y0 = np.repeat(g['y0'][:, None], n, axis=1)
+    t_end = float(g['t_end'])
+    cs = ChemistrySolver()
+    res = {}
+    for f32 in (False, True):
+        y = torch.tensor(y0, device=gpu)
+        opts = solve_opts(t_max=t_end, n_checkpoints=2, early_stop=False, rtol=1e-9, atol=1e-24,
+                          post_mode=-1, variant=variant, waves_per_cu=1 if variant == 4 else 0,
+                          rate_fp32=f32)
+        cs.solve_steady_batch(y, g['n_H'], g['T'], g['G0'], g['A_V'], g['zeta'], g['fsh_H2'],
+                              g['fsh_CO'], opts=opts)
+        res[f32] = y.cpu().numpy()
+    orc, _, _ = O.solve_steady_batch(y0, g['n_H'], g['T'], g['G0'], g['A_V'], g['zeta'],
+                                     g['fsh_H2'], g['fsh_CO'], t_max=t_end, n_checkpoints=2,
+                                     rtol=1e-9, atol=1e-24, early_stop=False, post_mode=-1)
+    m = orc > 1e-14
+    e64 = np.abs(res[False][m] - orc[m]) / orc[m]
+    e32 = np.abs(res[True][m] - orc[m]) / orc[m]
+    print(f"vs FP64 oracle: FP64 rates max {e64.max():.2e}; FP32 rates max {e32.max():.2e}, "
+          f"median {np.median(e32):.2e}, p99 {np.percentile(e32, 99):.2e}", flush=True)
+    assert e64.max() < 1e-6
+    assert e32.max() < 1e-4
+    assert np.median(e32) < 1e-6
+    assert not np.array_equal(res[True], res[False])

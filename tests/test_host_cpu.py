@@ -100,3 +100,28 @@ def test_embedded_star_cloud_and_blowout_preset_match_upstream():
     d, box, kw = blowout(16)
     assert np.array_equal(d, g['d16']) and box == 3.0 * PC
     assert kw['star']['L_FUV'] == float(g['L_FUV']) and kw['G0_external'] == 1.0
+
+
+def test_cloud_ensemble_parameters_and_fields():
+    """Per-cloud G0, median and sigma from one seeded draw, within the stated
+    ranges; each field is the log-normal generator at its own seed, so its
+    median and log-width match the drawn values; the default field seeds are
+    0..n-1."""
+    from prism3d_amd.density import PC, cloud_ensemble, lognormal_median
+    ens = cloud_ensemble(6, 16, param_seed=5, field_seeds=[9, 8, 7, 6, 5, 4])
+    again = cloud_ensemble(6, 16, param_seed=5, field_seeds=[9, 8, 7, 6, 5, 4])
+    rng = np.random.RandomState(5)
+    for i, e in enumerate(ens):
+        g0 = 10.0 ** rng.uniform(1, 5)
+        med = 10.0 ** rng.uniform(2, 6)
+        sig = rng.uniform(0.5, 1.5)
+        assert (e['G0'], e['median'], e['sigma_ln']) == (g0, med, sig)
+        assert 1e1 <= e['G0'] <= 1e5 and 1e2 <= e['median'] <= 1e6
+        assert 0.5 <= e['sigma_ln'] <= 1.5
+        d, _ = lognormal_median(16, 0.5 * PC, med, sig, seed=9 - i)
+        assert np.array_equal(e['density'], d)
+        assert np.array_equal(again[i]['density'], d)
+        ln = np.log(d)
+        assert abs(np.std(ln) - sig) < 1e-9 * sig + 1e-6 or np.any(d == 1.0)
+        assert abs(np.median(ln) - np.log(med)) < 0.5 * sig
+    assert [e['seed'] for e in cloud_ensemble(3, 8)] == [0, 1, 2]
