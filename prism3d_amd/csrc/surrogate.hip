@@ -7,13 +7,28 @@
 // leaves / number of trees), de-standardised and raised to 10^y; then H and
 // C renormalised, O by difference and e- from the cations.
 //
-// Trees are stored as perfect binary trees of one depth D (breadth-first,
+// Trees arrive as perfect binary trees of one depth D (breadth-first,
 // children of node i at 2i+1 and 2i+2; shallower leaves replicated), so a
-// walk is D dependent loads with no branch: the wavefront stays converged.
-// An internal node is one 64-bit word {float threshold, int feature}: the
-// library compares float32 features with float64 thresholds, which equals a
-// float32 comparison against the threshold rounded toward -inf (done on the
-// host).  Four trees are walked together so four load chains are in flight.
+// walk has no branch and the wavefront stays converged.  The library
+// compares float32 features with float64 thresholds, which equals a float32
+// comparison against the threshold rounded toward -inf (done by the caller).
+//
+// Rank encoding.  For each feature the library builds the sorted list of the
+// distinct thresholds the ensemble tests it against; a cell's feature value
+// x becomes its rank r = #{t < x} (a binary search, once per cell), and a
+// node testing x <= t_j becomes r <= j.  The two are equivalent for every
+// non-NaN x; NaN gets a rank above every j and goes right, as the library.
+// A node is then one 32-bit word, feature in the top 3 bits and j below.
+//
+// Two-level blocks.  A node at even depth and its two children share one
+// 16-byte block {node, left child, right child, 0}, so one vector load
+// serves two levels: a depth-6 walk is three block loads and one leaf load
+// instead of six node loads and a leaf load.  The walk is bound by the
+// rate at which the texture addresser takes lane-divergent addresses, so
+// fewer load instructions is the lever.  The cell's seven ranks sit in LDS
+// (lane-major, no bank conflicts) and each node's lookup is one LDS read.
+// Four trees are walked together so four load chains are in flight.
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -28,13 +43,26 @@ constexpr int kSurOut = 9;   // H, H2, C+, C, CO, O, e-, OH, HCO+
 constexpr int kSurFeat = 7;  // log nH, log T, log G0, A_V, log zeta, log fsh_H2, log fsh_CO
 constexpr int kSurMaxDepth = 16;
 
+constexpr int kSurRankBits = 29;                          // node word: j below bit 29
+constexpr uint32_t kSurRankMask = (1u << kSurRankBits) - 1u;
+constexpr int kSurBlock = 256;                           // threads per workgroup
+
 struct SurrogateDev {
   int n_trees[kSurOut], tree0[kSurOut];
   double base[kSurOut], scale[kSurOut], divisor[kSurOut];
   double x_mean[kSurFeat], x_scale[kSurFeat], y_mean[kSurOut], y_scale[kSurOut];
-  const uint2 *node;   // [tree][2^D - 1]
+  int thr0[kSurFeat], n_thr[kSurFeat];  // each feature's sorted distinct thresholds
+  const float *thr;
+  const uint4 *block;  // [tree][blocks of depth D], see block_count
   const double *leaf;  // [tree][2^D]
 };
+
+// two-level blocks of a depth-D tree: one per node at even depth
+__host__ __device__ constexpr int block_count(int D) {
+  int n = 0;
+  for (int d = 0; d < D; d += 2) n += 1 << d;
+  return n;
+}
 
 struct SurrogateOut {
   double *p[kSurOut];
@@ -53,58 +81,78 @@ static std::mutex g_sur_mu;
 static std::map<int64_t, std::unique_ptr<Surrogate>> g_surs;
 static int64_t g_next_sur = 1;
 
-__device__ __forceinline__ float pick_feature(const float (&x)[kSurFeat], int f) {
-  float v = x[0];
-#pragma unroll
-  for (int k = 1; k < kSurFeat; ++k) v = f == k ? x[k] : v;
-  return v;
+// does the node word w send this lane left?  rk: the workgroup's rank table
+// (feature-major, one entry per thread), tid: this thread
+__device__ __forceinline__ bool goes_left(uint32_t w, const uint32_t *rk, int tid) {
+  return rk[(w >> kSurRankBits) * kSurBlock + tid] <= (w & kSurRankMask);
 }
 
-// leaf index of the walk of one perfect tree of depth D
+// leaf index of the walk of one perfect tree of depth D (its blocks at bk)
 template <int D>
-__device__ __forceinline__ int walk(const uint2 *__restrict__ nd, const float (&x)[kSurFeat]) {
-  int i = 0;
+__device__ __forceinline__ int walk(const uint4 *__restrict__ bk, const uint32_t *rk, int tid) {
+  int p = 0, off = 0;
 #pragma unroll
-  for (int d = 0; d < D; ++d) {
-    const uint2 w = nd[i];
-    const float v = pick_feature(x, (int)w.y);
-    i = 2 * i + (v <= __uint_as_float(w.x) ? 1 : 2);  // NaN goes right, as the library
+  for (int d = 0; d < D; d += 2) {
+    const uint4 b = bk[off + p];
+    const bool l0 = goes_left(b.x, rk, tid);
+    p = 2 * p + (l0 ? 0 : 1);
+    if (d + 1 < D) {
+      const uint32_t w = l0 ? b.y : b.z;
+      p = 2 * p + (goes_left(w, rk, tid) ? 0 : 1);
+    }
+    off += 1 << d;
   }
-  return i - ((1 << D) - 1);
+  return p;
+}
+
+// rank of x among n sorted thresholds t: #{t < x}; NaN above every index
+__device__ __forceinline__ uint32_t rank_of(float x, const float *__restrict__ t, int n) {
+  if (x != x) return 0xFFFFFFFFu;
+  int lo = 0, hi = n;  // invariant: t[< lo] < x, t[>= hi] >= x
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (t[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return (uint32_t)lo;
 }
 
 template <int D>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSurBlock)
 surrogate_kernel(int64_t n, const double *__restrict__ nh, const double *__restrict__ tg,
                  const double *__restrict__ g0, const double *__restrict__ av,
                  const double *__restrict__ zeta, const double *__restrict__ fh2,
                  const double *__restrict__ fco, const SurrogateDev S, double xc_tot,
                  double xo_tot, const SurrogateOut out) {
 #pragma clang fp contract(off)
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  constexpr int NI = (1 << D) - 1, NL = 1 << D;
+  __shared__ uint32_t rk[kSurFeat * kSurBlock];
+  const int tid = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+  if (i >= n) return;  // every access below is this thread's own
+  constexpr int NB = block_count(D), NL = 1 << D;
   const double f[kSurFeat] = {log10(fmax(nh[i], 1.0)),    log10(fmax(tg[i], 5.0)),
                               log10(fmax(g0[i], 1e-2)),   av[i],
                               log10(fmax(zeta[i], 1e-18)), log10(fmax(fh2[i], 1e-10)),
                               log10(fmax(fco[i], 1e-10))};
-  float x[kSurFeat];
 #pragma unroll
-  for (int k = 0; k < kSurFeat; ++k) x[k] = (float)((f[k] - S.x_mean[k]) / S.x_scale[k]);
+  for (int k = 0; k < kSurFeat; ++k) {
+    const float x = (float)((f[k] - S.x_mean[k]) / S.x_scale[k]);
+    rk[k * kSurBlock + tid] = rank_of(x, S.thr + S.thr0[k], S.n_thr[k]);
+  }
   double r[kSurOut];
 #pragma unroll
   for (int o = 0; o < kSurOut; ++o) {
     double raw = S.base[o];
     const double sc = S.scale[o];
     const int nt = S.n_trees[o];
-    const uint2 *nd = S.node + (int64_t)S.tree0[o] * NI;
+    const uint4 *nd = S.block + (int64_t)S.tree0[o] * NB;
     const double *lf = S.leaf + (int64_t)S.tree0[o] * NL;
     int t = 0;
     for (; t + 4 <= nt; t += 4) {
-      const int a = walk<D>(nd + (int64_t)t * NI, x);
-      const int b = walk<D>(nd + (int64_t)(t + 1) * NI, x);
-      const int c = walk<D>(nd + (int64_t)(t + 2) * NI, x);
-      const int e = walk<D>(nd + (int64_t)(t + 3) * NI, x);
+      const int a = walk<D>(nd + (int64_t)t * NB, rk, tid);
+      const int b = walk<D>(nd + (int64_t)(t + 1) * NB, rk, tid);
+      const int c = walk<D>(nd + (int64_t)(t + 2) * NB, rk, tid);
+      const int e = walk<D>(nd + (int64_t)(t + 3) * NB, rk, tid);
       const double va = lf[(int64_t)t * NL + a], vb = lf[(int64_t)(t + 1) * NL + b];
       const double vc = lf[(int64_t)(t + 2) * NL + c], ve = lf[(int64_t)(t + 3) * NL + e];
       raw += sc * va;  // stage order of the library's accumulation
@@ -112,7 +160,7 @@ surrogate_kernel(int64_t n, const double *__restrict__ nh, const double *__restr
       raw += sc * vc;
       raw += sc * ve;
     }
-    for (; t < nt; ++t) raw += sc * lf[(int64_t)t * NL + walk<D>(nd + (int64_t)t * NI, x)];
+    for (; t < nt; ++t) raw += sc * lf[(int64_t)t * NL + walk<D>(nd + (int64_t)t * NB, rk, tid)];
     raw = raw / S.divisor[o];
     r[o] = pow(10.0, raw * S.y_scale[o] + S.y_mean[o]);
   }
@@ -135,7 +183,8 @@ surrogate_kernel(int64_t n, const double *__restrict__ nh, const double *__restr
 template <int D>
 static void launch_depth(hipStream_t s, int64_t n, const double *const *in, const SurrogateDev &S,
                          double xc, double xo, const SurrogateOut &o) {
-  hipLaunchKernelGGL(surrogate_kernel<D>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, in[0],
+  hipLaunchKernelGGL(surrogate_kernel<D>, dim3(grid_for(n, kSurBlock)), dim3(kSurBlock), 0, s,
+                     n, in[0],
                      in[1], in[2], in[3], in[4], in[5], in[6], S, xc, xo, o);
 }
 
@@ -206,17 +255,55 @@ int p3d_surrogate_create(int32_t n_outputs, int32_t n_features, int32_t depth,
     d.x_mean[k] = x_mean[k];
     d.x_scale[k] = x_scale[k];
   }
-  const size_t node_bytes = (size_t)(total * ni) * sizeof(uint2);
-  const size_t leaf_off = (node_bytes + 255) & ~(size_t)255;
-  const size_t bytes = leaf_off + (size_t)(total * nl) * sizeof(double);
+  // each feature's sorted distinct thresholds (equal values merged, so -0
+  // and +0 share one rank) and every node's index into its feature's list
+  std::vector<float> thr_all;
+  std::vector<uint32_t> node_rank((size_t)(total * ni), 0u);
+  for (int f = 0; f < kSurFeat; ++f) {
+    std::vector<float> t;
+    for (int64_t k = 0; k < total * ni; ++k)
+      if (feature[k] == f) t.push_back(threshold[k]);
+    for (float v : t) P3D_REQUIRE(v == v, "NaN threshold for feature %d", f);
+    std::sort(t.begin(), t.end());
+    t.erase(std::unique(t.begin(), t.end(), [](float a, float b) { return a == b; }), t.end());
+    P3D_REQUIRE(t.size() <= (size_t)kSurRankMask, "too many distinct thresholds for feature %d",
+                f);
+    d.thr0[f] = (int)thr_all.size();
+    d.n_thr[f] = (int)t.size();
+    for (int64_t k = 0; k < total * ni; ++k)
+      if (feature[k] == f)
+        node_rank[k] = (uint32_t)(std::lower_bound(t.begin(), t.end(), threshold[k]) - t.begin());
+    thr_all.insert(thr_all.end(), t.begin(), t.end());
+  }
+  const int nb = block_count(depth);
+  const size_t blk_bytes = (size_t)(total * nb) * sizeof(uint4);
+  const size_t leaf_off = (blk_bytes + 255) & ~(size_t)255;
+  const size_t thr_off = (leaf_off + (size_t)(total * nl) * sizeof(double) + 255) & ~(size_t)255;
+  const size_t bytes = thr_off + std::max<size_t>(thr_all.size(), 1) * sizeof(float);
   std::vector<char> host(bytes, 0);
-  uint2 *hn = reinterpret_cast<uint2 *>(host.data());
-  for (int64_t k = 0; k < total * ni; ++k) {
-    uint32_t bits;
-    std::memcpy(&bits, &threshold[k], 4);
-    hn[k] = make_uint2(bits, (uint32_t)feature[k]);
+  uint4 *hb = reinterpret_cast<uint4 *>(host.data());
+  auto word = [&](int64_t t, int64_t node) -> uint32_t {
+    const int64_t k = t * ni + node;
+    return ((uint32_t)feature[k] << kSurRankBits) | node_rank[k];
+  };
+  for (int64_t t = 0; t < total; ++t) {
+    int off = 0;
+    for (int dd = 0; dd < depth; dd += 2) {
+      for (int p = 0; p < (1 << dd); ++p) {
+        const int64_t node = ((int64_t)1 << dd) - 1 + p;  // breadth-first index
+        uint4 b = make_uint4(word(t, node), 0u, 0u, 0u);
+        if (dd + 1 < depth) {
+          b.y = word(t, 2 * node + 1);
+          b.z = word(t, 2 * node + 2);
+        }
+        hb[t * nb + off + p] = b;
+      }
+      off += 1 << dd;
+    }
   }
   std::memcpy(host.data() + leaf_off, leaf, (size_t)(total * nl) * sizeof(double));
+  if (!thr_all.empty())
+    std::memcpy(host.data() + thr_off, thr_all.data(), thr_all.size() * sizeof(float));
   if (bytes > 0) {
     if (hipMalloc(&sur->blob, bytes) != hipSuccess) {
       set_error("hipMalloc of %zu bytes for the tree ensemble failed", bytes);
@@ -224,8 +311,9 @@ int p3d_surrogate_create(int32_t n_outputs, int32_t n_features, int32_t depth,
     }
     P3D_CHECK_HIP(hipMemcpy(sur->blob, host.data(), bytes, hipMemcpyHostToDevice));
   }
-  d.node = reinterpret_cast<const uint2 *>(sur->blob);
+  d.block = reinterpret_cast<const uint4 *>(sur->blob);
   d.leaf = reinterpret_cast<const double *>(static_cast<char *>(sur->blob) + leaf_off);
+  d.thr = reinterpret_cast<const float *>(static_cast<char *>(sur->blob) + thr_off);
   std::lock_guard<std::mutex> lk(g_sur_mu);
   const int64_t h = g_next_sur++;
   g_surs[h] = std::move(sur);
