@@ -513,6 +513,10 @@ def main():
     ap.add_argument("--kn", type=int, default=256, help="grid cells per side of --kernel runs")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint / forward time of the differentiation workload")
+    ap.add_argument("--adjoint-n", type=int, default=32,
+                    help="cells per side of the differentiation workload's cube")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the differentiation workload's line (adjoint / forward time)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (a known-traffic "
                          "kernel that calibrates profiler byte counters)")
@@ -538,6 +542,13 @@ def main():
         del a, b
     if args.clouds > 0:
         clouds_line(args)
+        return
+    if args.adjoint_only:
+        adj = adjoint_measure(args.adjoint_n)
+        print(json.dumps({"metric": "adjoint / forward time, steady-state gradient",
+                          "value": adj["ratio"], "unit": "ratio", "n_gpus": 1,
+                          "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+                          "adjoint": adj}), flush=True)
         return
     if args.kernel:
         if args.steps < 3:
@@ -597,7 +608,7 @@ def main():
            "adjoint_over_forward": None,
            "roofline": roof}
     if not args.no_adjoint:
-        adj = adjoint_measure()
+        adj = adjoint_measure(args.adjoint_n)
         out["adjoint_over_forward"] = adj["ratio"]
         out["adjoint"] = adj
     if not args.no_cpu:
