@@ -38,15 +38,80 @@ FP64_PEAK_TFLOPS = 78.6
 N_SP, N_RX = 31, 75
 
 
-def flops_per_cell(stats):
+def flops_per_cell(stats, n=N_SP, n_rx=N_RX, n_terms=300):
     """Algorithmic FP64 operations of the BDF solve from its counters.
-    LU of the 31x31 iteration matrix 2n^3/3, triangular solves 2n^2, RHS
+    LU of the n x n iteration matrix 2n^3/3, triangular solves 2n^2, RHS
     ~8 per reaction + 1 per stoichiometric term, Jacobian 2 per term plus
     the matrix build n^2, per-step difference updates ~40 n."""
-    n = N_SP
     steps, lu, rhs, jac = [stats[:, k].astype(np.float64) for k in range(4)]
-    return (lu * (2.0 * n ** 3 / 3.0 + 2.0 * n * n) + rhs * (2.0 * n * n + 8 * N_RX + 300)
-            + jac * (2.0 * 300 + 4 * N_RX) + steps * 40.0 * n)
+    return (lu * (2.0 * n ** 3 / 3.0 + 2.0 * n * n) + rhs * (2.0 * n * n + 8 * n_rx + n_terms)
+            + jac * (2.0 * n_terms + 4 * n_rx) + steps * 40.0 * n)
+
+
+def synth_line(args):
+    """One JSON line for the 64-species synthetic network
+    (prism3d_amd.network_synth): args.synth_cells cells with n_H, T, G0,
+    A_V, zeta and shielding drawn log-uniform over PDR conditions, the
+    initial atomic/ionic state, integrated to 1e7 yr at rtol 1e-9 by the
+    one-cell-per-workgroup solve with the MFMA trailing-update LU
+    (variant 5), beside the same solve with the LDS LU (variant 1)."""
+    import torch
+    from prism3d_amd.chemistry import ChemistrySolver, solve_opts
+    from prism3d_amd.network import DeviceNetwork, Network
+    from prism3d_amd.network_synth import synthetic_network
+    sp, rx = synthetic_network()
+    net = Network(sp, rx)
+    cs = ChemistrySolver(DeviceNetwork(net))
+    n = args.synth_cells
+    rng = np.random.RandomState(4)
+    par = [10 ** rng.uniform(2, 5, n), 10 ** rng.uniform(1.2, 2.5, n), 10 ** rng.uniform(0, 3.5, n),
+           rng.uniform(0, 6, n), 10 ** rng.uniform(-16.5, -15.5, n), 10 ** rng.uniform(-4, 0, n),
+           10 ** rng.uniform(-4, 0, n)]
+    ins = [torch.tensor(v, device="cuda") for v in par]
+    y0 = np.full(len(sp), 1e-30)
+    for s_, v in (('H', 1.0), ('C+', 1.4e-4), ('O', 3e-4), ('N', 7.5e-5), ('He', 0.1),
+                  ('S+', 1e-6), ('Si+', 1e-7), ('Fe+', 1e-8)):
+        y0[sp.index(s_)] = v
+    y0[sp.index('e-')] = 1.4e-4 + 1e-6 + 1e-7 + 1e-8
+    n_terms = int(sum(len([x for x in r[2] if x in sp]) + len([x for x in r[3] if x in sp])
+                      for r in rx))
+    res = {}
+    for var in (5, 1):
+        y = torch.tensor(np.repeat(y0[:, None], n, axis=1), device="cuda")
+        o = solve_opts(t_max=1e7 * 3.15576e7, n_checkpoints=2, early_stop=False, rtol=1e-9,
+                       atol=1e-24, post_mode=-1, variant=var)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _, st = cs.solve_steady_batch(y, *ins, opts=o, sync=False)
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) * 1e-3
+        stn = st.cpu().numpy()
+        fl = float(flops_per_cell(stn, len(sp), len(rx), n_terms).sum())
+        res[var] = dict(s=sec, y=y.cpu().numpy(), flops=fl, lu=float(stn[:, 1].sum()),
+                        steps_mean=float(stn[:, 0].mean()))
+        print(f"[bench] synthetic network, variant {var}: {sec:.2f} s", file=sys.stderr,
+              flush=True)
+    a, b = res[5]["y"], res[1]["y"]
+    m = b > 1e-14
+    rel = np.abs(a[m] - b[m]) / b[m]
+    r5 = res[5]
+    achieved = r5["flops"] / r5["s"] / 1e12
+    lu_tf = r5["lu"] * 2.0 * len(sp) ** 3 / 3.0 / r5["s"] / 1e12
+    out = {"metric": "cells/s to 1e7 yr, 64-species synthetic network", "value": n / r5["s"],
+           "unit": "cells/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": f"{n} cells, 64 species / {len(rx)} reactions "
+                                  "(prism3d_amd.network_synth), BDF to 1e7 yr at rtol 1e-9",
+                      "kernel": "chem_solve_kernel<LinMfma> (variant 5)"},
+           "s_mfma_lu": r5["s"], "s_lds_lu": res[1]["s"],
+           "bdf_steps_per_cell": r5["steps_mean"],
+           "p999_rel_diff_mfma_vs_lds": float(np.percentile(rel, 99.9)) if rel.size else 0.0,
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                        "lu_only_tflops": lu_tf}}
+    print(json.dumps(out), flush=True)
 
 
 def make_solver(n=64, variant=0, waves_per_cu=0, autotune=False):
@@ -520,6 +585,9 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (a known-traffic "
                          "kernel that calibrates profiler byte counters)")
+    ap.add_argument("--synth-cells", type=int, default=0,
+                    help="64-species synthetic network line instead of the headline: this "
+                         "many cells")
     ap.add_argument("--clouds", type=int, default=0,
                     help="ensemble line instead of the headline: this many independent clouds")
     ap.add_argument("--cn", type=int, default=32, help="cells per side of each --clouds cloud")
@@ -542,6 +610,9 @@ def main():
         del a, b
     if args.clouds > 0:
         clouds_line(args)
+        return
+    if args.synth_cells > 0:
+        synth_line(args)
         return
     if args.adjoint_only:
         adj = adjoint_measure(args.adjoint_n)

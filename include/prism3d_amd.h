@@ -163,12 +163,15 @@ typedef struct p3d_solve_opts {
   int32_t max_steps;   /* per span                                          */
   double x_c_total, x_o_total; /* element totals for the conservation step */
   int32_t variant;     /* kernel variant: 0 auto (4 when the network is the
-                          bundled one and n >= resident lanes, else 2 or 1),
+                          bundled one and n >= resident lanes, else 2 for
+                          n_species <= 32, 5 for n_species == 64, else 1),
                           1 LU in LDS, 2 LU in registers
                           (n_species <= 32), 3 LU in LDS with unrolled loops
-                          (n_species <= 32): one cell per 64-lane workgroup;
-                          4 persistent one-cell-per-lane kernel with a device
-                          work queue (the bundled network structure only) */
+                          (n_species <= 32), 5 LU in LDS with the panel
+                          trailing updates on FP64 MFMA: one cell per 64-lane
+                          workgroup; 4 persistent one-cell-per-lane kernel
+                          with a device work queue (the bundled network
+                          structure only) */
   int32_t waves_per_cu; /* variant 4: resident wavefronts per compute unit
                           (0: library default 8); the kernel is compiled for
                           waves_per_cu / 4 wavefronts per SIMD (2..4) */

@@ -407,6 +407,93 @@ struct LinLds {
   }
 };
 
+// LinMfma: M = I - c J in LDS (n <= 64), factorised with partial pivoting
+// in panels of four columns (LAPACK's blocked right-looking order: panel
+// with row interchanges across all columns, U12 = L11^-1 A12, then
+// A22 -= L21 U12).  The trailing update is a rank-4 product per 16 x 16
+// tile, one v_mfma_f64_16x16x4f64 each.  Operand layout of that
+// instruction (64 lanes): lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator element r of lane l is D[4r + l / 16][l % 16].
+// The factors have the layout of lu_factor's (unit L below the diagonal, U
+// on and above it, interchanges in L.piv), so lu_solve applies.
+typedef double p3d_f64x4 __attribute__((ext_vector_type(4)));
+
+struct LinMfma {
+  __device__ void load_jac(const NetDev &, const Lds &, int) {}
+
+  __device__ void factor(const NetDev &N, const Lds &L, double c, int lane) {
+    const int n = N.ns, ld = L.ld;
+    for (int idx = lane; idx < n * ld; idx += 64) {
+      const int i = idx / ld, j = idx - i * ld;
+      L.M[idx] = (i == j ? 1.0 : 0.0) - c * L.J[idx];
+    }
+    __syncthreads();
+    const int nt = (n + 15) / 16;
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int k0 = 0; k0 < n; k0 += 4) {
+      const int k1 = min(k0 + 4, n);
+      // panel columns k0..k1-1, unblocked; lane = row for the column work,
+      // lane = column for the interchanges
+      for (int k = k0; k < k1; ++k) {
+        const double v = (lane >= k && lane < n) ? fabs(L.M[lane * ld + k]) : -1.0;
+        const int pk = wave_argmax(v, lane);
+        if (pk != k && lane < n) {
+          const double a = L.M[k * ld + lane];
+          L.M[k * ld + lane] = L.M[pk * ld + lane];
+          L.M[pk * ld + lane] = a;
+        }
+        if (lane == 0) L.piv[k] = pk;
+        __syncthreads();
+        const double inv = 1.0 / L.M[k * ld + k];
+        if (lane > k && lane < n) {
+          const double lik = L.M[lane * ld + k] * inv;
+          L.M[lane * ld + k] = lik;
+          for (int j = k + 1; j < k1; ++j) L.M[lane * ld + j] -= lik * L.M[k * ld + j];
+        }
+        __syncthreads();
+      }
+      if (k1 >= n) break;
+      // U12 = L11^-1 A12 (lane = column)
+      if (lane >= k1 && lane < n) {
+        for (int r = k0 + 1; r < k1; ++r) {
+          double acc = L.M[r * ld + lane];
+          for (int q = k0; q < r; ++q) acc -= L.M[r * ld + q] * L.M[q * ld + lane];
+          L.M[r * ld + lane] = acc;
+        }
+      }
+      __syncthreads();
+      // A22 -= L21 U12, tile by tile; operands outside rows/columns
+      // k1..n-1 or beyond the panel are zero, so the tiles need no alignment
+      const int q = k0 + lq;
+      for (int ti = k1 / 16; ti < nt; ++ti) {
+        const int ia = ti * 16 + lr;
+        const double a = (ia >= k1 && ia < n && q < k1) ? -L.M[ia * ld + q] : 0.0;
+        for (int tj = k1 / 16; tj < nt; ++tj) {
+          const int jb = tj * 16 + lr;
+          const double b = (jb >= k1 && jb < n && q < k1) ? L.M[q * ld + jb] : 0.0;
+          p3d_f64x4 acc;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = ti * 16 + 4 * r + lq, col = tj * 16 + lr;
+            acc[r] = (row < n && col < n) ? L.M[row * ld + col] : 0.0;
+          }
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = ti * 16 + 4 * r + lq, col = tj * 16 + lr;
+            if (row >= k1 && row < n && col >= k1 && col < n) L.M[row * ld + col] = acc[r];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  __device__ double solve(const NetDev &N, const Lds &L, double b, int lane) {
+    return lu_solve(N, L, b, lane);
+  }
+};
+
 // LinLdsU: M in LDS for n <= NP (32), every loop fully unrolled so that the
 // LDS loads of a step issue together, ahead of the dependency chain.  Lane L
 // handles column L & 31 of the trailing update on rows of parity L >> 5.
@@ -1565,7 +1652,7 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
     for (int k = 0; k < 7; ++k)
       P3D_REQUIRE(net->d.roles[k] >= 0,
                   "post-solve corrections / early stop need the species roles of the network");
-  P3D_REQUIRE(o.variant >= 0 && o.variant <= 4, "variant must be 0 .. 4");
+  P3D_REQUIRE(o.variant >= 0 && o.variant <= 5, "variant must be 0 .. 5");
   P3D_REQUIRE((o.variant != 2 && o.variant != 3) || net->d.ns <= 32,
               "variants 2 and 3 need n_species <= 32");
   P3D_REQUIRE(o.variant != 4 || net->lane_ok,
@@ -1578,7 +1665,9 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
   const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
   int var = o.variant;
-  if (var == 0) var = net->d.ns <= 32 ? 2 : 1;
+  // auto: registers for small networks, the MFMA trailing update from 64
+  // species (where the panel products are 16 x 16 x 4 tiles), else LDS
+  if (var == 0) var = net->d.ns <= 32 ? 2 : (net->d.ns >= 64 ? 5 : 1);
   // auto: the persistent lane kernel once the batch fills its resident lanes
   // (below that the slowest cell's tail dominates and one cell per
   // workgroup is faster; measured crossover ~1e5 cells on MI355X)
@@ -1590,7 +1679,11 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
       return lane_solve(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o, converged,
                         stats, stream);
   }
-  if (var == 3)
+  if (var == 5)
+    hipLaunchKernelGGL(chem_solve_kernel<LinMfma>, dim3((unsigned)n), dim3(64), sh,
+                       as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,
+                       o, converged, stats, nullptr, nullptr);
+  else if (var == 3)
     hipLaunchKernelGGL(chem_solve_kernel<LinLdsU<32>>, dim3((unsigned)n), dim3(64), sh,
                        as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,
                        o, converged, stats, nullptr, nullptr);

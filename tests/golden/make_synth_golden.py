@@ -1,0 +1,52 @@
+"""Write synth_golden.npz: steady states of the 64-species synthetic network
+(prism3d_amd.network_synth) on six seeded cells, from scipy's BDF on the
+oracle's restatement of the upstream RHS (oracle/generic_network.py) at
+rtol 1e-10, atol 1e-24, with the analytic Jacobian.  No upstream code runs;
+this checks the GPU solve on a network the reference does not ship.
+
+    python tests/golden/make_synth_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle.generic_network import GenericNetwork, integrate  # noqa: E402
+from prism3d_amd.network_synth import synthetic_network  # noqa: E402
+
+INIT = (('H', 1.0), ('C+', 1.4e-4), ('O', 3e-4), ('N', 7.5e-5), ('He', 0.1), ('S+', 1e-6),
+        ('Si+', 1e-7), ('Fe+', 1e-8))
+
+
+def initial(species):
+    y0 = np.full(len(species), 1e-30)
+    for s, v in INIT:
+        y0[species.index(s)] = v
+    y0[species.index('e-')] = 1.4e-4 + 1e-6 + 1e-7 + 1e-8
+    return y0
+
+
+def main():
+    sp, rx = synthetic_network()
+    net = GenericNetwork(sp, rx)
+    y0 = initial(sp)
+    t_end = 1e7 * 3.15576e7
+    rng = np.random.RandomState(4)
+    par = np.array([(10 ** rng.uniform(2, 5), 10 ** rng.uniform(1.2, 2.5),
+                     10 ** rng.uniform(0, 3.5), rng.uniform(0, 6),
+                     10 ** rng.uniform(-16.5, -15.5), 10 ** rng.uniform(-4, 0),
+                     10 ** rng.uniform(-4, 0)) for _ in range(6)])
+    out = []
+    for c, p in enumerate(par):
+        out.append(integrate(net, y0, *p, t_end, rtol=1e-10, atol=1e-24))
+        print(f"cell {c} done", flush=True)
+    np.savez_compressed(os.path.join(HERE, "synth_golden.npz"), params=par, y0=y0,
+                        t_end=t_end, y_end=np.array(out), rtol=1e-10, atol=1e-24)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,114 @@
+"""GPU parity of the workgroup solve with the FP64-MFMA trailing-update LU
+(p3d_chem_solve variant 5): on the bundled network against the oracle and
+the upstream scipy fixture, and on the 64-species synthetic network against
+scipy's BDF on the oracle's restated RHS and against the LDS LU."""
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_lu_bundled_network_tight_fixture(gpu):
+    import torch
+    from oracle import oracle as O
+    from prism3d_amd.chemistry import ChemistrySolver, solve_opts
+    print("case: variant 5 on the tight fixture", flush=True)
+    g = golden("tight_golden.npz")
+    n = g['n_H'].size
+    y0 = np.repeat(g['y0'][:, None], n, axis=1)
+    t_end = float(g['t_end'])
+    y = torch.tensor(y0, device=gpu)
+    opts = solve_opts(t_max=t_end, n_checkpoints=2, early_stop=False, rtol=1e-9, atol=1e-24,
+                      post_mode=-1, variant=5)
+    _, st = ChemistrySolver().solve_steady_batch(y, g['n_H'], g['T'], g['G0'], g['A_V'],
+                                                 g['zeta'], g['fsh_H2'], g['fsh_CO'], opts=opts)
+    got = y.cpu().numpy()
+    orc, _, _ = O.solve_steady_batch(y0, g['n_H'], g['T'], g['G0'], g['A_V'], g['zeta'],
+                                     g['fsh_H2'], g['fsh_CO'], t_max=t_end, n_checkpoints=2,
+                                     rtol=1e-9, atol=1e-24, early_stop=False, post_mode=-1)
+    m = orc > 1e-14
+    e_or = np.max(np.abs(got[m] - orc[m]) / orc[m])
+    ref = g['y_end'].T
+    m = ref > 1e-14
+    e_up = np.max(np.abs(got[m] - ref[m]) / ref[m])
+    print(f"variant 5 tight: vs oracle {e_or:.2e}, vs upstream scipy {e_up:.2e}", flush=True)
+    assert e_or < 1e-6 and e_up < 1e-6
+    assert (st.cpu().numpy()[:, 1] > 0).all()
+
+
+def _synth_solver():
+    from prism3d_amd.chemistry import ChemistrySolver
+    from prism3d_amd.network import Network
+    from prism3d_amd.network_synth import synthetic_network
+    sp, rx = synthetic_network()
+    return sp, ChemistrySolver(Network(sp, rx))
+
+
+def test_synthetic_network_against_scipy(gpu):
+    """Six cells of the 64-species network integrated to 1e7 yr: both LU
+    forms against scipy's BDF (rtol 1e-10) on the restated RHS, species
+    above 1e-14."""
+    import torch
+    from prism3d_amd.chemistry import solve_opts
+    g = golden("synth_golden.npz")
+    sp, cs = _synth_solver()
+    par = g['params']
+    n = par.shape[0]
+    ref = g['y_end'].T
+    m = ref > 1e-14
+    for var in (5, 1):
+        print(f"case: synthetic network, variant {var}", flush=True)
+        y = torch.tensor(np.repeat(g['y0'][:, None], n, axis=1), device=gpu)
+        opts = solve_opts(t_max=float(g['t_end']), n_checkpoints=2, early_stop=False, rtol=1e-9,
+                          atol=1e-24, post_mode=-1, variant=var)
+        cs.solve_steady_batch(y, *[par[:, k] for k in range(7)], opts=opts)
+        got = y.cpu().numpy()
+        rel = np.abs(got[m] - ref[m]) / ref[m]
+        print(f"variant {var} vs scipy: max {rel.max():.2e}, median {np.median(rel):.2e}",
+              flush=True)
+        assert rel.max() < 1e-5
+        assert np.median(rel) < 1e-8
+
+
+def test_mfma_and_lds_lu_agree_on_random_cells(gpu):
+    """512 cells over PDR conditions on the 64-species network: the MFMA
+    panel LU and the unblocked LDS LU give the same steady states (they
+    differ only in the rounding of the trailing updates)."""
+    import torch
+    from prism3d_amd.chemistry import solve_opts
+    print("case: synthetic network, variant 5 vs 1 on 512 cells", flush=True)
+    sp, cs = _synth_solver()
+    rng = np.random.RandomState(9)
+    n = 512
+    par = [10 ** rng.uniform(2, 5, n), 10 ** rng.uniform(1.2, 2.5, n),
+           10 ** rng.uniform(0, 3.5, n), rng.uniform(0, 6, n), 10 ** rng.uniform(-16.5, -15.5, n),
+           10 ** rng.uniform(-4, 0, n), 10 ** rng.uniform(-4, 0, n)]
+    g = golden("synth_golden.npz")
+    out = {}
+    for var in (5, 1):
+        y = torch.tensor(np.repeat(g['y0'][:, None], n, axis=1), device=gpu)
+        opts = solve_opts(t_max=float(g['t_end']), n_checkpoints=2, early_stop=False, rtol=1e-9,
+                          atol=1e-24, post_mode=-1, variant=var)
+        _, st = cs.solve_steady_batch(y, *par, opts=opts)
+        out[var] = (y.cpu().numpy(), st.cpu().numpy())
+    a, b = out[5][0], out[1][0]
+    m = b > 1e-14
+    rel = np.abs(a[m] - b[m]) / b[m]
+    print(f"variant 5 vs 1: p99.9 {np.percentile(rel, 99.9):.2e}, max {rel.max():.2e}; "
+          f"steps {out[5][1][:, 0].mean():.0f} / {out[1][1][:, 0].mean():.0f}", flush=True)
+    assert np.percentile(rel, 99.9) < 1e-6
+    assert rel.max() < 1e-4
+
+
+def test_variant_5_rejects_oversized_networks(gpu):
+    """The MFMA LU variant, like every workgroup form, takes at most 64
+    species; a larger network is refused at creation."""
+    from prism3d_amd.network import DeviceNetwork, Network
+    from prism3d_amd.network_synth import synthetic_network
+    sp, rx = synthetic_network()
+    sp2 = sp + ['X1']
+    with pytest.raises(ValueError):
+        DeviceNetwork(Network(sp2, rx))
