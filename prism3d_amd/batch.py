@@ -103,6 +103,7 @@ class CloudBatch:
                     old_T, old_H2, old_CO, w_damp[i] = c._after_chemistry(it, w_damp[i],
                                                                           *prev[j])
                     rec = c._convergence(old_T, old_H2, old_CO, convergence_tol)
+                    c._resolve_timing()
                     rec['dust_step'] = dust_step
                     rec['iteration'] = it
                     rec['f_nano_mean'] = float(c.f_nano.mean())

@@ -71,11 +71,6 @@ def synthetic_network(seed=2024, n_extra=None, max_channels=3):
     species = list(TRACKED) + list(extra)
     rng = np.random.RandomState(seed)
     comp = {s: formula(s) for s in species}
-    neutrals = [s for s in species if comp[s][1] == 0]
-    cations = [s for s in species if comp[s][1] == 1]
-    by_key = {}
-    for s in species:
-        by_key.setdefault(_key([s]), []).append(s)
     pairs = {}
     for i, a in enumerate(species):
         for b in species[i:]:

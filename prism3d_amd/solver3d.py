@@ -133,6 +133,14 @@ class PDRSolverGPU:
         # chemistry of every inner iteration, as upstream (solver_3d.py:142-143)
         self.accelerator = None
         self.chem_stats = None
+        # seconds of each phase of the last eager inner iteration, as upstream's
+        # self.timing (solver_3d.py:92, 246-376): 'RT', 'shielding' (H2/CO
+        # shielding and the cosmic-ray rate, one fused kernel here), 'dust',
+        # 'chemistry', 'thermal', 'dust_evolve'; measured with HIP events on the
+        # launch stream and read after the iteration's convergence test
+        self.timing = {}
+        self._phase_events = []
+        self.time_phases = True
         self.record_chem = False  # keep (start, end, stats) of each chemistry launch
         self.chem_records = []
         self.convergence_history = []
@@ -291,8 +299,37 @@ class PDRSolverGPU:
         """Device work of one inner iteration (solver_3d.py:262-380); returns the
         previous T, x_H2, x_CO and the new damping weights."""
         prev = self._before_chemistry(first)
-        self.chemistry_step()
+        self._phase('chemistry', self.chemistry_step)
         return self._after_chemistry(iteration, w_damp, *prev)
+
+    def _phase(self, name, fn, *a):
+        """Run fn(*a), bracketing its launches with timing events (eager
+        iterations only: no events are recorded while a graph is captured)."""
+        if not self.time_phases or torch.cuda.is_current_stream_capturing():
+            return fn(*a)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out = fn(*a)
+        e1.record()
+        self._phase_events.append((name, e0, e1))
+        return out
+
+    def _resolve_timing(self):
+        """Fold the recorded phase events into self.timing (after a sync)."""
+        seen = {}
+        for name, e0, e1 in self._phase_events:
+            seen[name] = seen.get(name, 0.0) + e0.elapsed_time(e1) * 1e-3
+        self.timing.update(seen)
+        self._phase_events = []
+
+    def print_timing(self):
+        """Phase breakdown of the last iteration (solver_3d.py:1192-1197)."""
+        total = sum(self.timing.values())
+        if total > 0:
+            print("\nTiming breakdown:", file=sys.stderr)
+            for k, v in sorted(self.timing.items(), key=lambda x: -x[1]):
+                print(f"  {k:15s}: {v:6.2f}s ({100 * v / total:5.1f}%)", file=sys.stderr)
 
     def _before_chemistry(self, first):
         """Radiation, shielding and dust of an inner iteration; returns the
@@ -300,11 +337,11 @@ class PDRSolverGPU:
         old_T = self.T_gas.clone()
         old_H2 = self.x_H2.clone()
         old_CO = self.x_CO.clone()
-        self.rt_step()
+        self._phase('RT', self.rt_step)
         if first and (self.chemistry == 'euler' or self.accelerator is not None):
             self._set_analytical_ic()
-        self.shielding_cr_step()
-        self.dust_step()
+        self._phase('shielding', self.shielding_cr_step)
+        self._phase('dust', self.dust_step)
         return old_T, old_H2, old_CO
 
     def _after_chemistry(self, iteration, w_damp, old_T, old_H2, old_CO):
@@ -329,7 +366,7 @@ class PDRSolverGPU:
             self._set_row('x_CO', 0.5 * old_CO + 0.5 * self.x_CO)
             self._set_row('x_Cp', torch.clamp(self.x_c_total - self.x_CO - self.x_C, min=1e-30))
         if self.fixed_T is None:
-            self.thermal_step()
+            self._phase('thermal', self.thermal_step)
             self.diagnostics()
             if iteration > 0:
                 rel = torch.abs(self.T_gas - old_T) / torch.clamp(old_T, min=10.0)
@@ -342,7 +379,9 @@ class PDRSolverGPU:
     def iterate(self, iteration, w_damp, first=False, convergence_tol=0.05):
         """One inner iteration; returns the convergence record."""
         old_T, old_H2, old_CO, w_damp = self._advance(iteration, w_damp, first)
-        return self._convergence(old_T, old_H2, old_CO, convergence_tol), w_damp
+        rec = self._convergence(old_T, old_H2, old_CO, convergence_tol)
+        self._resolve_timing()
+        return rec, w_damp
 
     def _capture(self, w_damp):
         """Records one (non-first) inner iteration as a HIP graph.  The state
@@ -407,7 +446,7 @@ class PDRSolverGPU:
         converged = False
         for dust_step in range(int(dust_steps)):
             if dust_step > 0:
-                self.evolve_dust()
+                self._phase('dust_evolve', self.evolve_dust)
             converged = False
             w_damp = torch.full(self.shape, 0.5, dtype=torch.float64, device=self.dev)
             for it in range(max_iterations):
@@ -429,12 +468,16 @@ class PDRSolverGPU:
                 rec['f_nano_mean'] = float(self.f_nano.mean())
                 self.convergence_history.append(rec)
                 if verbose:
+                    tm = self.timing
                     print(f"  {dust_step}.{it:2d} dT={rec['dT']:.2e} dH2={rec['dH2']:.2e} "
-                          f"dCO={rec['dCO']:.2e} n_osc={rec['n_oscillating']}",
-                          file=sys.stderr, flush=True)
+                          f"dCO={rec['dCO']:.2e} n_osc={rec['n_oscillating']} "
+                          f"[RT:{tm.get('RT', 0):.3f} chem:{tm.get('chemistry', 0):.3f} "
+                          f"therm:{tm.get('thermal', 0):.3f}]", file=sys.stderr, flush=True)
                 if rec['delta_max'] < convergence_tol and it > 2:
                     converged = True
                     break
+        if verbose:
+            self.print_timing()
         return converged
 
     # --- BDF refinement (solver_3d.py:1056-1144) -----------------------------
