@@ -252,8 +252,11 @@ def adjoint_measure(n=32, steps=2):
 KERNEL_BYTES = {
     "columns": 64, "point_source": 64, "shield_cr": 48, "pe_heating": 56, "dust_state": 56,
     "euler_rates": 112, "euler_chemistry": 160, "thermal_ctx": 184, "thermal": 88,
-    "heating_cooling": 144, "dust_evolve": 48, "surrogate": 128,
+    "heating_cooling": 144, "dust_evolve": 48, "surrogate": 128, "surrogate_mlp": 128,
 }
+# MFMA-bound lines: FP64 operations per cell (the upstream MLP architecture,
+# 7-128-64-32-1 per species, 2 per multiply-add, nine species)
+KERNEL_FLOPS = {"surrogate_mlp": 2 * 9 * (7 * 128 + 128 * 64 + 64 * 32 + 32)}
 KERNEL_NAMES = {  # the device kernels each operation launches
     "columns": "prefix_strided x2 + prefix_contig + ray_combine (p3d_fuv_columns)",
     "point_source": "point_source_kernel", "shield_cr": "shield_cr_kernel",
@@ -261,7 +264,7 @@ KERNEL_NAMES = {  # the device kernels each operation launches
     "euler_rates": "euler_rates_kernel", "euler_chemistry": "euler_kernel",
     "thermal_ctx": "thermal_ctx_kernel", "thermal": "thermal_kernel",
     "heating_cooling": "heat_cool_kernel", "dust_evolve": "dust_evolve_kernel",
-    "surrogate": "surrogate_kernel<6>",
+    "surrogate": "surrogate_kernel<6>", "surrogate_mlp": "surrogate_mlp_kernel",
 }
 
 
@@ -298,7 +301,7 @@ def kernel_lines(args):
     from prism3d_amd.density import PC, lognormal_median
     from prism3d_amd.device import stream_handle
     from prism3d_amd.solver3d import PDRSolverGPU
-    from prism3d_amd.surrogate import ChemistrySurrogate
+    from prism3d_amd.surrogate import ChemistrySurrogate, MLPChemistrySurrogate
 
     n = args.kn
     box = 0.5 * PC
@@ -315,6 +318,10 @@ def kernel_lines(args):
     forest = synthetic_forest(rng)
     sur = ChemistrySurrogate(forest, rng.normal(size=7), rng.uniform(0.5, 2, 7),
                              rng.uniform(-8, -2, 9), rng.uniform(0.2, 1, 9))
+    dims = [7, 128, 64, 32, 1]
+    mlps = [([rng.normal(size=(a, b)) * np.sqrt(2.0 / a) for a, b in zip(dims[:-1], dims[1:])],
+             [rng.normal(size=b) * 0.1 for b in dims[1:]]) for _ in range(9)]
+    mlp = MLPChemistrySurrogate(mlps, sur.x_mean, sur.x_scale, sur.y_mean, sur.y_scale)
     L = _lib.lib()
     flat = lambda t: t.reshape(-1)
     ps_out = [torch.empty_like(sv.density) for _ in range(5)]
@@ -364,6 +371,9 @@ def kernel_lines(args):
         "surrogate": (lambda: sur.predict_into(sur_out, flat(sv.density), flat(sv.T_gas),
                                                flat(sv.G0), flat(sv.A_V), flat(sv.zeta_CR),
                                                flat(sv.fsh_H2), flat(sv.fsh_CO)), None),
+        "surrogate_mlp": (lambda: mlp.predict_into(sur_out, flat(sv.density), flat(sv.T_gas),
+                                                   flat(sv.G0), flat(sv.A_V), flat(sv.zeta_CR),
+                                                   flat(sv.fsh_H2), flat(sv.fsh_CO)), None),
     }
     # the pe kernel alone: the driver's dust step minus its dust_state launch
     def pe_only():
@@ -432,9 +442,16 @@ def kernel_lines(args):
             forest, sur.x_mean, sur.x_scale, sur.y_mean, sur.y_scale,
             *[hs[nm][:4096] for nm in ('density', 'T_gas', 'G0', 'A_V', 'zeta_CR', 'fsh_H2',
                                        'fsh_CO')]), 4096, "numpy"),
+        "surrogate_mlp": (lambda: O.surrogate_predict_batch(
+            mlps, sur.x_mean, sur.x_scale, sur.y_mean, sur.y_scale,
+            *[hs[nm][:65536] for nm in ('density', 'T_gas', 'G0', 'A_V', 'zeta_CR', 'fsh_H2',
+                                        'fsh_CO')]), 65536, "numpy"),
     }
     del forest_mean
-    names = list(ops) if args.kernel == "all" else args.kernel.split(",")
+    # 'all' is the whole-grid set at 256^3; the MLP surrogate (~45 s per
+    # launch there) is run by name
+    names = [k for k in ops if k != "surrogate_mlp"] if args.kernel == "all" \
+        else args.kernel.split(",")
     for name in names:
         if name not in ops:
             raise SystemExit(f"unknown kernel {name!r}; one of {', '.join(ops)} or 'all'")
@@ -473,6 +490,11 @@ def kernel_lines(args):
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                              "bytes_per_cell": KERNEL_BYTES[name]}}
+        if name in KERNEL_FLOPS:
+            tf = KERNEL_FLOPS[name] * N / avg / 1e12
+            line["roofline"] = {"bound": "mfma", "achieved": tf, "peak": FP64_PEAK_TFLOPS,
+                                "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS,
+                                "traffic": None, "flops_per_cell": KERNEL_FLOPS[name]}
         if not args.no_cpu:
             fn, cells, kind = cpu[name]
             fn()  # first call builds / pages in

@@ -362,6 +362,20 @@ int p3d_surrogate_create(int32_t n_outputs, int32_t n_features, int32_t depth,
                          const double *x_scale, const double *y_mean,
                          const double *y_scale, int64_t *handle);
 
+/* Upload a multilayer-perceptron surrogate, the accelerator's 'mlp' model
+ * type (accelerator.py:267-272: one MLPRegressor per species, ReLU hidden
+ * layers, identity output), in the same handle space as the tree form.
+ * n_hidden (1..4) hidden layers of hidden[l] units (1..128 each).  weights
+ * (HOST, copied) holds, per species in output order and per layer in order,
+ * the layer's W (n_in x n_out, row-major, as MLPRegressor.coefs_[l]) then
+ * its b (n_out, as intercepts_[l]); 7 inputs, 1 output.  Inference is FP64
+ * on MFMA tiles of 16 cells.  Standardisations as for the tree form. */
+int p3d_surrogate_mlp_create(int32_t n_outputs, int32_t n_features,
+                             int32_t n_hidden, const int32_t *hidden,
+                             const double *weights, const double *x_mean,
+                             const double *x_scale, const double *y_mean,
+                             const double *y_scale, int64_t *handle);
+
 int p3d_surrogate_destroy(int64_t handle);
 
 /* Predicted abundances of every cell: out is a HOST array of 9 device

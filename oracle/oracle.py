@@ -683,12 +683,24 @@ def tree_ensemble_predict(model, k, X):
     return raw / float(model['divisor'][k])
 
 
+def mlp_predict(coefs, intercepts, X):
+    """MLPRegressor.predict for ReLU hidden layers and an identity output
+    (sklearn's forward pass: a = a @ W + b per layer, max(a, 0) between)."""
+    a = np.asarray(X, dtype=np.float64)
+    for i, (W, b) in enumerate(zip(coefs, intercepts)):
+        a = a @ np.asarray(W, dtype=np.float64) + np.asarray(b, dtype=np.float64)
+        if i < len(coefs) - 1:
+            a = np.maximum(a, 0.0)
+    return a[:, 0]
+
+
 def surrogate_predict_batch(model, x_mean, x_scale, y_mean, y_scale, n_H, T, G0, A_V, zeta,
                             fsh_H2, fsh_CO, x_c_total=GAS_C, x_o_total=GAS_O):
     """ChemistryAccelerator.predict_batch (accelerator.py:346-411): log
-    features with floors, standardised; per species ensemble prediction,
+    features with floors, standardised; per species model prediction,
     de-standardised, 10**y; H and C renormalised, O by difference, e- from the
-    cations.  Returns (9, n) in SURROGATE_SPECIES order."""
+    cations.  `model` is a packed tree ensemble, or a list of nine
+    (coefs, intercepts) MLPs.  Returns (9, n) in SURROGATE_SPECIES order."""
     X = np.column_stack([np.log10(np.maximum(n_H, 1)), np.log10(np.maximum(T, 5)),
                          np.log10(np.maximum(G0, 1e-2)), np.asarray(A_V, dtype=np.float64),
                          np.log10(np.maximum(zeta, 1e-18)), np.log10(np.maximum(fsh_H2, 1e-10)),
@@ -696,7 +708,11 @@ def surrogate_predict_batch(model, x_mean, x_scale, y_mean, y_scale, n_H, T, G0,
     Xs = (X - x_mean) / x_scale
     r = {}
     for k, sp in enumerate(SURROGATE_SPECIES):
-        r[sp] = 10 ** (tree_ensemble_predict(model, k, Xs) * y_scale[k] + y_mean[k])
+        if isinstance(model, dict):
+            y = tree_ensemble_predict(model, k, Xs)
+        else:
+            y = mlp_predict(model[k][0], model[k][1], Xs)
+        r[sp] = 10 ** (y * y_scale[k] + y_mean[k])
     hs = r['H'] + 2 * r['H2']
     hs = np.where(hs > 0, 1.0 / hs, 1.0)
     r['H'], r['H2'] = r['H'] * hs, r['H2'] * hs

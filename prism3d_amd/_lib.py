@@ -64,6 +64,7 @@ SIGNATURES = {
     "p3d_dust_evolve": (_i32, [_i64, _VP, _VP, _VP, _VP, _VP]),
     "p3d_surrogate_create": (_i32, [_i32, _i32, _i32, _I32P, _D, _D, _D, _I32P,
                                      ctypes.POINTER(ctypes.c_float), _D, _D, _D, _D, _D, _I64P]),
+    "p3d_surrogate_mlp_create": (_i32, [_i32, _i32, _i32, _I32P, _D, _D, _D, _D, _D, _I64P]),
     "p3d_surrogate_destroy": (_i32, [_i64]),
     "p3d_surrogate_predict": (_i32, [_i64, _i64] + [_VP] * 7 + [_dbl, _dbl,
                                      ctypes.POINTER(_VP), _VP]),

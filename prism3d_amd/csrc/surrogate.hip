@@ -29,6 +29,7 @@
 // (lane-major, no bank conflicts) and each node's lookup is one LDS read.
 // Four trees are walked together so four load chains are in flight.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -68,14 +69,119 @@ struct SurrogateOut {
   double *p[kSurOut];
 };
 
+// ---------------------------------------------------------------------------
+// Multilayer-perceptron form (the accelerator's 'mlp' model type,
+// accelerator.py:267-272: one regressor per species, ReLU hidden layers,
+// identity output, float64 throughout).  Every layer is a small GEMM over a
+// tile of 16 cells, so it runs on FP64 MFMA: a workgroup of four wavefronts
+// takes 16 cells, keeps their activations in LDS (16 x width), and splits
+// each layer's 16-wide output tiles over its wavefronts; one
+// v_mfma_f64_16x16x4f64 per tile and 4 inputs, the bias as the initial
+// accumulator, so every output sums bias, then the inputs in order.
+// Widths are padded with zero weights to multiples of 16 (outputs) and 4
+// (inputs); padded units stay 0 through the ReLU.
+constexpr int kMlpMaxLayers = 5;    // up to 4 hidden layers + the output layer
+constexpr int kMlpMaxWidth = 128;   // widest padded hidden layer
+constexpr int kMlpLd = kMlpMaxWidth + 4;  // LDS row stride (doubles): 2-way bank use
+constexpr int kMlpRows = 16;        // cells per workgroup
+constexpr int kMlpWaves = 4;
+
+struct MlpDev {
+  int n_layers;
+  int kpad[kMlpMaxLayers], npad[kMlpMaxLayers];  // padded in / out widths
+  int64_t woff[kMlpMaxLayers], boff[kMlpMaxLayers];  // within one species' block
+  int64_t blk;                                      // doubles per species
+  const double *w;
+};
+
 struct Surrogate {
   SurrogateDev d;
+  MlpDev m;
+  bool mlp = false;
   int depth = 0;
   void *blob = nullptr;
   ~Surrogate() {
     if (blob) (void)hipFree(blob);
   }
 };
+
+typedef double p3d_mlp_f64x4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(64 * kMlpWaves)
+surrogate_mlp_kernel(int64_t n, const double *__restrict__ nh, const double *__restrict__ tg,
+                     const double *__restrict__ g0, const double *__restrict__ av,
+                     const double *__restrict__ zeta, const double *__restrict__ fh2,
+                     const double *__restrict__ fco, const SurrogateDev S, const MlpDev M,
+                     double xc_tot, double xo_tot, const SurrogateOut out) {
+#pragma clang fp contract(off)
+  __shared__ double feat[kMlpRows * 8];
+  __shared__ double hid[2][kMlpRows * kMlpLd];
+  __shared__ double yres[kSurOut][kMlpRows];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lq = lane >> 4;
+  const int64_t c0 = (int64_t)blockIdx.x * kMlpRows;
+  if (t < kMlpRows) {
+    // rows past the end repeat the last cell; they are not stored
+    const int64_t i = c0 + t < n ? c0 + t : n - 1;
+    const double f[kSurFeat] = {log10(fmax(nh[i], 1.0)),    log10(fmax(tg[i], 5.0)),
+                                log10(fmax(g0[i], 1e-2)),   av[i],
+                                log10(fmax(zeta[i], 1e-18)), log10(fmax(fh2[i], 1e-10)),
+                                log10(fmax(fco[i], 1e-10))};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      feat[t * 8 + k] = k < kSurFeat ? (f[k] - S.x_mean[k]) / S.x_scale[k] : 0.0;
+  }
+  __syncthreads();
+  for (int o = 0; o < kSurOut; ++o) {
+    const double *wo = M.w + (int64_t)o * M.blk;
+    for (int l = 0; l < M.n_layers; ++l) {
+      const double *ain = l == 0 ? feat : hid[(l - 1) & 1];
+      const int lda = l == 0 ? 8 : kMlpLd;
+      double *aout = hid[l & 1];
+      const double *W = wo + M.woff[l], *B = wo + M.boff[l];
+      const int K = M.kpad[l], N = M.npad[l];
+      const bool last = l == M.n_layers - 1;
+      for (int nt = wave; nt < N / 16; nt += kMlpWaves) {  // wave-uniform
+        const double bias = B[nt * 16 + lr];
+        p3d_mlp_f64x4 acc = {bias, bias, bias, bias};
+        const double *wc = W + nt * 16 + lr;
+#pragma unroll 4
+        for (int k0 = 0; k0 < K; k0 += 4) {
+          const double a = ain[lr * lda + k0 + lq];
+          const double b = wc[(int64_t)(k0 + lq) * N];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+        if (!last) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) aout[(4 * r + lq) * kMlpLd + nt * 16 + lr] = fmax(acc[r], 0.0);
+        } else if (lr == 0) {  // output column 0 of tile 0: rows 4r + lq
+#pragma unroll
+          for (int r = 0; r < 4; ++r) yres[o][4 * r + lq] = acc[r];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (t < kMlpRows && c0 + t < n) {
+    const int64_t i = c0 + t;
+    double r[kSurOut];
+#pragma unroll
+    for (int o = 0; o < kSurOut; ++o) r[o] = pow(10.0, yres[o][t] * S.y_scale[o] + S.y_mean[o]);
+    // conservation of the batch prediction (accelerator.py:388-409)
+    const double hsum = r[0] + 2.0 * r[1];
+    const double hs = hsum > 0.0 ? 1.0 / hsum : 1.0;
+    r[0] = r[0] * hs;
+    r[1] = r[1] * hs;
+    const double csum = r[2] + r[3] + r[4];
+    const double cs = csum > 0.0 ? xc_tot / csum : 1.0;
+    r[2] = r[2] * cs;
+    r[3] = r[3] * cs;
+    r[4] = r[4] * cs;
+    r[5] = fmax(xo_tot - r[4] - r[7], 1e-30);
+    r[6] = r[2] + r[8];
+#pragma unroll
+    for (int k = 0; k < kSurOut; ++k) out.p[k][i] = r[k];
+  }
+}
 
 static std::mutex g_sur_mu;
 static std::map<int64_t, std::unique_ptr<Surrogate>> g_surs;
@@ -350,7 +456,86 @@ int p3d_surrogate_predict(int64_t handle, int64_t n, const double *n_h, const do
     o.p[k] = out[k];
   }
   const double *in[kSurFeat] = {n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co};
+  if (sur->mlp) {
+    const int64_t blocks = (n + kMlpRows - 1) / kMlpRows;
+    P3D_REQUIRE(blocks <= 0x7fffffffLL, "too many cells for one launch");
+    hipLaunchKernelGGL(surrogate_mlp_kernel, dim3((unsigned)blocks), dim3(64 * kMlpWaves), 0,
+                       as_stream(stream), n, in[0], in[1], in[2], in[3], in[4], in[5], in[6],
+                       sur->d, sur->m, x_c_total, x_o_total, o);
+    return launch_status("surrogate_mlp_kernel");
+  }
   return launch_surrogate(sur->depth, as_stream(stream), n, in, sur->d, x_c_total, x_o_total, o);
+}
+
+int p3d_surrogate_mlp_create(int32_t n_outputs, int32_t n_features, int32_t n_hidden,
+                             const int32_t *hidden, const double *weights, const double *x_mean,
+                             const double *x_scale, const double *y_mean, const double *y_scale,
+                             int64_t *handle) {
+  P3D_REQUIRE(handle, "handle pointer is required");
+  P3D_REQUIRE(n_outputs == kSurOut && n_features == kSurFeat,
+              "the surrogate maps %d features to %d species (got %d -> %d)", kSurFeat, kSurOut,
+              n_features, n_outputs);
+  P3D_REQUIRE(n_hidden >= 1 && n_hidden <= kMlpMaxLayers - 1,
+              "n_hidden must be in [1, %d] (got %d)", kMlpMaxLayers - 1, n_hidden);
+  P3D_REQUIRE(hidden && weights && x_mean && x_scale && y_mean && y_scale, "null host array");
+  int width[kMlpMaxLayers + 1];
+  width[0] = kSurFeat;
+  for (int l = 0; l < n_hidden; ++l) {
+    P3D_REQUIRE(hidden[l] >= 1 && hidden[l] <= kMlpMaxWidth,
+                "hidden layer %d width must be in [1, %d] (got %d)", l, kMlpMaxWidth, hidden[l]);
+    width[l + 1] = hidden[l];
+  }
+  width[n_hidden + 1] = 1;
+  auto sur = std::make_unique<Surrogate>();
+  sur->mlp = true;
+  MlpDev &m = sur->m;
+  m.n_layers = n_hidden + 1;
+  int64_t blk = 0;
+  for (int l = 0; l < m.n_layers; ++l) {
+    m.kpad[l] = l == 0 ? 8 : (width[l] + 15) / 16 * 16;  // inputs: the previous padded width
+    m.npad[l] = (width[l + 1] + 15) / 16 * 16;
+    m.woff[l] = blk;
+    blk += (int64_t)m.kpad[l] * m.npad[l];
+    m.boff[l] = blk;
+    blk += m.npad[l];
+  }
+  m.blk = blk;
+  // the caller's weights: per species, per layer W (in x out, row-major)
+  // then b (out), unpadded; laid out padded here
+  std::vector<double> host((size_t)(blk * kSurOut), 0.0);
+  const double *src = weights;
+  for (int o = 0; o < kSurOut; ++o) {
+    double *dst = host.data() + (size_t)o * blk;
+    for (int l = 0; l < m.n_layers; ++l) {
+      const int in = width[l], outw = width[l + 1];
+      for (int k = 0; k < in; ++k)
+        for (int j = 0; j < outw; ++j) dst[m.woff[l] + (int64_t)k * m.npad[l] + j] = *src++;
+      for (int j = 0; j < outw; ++j) dst[m.boff[l] + j] = *src++;
+    }
+  }
+  for (size_t k = 0; k < host.size(); ++k)
+    P3D_REQUIRE(std::isfinite(host[k]), "non-finite MLP weight");
+  SurrogateDev &d = sur->d;
+  for (int o = 0; o < kSurOut; ++o) {
+    d.y_mean[o] = y_mean[o];
+    d.y_scale[o] = y_scale[o];
+  }
+  for (int k = 0; k < kSurFeat; ++k) {
+    d.x_mean[k] = x_mean[k];
+    d.x_scale[k] = x_scale[k];
+  }
+  const size_t bytes = host.size() * sizeof(double);
+  if (hipMalloc(&sur->blob, bytes) != hipSuccess) {
+    set_error("hipMalloc of %zu bytes for the MLP weights failed", bytes);
+    return P3D_ENOMEM;
+  }
+  P3D_CHECK_HIP(hipMemcpy(sur->blob, host.data(), bytes, hipMemcpyHostToDevice));
+  m.w = static_cast<const double *>(sur->blob);
+  std::lock_guard<std::mutex> lk(g_sur_mu);
+  const int64_t h = g_next_sur++;
+  g_surs[h] = std::move(sur);
+  *handle = h;
+  return P3D_OK;
 }
 
 }  // extern "C"

@@ -154,7 +154,10 @@ class ChemistrySurrogate:
     @classmethod
     def from_models(cls, models, scaler_X, scaler_Y, **kw):
         """From nine fitted regressors (species order of SPECIES) and the two
-        standardisations (objects with mean_ and scale_)."""
+        standardisations (objects with mean_ and scale_): tree ensembles, or
+        multilayer perceptrons (an MLPChemistrySurrogate)."""
+        if models and all(hasattr(m, 'coefs_') for m in models):
+            return MLPChemistrySurrogate.from_models(models, scaler_X, scaler_Y, **kw)
         return cls(pack_models(models), scaler_X.mean_, scaler_X.scale_, scaler_Y.mean_,
                    scaler_Y.scale_, **kw)
 
@@ -173,6 +176,8 @@ class ChemistrySurrogate:
     @classmethod
     def load(cls, path, **kw):
         with np.load(path, allow_pickle=False) as d:
+            if 'mlp_hidden' in d:
+                return MLPChemistrySurrogate._from_npz(d, **kw)
             return cls({k: d[k] for k in PACKED_KEYS}, d['x_mean'], d['x_scale'], d['y_mean'],
                        d['y_scale'], **kw)
 
@@ -224,3 +229,71 @@ class ChemistrySurrogate:
         out = [torch.empty(n, dtype=torch.float64, device=dev) for _ in SPECIES]
         self.predict_into(out, *ins)
         return dict(zip(SPECIES, out))
+
+
+class MLPChemistrySurrogate(ChemistrySurrogate):
+    """The accelerator's 'mlp' model type (accelerator.py:267-272): nine
+    MLPRegressors (ReLU hidden layers, identity output), inferred in FP64 on
+    MFMA tiles (p3d_surrogate_mlp_create).  `mlps` holds, per species in
+    SPECIES order, (coefs, intercepts): the regressor's coefs_ (one
+    (n_in, n_out) array per layer) and intercepts_; every species shares the
+    hidden widths."""
+
+    def __init__(self, mlps, x_mean, x_scale, y_mean, y_scale, x_c_total=GAS_C,
+                 x_o_total=GAS_O):
+        if len(mlps) != len(SPECIES):
+            raise ValueError(f"need one MLP per species ({len(SPECIES)}), got {len(mlps)}")
+        self.mlps = [([np.asarray(w, dtype=np.float64) for w in c],
+                      [np.asarray(b, dtype=np.float64).reshape(-1) for b in i]) for c, i in mlps]
+        widths = [w.shape[1] for w in self.mlps[0][0][:-1]]
+        for c, i in self.mlps:
+            if len(c) != len(i) or [w.shape[1] for w in c[:-1]] != widths:
+                raise ValueError("every species' MLP needs the same hidden widths")
+            dims = [len(FEATURES)] + widths + [1]
+            for k, (w, b) in enumerate(zip(c, i)):
+                if w.shape != (dims[k], dims[k + 1]) or b.shape != (dims[k + 1],):
+                    raise ValueError(f"layer {k} has shape {w.shape} / {b.shape}, expected "
+                                     f"({dims[k]}, {dims[k + 1]})")
+        self.x_mean = np.ascontiguousarray(x_mean, dtype=np.float64).reshape(-1)
+        self.x_scale = np.ascontiguousarray(x_scale, dtype=np.float64).reshape(-1)
+        self.y_mean = np.ascontiguousarray(y_mean, dtype=np.float64).reshape(-1)
+        self.y_scale = np.ascontiguousarray(y_scale, dtype=np.float64).reshape(-1)
+        if self.x_mean.size != len(FEATURES) or self.x_scale.size != len(FEATURES):
+            raise ValueError(f"feature standardisation needs {len(FEATURES)} values")
+        if self.y_mean.size != len(SPECIES) or self.y_scale.size != len(SPECIES):
+            raise ValueError(f"output standardisation needs {len(SPECIES)} values")
+        self.x_c_total, self.x_o_total = float(x_c_total), float(x_o_total)
+        self.hidden = np.array(widths, dtype=np.int32)
+        flat = np.ascontiguousarray(np.concatenate(
+            [np.concatenate([np.concatenate([w.ravel(), b]) for w, b in zip(c, i)])
+             for c, i in self.mlps]))
+        h = ctypes.c_int64(0)
+        device()
+        rc = _lib.lib().p3d_surrogate_mlp_create(
+            len(SPECIES), len(FEATURES), len(widths), _lib.i32ptr(self.hidden), _lib.dptr(flat),
+            _lib.dptr(self.x_mean), _lib.dptr(self.x_scale), _lib.dptr(self.y_mean),
+            _lib.dptr(self.y_scale), ctypes.byref(h))
+        _lib.check(rc, "p3d_surrogate_mlp_create")
+        self.handle = h.value
+
+    @classmethod
+    def from_models(cls, models, scaler_X, scaler_Y, **kw):
+        return cls([(m.coefs_, m.intercepts_) for m in models], scaler_X.mean_, scaler_X.scale_,
+                   scaler_Y.mean_, scaler_Y.scale_, **kw)
+
+    def save(self, path):
+        arrs = {}
+        for s_, (c, i) in enumerate(self.mlps):
+            for k, (w, b) in enumerate(zip(c, i)):
+                arrs[f'mlp_{s_}_w{k}'] = w
+                arrs[f'mlp_{s_}_b{k}'] = b
+        np.savez_compressed(path, x_mean=self.x_mean, x_scale=self.x_scale, y_mean=self.y_mean,
+                            y_scale=self.y_scale, mlp_hidden=self.hidden, **arrs)
+
+    @classmethod
+    def _from_npz(cls, d, **kw):
+        n_layers = int(np.asarray(d['mlp_hidden']).size) + 1
+        mlps = [([d[f'mlp_{s_}_w{k}'] for k in range(n_layers)],
+                 [d[f'mlp_{s_}_b{k}'] for k in range(n_layers)]) for s_ in range(len(SPECIES))]
+        return cls(mlps, d['x_mean'], d['x_scale'], d['y_mean'], d['y_scale'], **kw)
+

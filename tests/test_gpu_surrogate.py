@@ -157,3 +157,63 @@ def test_blowout_preset_first_iteration_matches_oracle(gpu):
         own = _rel(sp[k].ravel(), st[k].ravel())
         assert np.median(rel) < 1e-12, k
         assert np.mean(rel > 1e-6) <= 1.5 * np.mean(own > 1e-6) + 0.01, k
+
+
+def _random_mlps(rng, widths):
+    dims = [7] + list(widths) + [1]
+    out = []
+    for _ in range(9):
+        coefs = [rng.normal(size=(a, b)) * np.sqrt(2.0 / a) for a, b in zip(dims[:-1], dims[1:])]
+        inter = [rng.normal(size=b) * 0.1 for b in dims[1:]]
+        out.append((coefs, inter))
+    return out
+
+
+@pytest.mark.parametrize("widths", [(128, 64, 32), (20, 7), (128,), (16, 16, 16, 16)])
+def test_mlp_surrogate_matches_oracle(gpu, widths):
+    """The accelerator's MLP model type on FP64 MFMA tiles against the
+    oracle's forward pass (pinned to MLPRegressor.predict), 20k random cells:
+    the upstream architecture (128, 64, 32) and widths that need padding."""
+    import torch
+    from oracle import oracle as O
+    from prism3d_amd.surrogate import SPECIES, ChemistrySurrogate, MLPChemistrySurrogate
+    print(f"case: MLP surrogate {widths}", flush=True)
+    rng = np.random.RandomState(len(widths) * 7 + widths[0])
+    mlps = _random_mlps(rng, widths)
+    xm, xs = rng.normal(size=7), rng.uniform(0.5, 2, 7)
+    ym, ys = rng.uniform(-8, -2, 9), rng.uniform(0.2, 1.0, 9)
+    s = MLPChemistrySurrogate(mlps, xm, xs, ym, ys)
+    n = 20000 + 5  # not a multiple of the 16-cell tile
+    ins = [10 ** rng.uniform(0, 7, n), 10 ** rng.uniform(0.5, 4, n), 10 ** rng.uniform(-3, 5, n),
+           rng.uniform(0, 20, n), 10 ** rng.uniform(-19, -15, n), 10 ** rng.uniform(-11, 0, n),
+           10 ** rng.uniform(-11, 0, n)]
+    out = s.predict_batch(*ins)
+    ref = O.surrogate_predict_batch(mlps, xm, xs, ym, ys, *ins)
+    from prism3d_amd.chemistry import GAS_O
+    for k, sp in enumerate(SPECIES):
+        got = out[sp].cpu().numpy()
+        if sp == 'O':  # x_O_total - CO - OH: rounding relative to the total
+            assert np.max(np.abs(got - ref[k])) < 1e-12 * GAS_O
+        else:
+            rel = _rel(got, ref[k])
+            assert rel.max() < 1e-12, (sp, rel.max())
+    if widths == (128, 64, 32):
+        import tempfile, os
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "mlp.npz")
+            s.save(p)
+            s2 = ChemistrySurrogate.load(p)
+            assert isinstance(s2, MLPChemistrySurrogate)
+            b = s2.predict_batch(*ins)
+            for sp in SPECIES:
+                assert torch.equal(b[sp], out[sp])
+            s2.close()
+    s.close()
+
+
+def test_mlp_surrogate_rejects_bad_models(gpu):
+    from prism3d_amd.surrogate import MLPChemistrySurrogate
+    rng = np.random.RandomState(0)
+    mlps = _random_mlps(rng, (256,))
+    with pytest.raises(ValueError):  # wider than the 128-unit tile budget
+        MLPChemistrySurrogate(mlps, np.zeros(7), np.ones(7), np.zeros(9), np.ones(9))

@@ -125,3 +125,29 @@ def test_cloud_ensemble_parameters_and_fields():
         assert abs(np.std(ln) - sig) < 1e-9 * sig + 1e-6 or np.any(d == 1.0)
         assert abs(np.median(ln) - np.log(med)) < 0.5 * sig
     assert [e['seed'] for e in cloud_ensemble(3, 8)] == [0, 1, 2]
+
+
+def test_mlp_oracle_equals_sklearn_predict():
+    """The oracle's MLP forward pass (the checker of the MFMA MLP surrogate)
+    against MLPRegressor.predict of a fitted network of the accelerator's
+    architecture, and the host-side shape checks of MLPChemistrySurrogate."""
+    import warnings
+    import pytest
+    from sklearn.neural_network import MLPRegressor
+    from oracle import oracle as O
+    from prism3d_amd.surrogate import MLPChemistrySurrogate
+    rng = np.random.RandomState(0)
+    X = rng.normal(size=(300, 7))
+    y = np.sin(X[:, 0]) + X[:, 1] * X[:, 2]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        m = MLPRegressor(hidden_layer_sizes=(128, 64, 32), activation='relu', max_iter=30,
+                         random_state=42).fit(X, y)
+    Xt = rng.normal(size=(500, 7))
+    assert np.array_equal(m.predict(Xt), O.mlp_predict(m.coefs_, m.intercepts_, Xt))
+    bad = [(m.coefs_, m.intercepts_)] * 8  # one short
+    with pytest.raises(ValueError):
+        MLPChemistrySurrogate(bad, np.zeros(7), np.ones(7), np.zeros(9), np.ones(9))
+    wrong = [([c.T for c in m.coefs_], m.intercepts_)] * 9
+    with pytest.raises(ValueError):
+        MLPChemistrySurrogate(wrong, np.zeros(7), np.ones(7), np.zeros(9), np.ones(9))
