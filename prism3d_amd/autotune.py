@@ -8,17 +8,21 @@ many steps their cells need, so time alone does not compare them), and then
 keeps the best.  Choices are cached per (device, network, cell-count bucket)
 in a small JSON file so later runs start with the tuned configuration.
 
-A configuration is (variant, waves_per_cu, rescue_frac) of p3d_solve_opts:
-variant 2 is one cell per 64-lane workgroup, variant 4 the persistent
-one-cell-per-lane kernel with `waves_per_cu` resident wavefronts per CU that
-hands its long-tail cells over once `rescue_frac` of its lanes are idle.
+A configuration is (variant, waves_per_cu, rescue_frac) of p3d_solve_opts,
+so the tuner picks both the cells per workgroup and the LU shape: variants
+2 and 3 are one cell per 64-lane workgroup with the LU in registers (lane =
+row) or in LDS with unrolled elimination steps; variant 4 is the persistent
+one-cell-per-lane kernel (64 cells per wavefront, statically ordered sparse
+LU) with `waves_per_cu` resident wavefronts per CU that hands its long-tail
+cells over once `rescue_frac` of its lanes are idle.
 """
 
 import json
 import math
 import os
 
-DEFAULT_CANDIDATES = ((4, 8, 0.95), (4, 8, 0.98), (4, 8, 0.99), (4, 12, 0.95), (2, 0, 0.0))
+DEFAULT_CANDIDATES = ((4, 8, 0.95), (4, 8, 0.98), (4, 8, 0.99), (4, 12, 0.95), (2, 0, 0.0),
+                      (3, 0, 0.0))
 
 
 def _cache_path():

@@ -107,8 +107,8 @@ def test_lane_matches_workgroup_kernel_on_mixed_grid(gpu):
 
 
 def test_driver_autotune_picks_a_config_and_keeps_results(gpu, tmp_path, monkeypatch):
-    """16^3 slab, network chemistry, 7 fixed-point iterations with the online
-    tuner (warm-up + 5 candidates + tuned): a configuration is chosen, and the
+    """16^3 slab, network chemistry, 8 fixed-point iterations with the online
+    tuner (warm-up + 6 candidates + tuned): a configuration is chosen, and the
     state matches the untuned run to the solver tolerance."""
     from oracle import oracle as O
     from prism3d_amd.solver3d import PDRSolverGPU
@@ -119,10 +119,10 @@ def test_driver_autotune_picks_a_config_and_keeps_results(gpu, tmp_path, monkeyp
     for tune in (False, True):
         sv = PDRSolverGPU(dens, 16 * cell, G0_external=10.0, zeta_CR_0=5e-17, nside_rt=1,
                           chemistry='network', chem_autotune=tune)
-        sv.run(max_iterations=7, convergence_tol=0.0)
+        sv.run(max_iterations=8, convergence_tol=0.0)
         runs.append(sv)
     a, b = runs
-    assert b.tuner is not None and not b.tuner.tuning and len(b.tuner.scores) == 5
+    assert b.tuner is not None and not b.tuner.tuning and len(b.tuner.scores) == 6
     for k in ('T_gas', 'x_H2', 'x_CO', 'x_e'):
         x, y = getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy()
         m = np.abs(x) > 1e-14
