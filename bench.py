@@ -581,6 +581,66 @@ def clouds_line(args):
     print(json.dumps(out), flush=True)
 
 
+def forms_line(args):
+    """One JSON line: the chemistry solve of the batching grid (args.forms^3
+    log-normal cube, sigma_ln 1.5, n_H clipped to 1e2-1e6, median 1e4,
+    0.5 pc, G0 1e4 from -x over 48 rays) by the persistent one-cell-per-lane
+    kernel and by the one-cell-per-workgroup drop-in, on the same inputs:
+    the first fixed-point iteration, and a warm one after two iterations."""
+    import torch
+    from prism3d_amd.chemistry import solve_opts
+    from prism3d_amd.density import PC, lognormal_median
+    from prism3d_amd.grid import directional_G0
+    from prism3d_amd.solver3d import PDRSolverGPU
+    n = args.forms
+    dens, _ = lognormal_median(n, 0.5 * PC, 1e4, 1.5, seed=42, clip=(1e2, 1e6))
+    sv = PDRSolverGPU(dens, 0.5 * PC, G0_external=directional_G0(1e4, [-1.0, 0.0, 0.0], nside=2),
+                      zeta_CR_0=2e-16, nside_rt=2, chemistry='network', x_c_total=1.4e-4,
+                      x_o_total=3.0e-4,
+                      initial_abundances={'H': 1.0, 'C+': 1.4e-4, 'O': 3.0e-4, 'e-': 1.4e-4},
+                      T_init=50.0, chem_atol=1e-20)
+
+    def solve(var, y0):
+        ins = [t.reshape(-1) for t in (sv.density, sv.T_gas, sv.G0, sv.A_V, sv.zeta_CR,
+                                       sv.fsh_H2, sv.fsh_CO)]
+        y = y0.clone()
+        o = solve_opts(t_max=None, t_start=1e4, n_checkpoints=2, early_stop=False, rtol=1e-9,
+                       atol=1e-20, post_mode=-1, variant=var)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        _, st = sv.chem.solve_steady_batch(y, *ins, opts=o, sync=False)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e-3, y, st.cpu().numpy()
+
+    sv.rt_step(); sv.shielding_cr_step(); sv.dust_step()
+    res = {}
+    for tag in ("iteration_0", "warm"):
+        if tag == "warm":
+            sv.run(max_iterations=2, convergence_tol=0.0)
+            sv.rt_step(); sv.shielding_cr_step(); sv.dust_step()
+        y0 = sv.y.clone()
+        tl, yl, stl = solve(4, y0)
+        tw, yw, _ = solve(2, y0)
+        m = yw > 1e-14
+        rel = (torch.abs(yl - yw)[m] / yw[m]).cpu().numpy()
+        res[tag] = {"s_batched": tl, "s_drop_in": tw, "ratio": tw / tl,
+                    "bdf_steps_mean": float(stl[:, 0].mean()), "bdf_steps_max": int(stl[:, 0].max()),
+                    "p999_rel_diff": float(np.percentile(rel, 99.9))}
+        print(f"[bench] {n}^3 {tag}: batched {tl:.2f} s, drop-in {tw:.2f} s", file=sys.stderr,
+              flush=True)
+    out = {"metric": "batched / plain drop-in chemistry speed-up, batching grid",
+           "value": res["warm"]["ratio"], "unit": "x", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{n}^3 log-normal cube (sigma_ln 1.5, n_H clipped to 1e2-1e6, "
+                                  "median 1e4, 0.5 pc), G0 1e4 from -x, 48 rays, 31-species "
+                                  "network, steady-state BDF at rtol 1e-9",
+                      "cells": sv.n_cells},
+           "solves": res}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -607,6 +667,9 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (a known-traffic "
                          "kernel that calibrates profiler byte counters)")
+    ap.add_argument("--forms", type=int, default=0,
+                    help="batched vs drop-in line on the batching grid of this many cells "
+                         "per side (128: the batching configuration)")
     ap.add_argument("--synth-cells", type=int, default=0,
                     help="64-species synthetic network line instead of the headline: this "
                          "many cells")
@@ -635,6 +698,9 @@ def main():
         return
     if args.synth_cells > 0:
         synth_line(args)
+        return
+    if args.forms > 0:
+        forms_line(args)
         return
     if args.adjoint_only:
         adj = adjoint_measure(args.adjoint_n)
