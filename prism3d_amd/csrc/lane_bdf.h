@@ -296,6 +296,7 @@ struct Lane {
   double t, t1, h, max_step, hmin, tn, c, c_lu, dy_old, pend;
   int order, n_equal, it, steps_span, nres;
   int jac_current, lu_valid, need_lu, need_jac;
+  int fresh;  // Newton iterate = predictor and correction = 0, not yet stored
   // counters over the whole cell
   int st_steps, st_lu, st_rhs, st_jac;
 };
@@ -510,8 +511,6 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
           for (int j = 1; j <= kMaxOrder; ++j)
             if (j <= L.order) ps += ndf_gamma(j) * Dv[q][j];
           m.yp(c0 + q) = yp;
-          m.yn(c0 + q) = yp;
-          m.dd(c0 + q) = 0.0;
           m.psi(c0 + q) = ps / al;
         }
       }
@@ -520,23 +519,14 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
     L.need_lu = (!L.lu_valid || L.c != L.c_lu) ? 1 : 0;
     L.it = 0;
     L.dy_old = -1.0;
+    L.fresh = 1;
     L.phase = kNewton;
   }
   // ---- Jacobian refresh at the predictor after a failed Newton solve ------
   P3D_FENCE();
   if (L.phase == kNewton && L.need_jac) {
     jacobian(L.p, m, kOffYP);
-    P3D_FENCE();
-    {
-      double yv[kNS];
-#pragma unroll
-      for (int s = 0; s < kNS; ++s) yv[s] = m.yp(s);
-#pragma unroll
-      for (int s = 0; s < kNS; ++s) {
-        m.yn(s) = yv[s];
-        m.dd(s) = 0.0;
-      }
-    }
+    L.fresh = 1;
     L.st_jac++;
     L.jac_current = 1;
     L.lu_valid = 0;
@@ -564,7 +554,7 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
     P3D_FENCE();
     if (L.phase == kNewton && L.nres == 0) {
       double f[kNS];
-      rhs(L.p, m, kOffYN, f);
+      rhs(L.p, m, L.fresh ? kOffYP : kOffYN, f);
       L.st_rhs++;
       double fs = 0.0;
 #pragma unroll
@@ -573,7 +563,11 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
         L.nres = 2;
       } else {
 #pragma unroll
-        for (int s = 0; s < kNS; ++s) f[s] = L.c * f[s] - m.psi(s) - m.dd(s);
+        for (int s = 0; s < kNS; ++s) f[s] = L.c * f[s] - m.psi(s);
+        if (!L.fresh) {
+#pragma unroll
+          for (int s = 0; s < kNS; ++s) f[s] -= m.dd(s);
+        }
         lu_solve(m, f);
         double sq = 0.0;
 #pragma unroll
@@ -590,9 +584,15 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
 #pragma unroll
           for (int c0 = 0; c0 < kNS; c0 += kChunk8) {
             double yv[kChunk8], dv[kChunk8];
+            if (L.fresh) {
 #pragma unroll
-            for (int q = 0; q < kChunk8; ++q)
-              if (c0 + q < kNS) { yv[q] = m.yn(c0 + q); dv[q] = m.dd(c0 + q); }
+              for (int q = 0; q < kChunk8; ++q)
+                if (c0 + q < kNS) { yv[q] = m.yp(c0 + q); dv[q] = 0.0; }
+            } else {
+#pragma unroll
+              for (int q = 0; q < kChunk8; ++q)
+                if (c0 + q < kNS) { yv[q] = m.yn(c0 + q); dv[q] = m.dd(c0 + q); }
+            }
 #pragma unroll
             for (int q = 0; q < kChunk8; ++q)
               if (c0 + q < kNS) {
@@ -600,6 +600,7 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
                 m.dd(c0 + q) = dv[q] + f[c0 + q];
               }
           }
+          L.fresh = 0;
           if (dyn == 0.0 || (rate >= 0.0 && rate / (1.0 - rate) * dyn < o.newton_tol)) {
             L.nres = 1;
           } else {
