@@ -297,6 +297,7 @@ struct Lane {
   int order, n_equal, it, steps_span, nres;
   int jac_current, lu_valid, need_lu, need_jac;
   int fresh;  // Newton iterate = predictor and correction = 0, not yet stored
+  double err_sq;  // error-test sum of squares of the converged iterate
   // counters over the whole cell
   int st_steps, st_lu, st_rhs, st_jac;
 };
@@ -581,6 +582,12 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
             (rate >= 1.0 || pow(rate, (double)(4 - L.it)) / (1.0 - rate) * dyn > o.newton_tol)) {
           L.nres = 2;
         } else {
+          // converged with this update: the error test's sum is taken on the
+          // updated iterate and correction while they are in registers
+          const bool conv =
+              dyn == 0.0 || (rate >= 0.0 && rate / (1.0 - rate) * dyn < o.newton_tol);
+          const double ecq = ndf_err(L.order);
+          double esq = 0.0;
 #pragma unroll
           for (int c0 = 0; c0 < kNS; c0 += kChunk8) {
             double yv[kChunk8], dv[kChunk8];
@@ -596,12 +603,19 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
 #pragma unroll
             for (int q = 0; q < kChunk8; ++q)
               if (c0 + q < kNS) {
-                m.yn(c0 + q) = yv[q] + f[c0 + q];
-                m.dd(c0 + q) = dv[q] + f[c0 + q];
+                const double yn = yv[q] + f[c0 + q], dn = dv[q] + f[c0 + q];
+                m.yn(c0 + q) = yn;
+                m.dd(c0 + q) = dn;
+                if (conv) {
+                  const double sc = o.atol + o.rtol * fabs(yn);
+                  const double e = ecq * dn;
+                  esq += (e / sc) * (e / sc);
+                }
               }
           }
           L.fresh = 0;
-          if (dyn == 0.0 || (rate >= 0.0 && rate / (1.0 - rate) * dyn < o.newton_tol)) {
+          if (conv) {
+            L.err_sq = esq;
             L.nres = 1;
           } else {
             L.dy_old = dyn;
@@ -631,14 +645,7 @@ P3D_HD P3D_INLINE void trip(Lane &L, const S &m, const Tables &tb, const Opts &o
     }
   } else if (L.phase == kNewton && L.nres == 1) {
     const double ecq = ndf_err(L.order);
-    double sq = 0.0;
-#pragma unroll
-    for (int s = 0; s < kNS; ++s) {
-      const double sc = o.atol + o.rtol * fabs((double)m.yn(s));
-      const double e = ecq * m.dd(s);
-      sq += (e / sc) * (e / sc);
-    }
-    const double en = rms_sum(sq);
+    const double en = rms_sum(L.err_sq);
     if (en > 1.0) {
       const double factor = fmax(0.2, 0.9 * pow(en, -1.0 / (L.order + 1)));
       L.h *= factor;
