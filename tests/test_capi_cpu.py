@@ -108,3 +108,24 @@ def test_package_fails_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(ImportError, match="build the HIP library"):
         _lib.lib()
+
+
+def test_surrogate_creation_validation_without_gpu():
+    """The surrogate uploads reject bad shapes before any device call."""
+    from prism3d_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_int64(0)
+    z = np.zeros(16)
+    w = np.zeros(64)
+    hid = np.array([128, 64, 32], dtype=np.int32)
+    args = lambda nh, hidden: (9, 7, nh, _lib.i32ptr(hidden), _lib.dptr(w), _lib.dptr(z),
+                               _lib.dptr(z), _lib.dptr(z), _lib.dptr(z), ctypes.byref(h))
+    assert L.p3d_surrogate_mlp_create(*args(0, hid)) == -1
+    assert b"n_hidden" in L.p3d_last_error()
+    assert L.p3d_surrogate_mlp_create(*args(5, np.full(5, 16, dtype=np.int32))) == -1
+    assert L.p3d_surrogate_mlp_create(*args(1, np.array([129], dtype=np.int32))) == -1
+    assert b"width" in L.p3d_last_error()
+    rc = L.p3d_surrogate_mlp_create(8, 7, 3, _lib.i32ptr(hid), _lib.dptr(w), _lib.dptr(z),
+                                    _lib.dptr(z), _lib.dptr(z), _lib.dptr(z), ctypes.byref(h))
+    assert rc == -1 and b"species" in L.p3d_last_error()
+    assert h.value == 0
