@@ -115,7 +115,9 @@ int p3d_pe_heating_vec(int64_t n_cells, const double *pe_table, int64_t n_bins,
 
 /* Upload a network.  reactants: 2 per reaction, products: 3 per reaction
  * (P3D_SP_NONE for unused slots); e_index is the electron species (its
- * abundance sets the PAH charge state).  n_species <= 64.
+ * abundance sets the PAH charge state).  n_species <= 128; above 64 species
+ * only p3d_chem_solve (variant 0 or 5) takes the network, and the entry
+ * points that map one species to one lane of a wavefront refuse it.
  * roles7 = species indices of H, H2, H+, C+, C, CO, O used by the post-solve
  * corrections (solver.py:227-388), -1 when the network has none (then only
  * post_mode -1 is accepted); post_w4 = four weight vectors of n_species each:
@@ -164,12 +166,13 @@ typedef struct p3d_solve_opts {
   double x_c_total, x_o_total; /* element totals for the conservation step */
   int32_t variant;     /* kernel variant: 0 auto (4 when the network is the
                           bundled one and n >= resident lanes, else 2 for
-                          n_species <= 32, 5 for n_species == 64, else 1),
+                          n_species <= 32, 5 for n_species >= 64, else 1),
                           1 LU in LDS, 2 LU in registers
                           (n_species <= 32), 3 LU in LDS with unrolled loops
                           (n_species <= 32), 5 LU in LDS with the panel
                           trailing updates on FP64 MFMA: one cell per 64-lane
-                          workgroup; 4 persistent one-cell-per-lane kernel
+                          workgroup (above 64 species: one cell per two
+                          wavefronts, J sparse in LDS); 4 persistent one-cell-per-lane kernel
                           with a device work queue (the bundled network
                           structure only) */
   int32_t waves_per_cu; /* variant 4: resident wavefronts per compute unit
@@ -186,7 +189,7 @@ typedef struct p3d_solve_opts {
 
 /* Integrate every cell to its steady state / end time with the
  * variable-order BDF (orders 1-5, Newton with dense partially pivoted LU,
- * one cell per 64-lane workgroup).  y (n_species, n) is updated in place;
+ * one cell per 64-lane workgroup, or 128 lanes above 64 species).  y (n_species, n) is updated in place;
  * converged[i] = early-stop flag; stats (n x 4, row per cell) = accepted BDF
  * steps, LU factorisations, RHS evaluations, Jacobian evaluations (either
  * output may be NULL).  Replaces ChemistrySolver.solve_steady_state

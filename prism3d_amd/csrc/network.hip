@@ -31,7 +31,8 @@
 
 namespace p3d {
 
-constexpr int kMaxSpecies = 64;
+constexpr int kMaxSpecies = 64;       // one-wavefront forms: lane = species
+constexpr int kMaxSpeciesWide = 128;  // two-wavefront form of the solve
 constexpr int kMaxOrder = 5;
 constexpr int kNumD = kMaxOrder + 3;
 
@@ -91,6 +92,8 @@ struct Lds {
   double *ru;    // 36: step-size change matrix RU of the backward differences
   int *csr_off;  // 65: per-species offsets into csr (LDS copy of rhs_off)
   int *csr;      // n_rhs_e packed RHS entries (LDS copy of rhs_pk)
+  double *red;   // two-wavefront form: 4 doubles of cross-wave reduction slots
+  double *xs;    // two-wavefront form: 128-entry vector of the triangular solves
   int ld;
 };
 
@@ -106,6 +109,28 @@ __device__ __forceinline__ Lds carve(double *base, const NetDev &N) {
   L.ru = reinterpret_cast<double *>(L.piv + kMaxSpecies);
   L.csr_off = reinterpret_cast<int *>(L.ru + 36);
   L.csr = L.csr_off + kMaxSpecies + 1;
+  L.red = nullptr;
+  L.xs = nullptr;
+  return L;
+}
+
+// LDS of one cell in the two-wavefront form (up to 128 species): J is kept
+// sparse (its nnz structural entries, in the order of jrow/jcol) beside the
+// dense M, and the RHS lists are read from global memory (L2).
+__device__ __forceinline__ Lds carve_wide(double *base, const NetDev &N) {
+  Lds L;
+  L.ld = N.ns | 1;
+  L.kco = base;
+  L.part = L.kco + N.nr;
+  L.ys = L.part + 2 * N.nr;
+  L.J = L.ys + kMaxSpeciesWide;
+  L.M = L.J + N.nnz;
+  L.red = L.M + N.ns * L.ld;
+  L.xs = L.red + 4;
+  L.ru = L.xs + kMaxSpeciesWide;
+  L.piv = reinterpret_cast<int *>(L.ru + 36);
+  L.csr_off = nullptr;
+  L.csr = nullptr;
   return L;
 }
 
@@ -113,6 +138,66 @@ static size_t lds_bytes(int ns, int nr, int n_rhs_e) {
   int ld = ns | 1;
   return sizeof(double) * (size_t)(3 * nr + kMaxSpecies + 2 * ns * ld + 36) +
          sizeof(int) * (size_t)(2 * kMaxSpecies + 1 + n_rhs_e);
+}
+
+static size_t lds_bytes_wide(int ns, int nr, int nnz) {
+  const int ld = ns | 1;
+  return sizeof(double) * (size_t)(3 * nr + 2 * kMaxSpeciesWide + nnz + ns * ld + 4 + 36) +
+         sizeof(int) * (size_t)kMaxSpeciesWide;
+}
+
+// Workgroup collectives.  Wave1: one wavefront, the DPP/readlane collectives.
+// Wave2: two wavefronts (128 lanes); each wavefront reduces on its own and
+// the two results combine through LDS in a fixed order (wavefront 0 first),
+// so every lane of both gets the same bits and branches on them stay
+// uniform across the workgroup.  Every lane of the workgroup must call.
+struct Wave1 {
+  static constexpr int W = 64;
+  double *red;
+  __device__ __forceinline__ double sum(double v) const { return wave_sum(v); }
+  __device__ __forceinline__ double bc(double v, int src) const { return bcast(v, src); }
+  __device__ __forceinline__ int argmax(double v, int lane) const {
+    return wave_argmax(v, lane);
+  }
+};
+struct Wave2 {
+  static constexpr int W = 128;
+  double *red;  // 4 doubles of LDS
+  __device__ __forceinline__ double sum(double v) const {
+    const double s = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    const double r = red[0] + red[1];
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ double bc(double v, int src) const {
+    if ((int)threadIdx.x == src) red[0] = v;
+    __syncthreads();
+    const double r = red[0];
+    __syncthreads();
+    return r;
+  }
+  // largest v and the lowest lane holding it
+  __device__ __forceinline__ int argmax(double v, int lane) const {
+    const int i = wave_argmax(v, lane);
+    const double bv = bcast(v, i & 63);
+    int *ri = reinterpret_cast<int *>(red + 2);
+    if ((lane & 63) == 0) {
+      red[lane >> 6] = bv;
+      ri[lane >> 6] = i;
+    }
+    __syncthreads();
+    const double v0 = red[0], v1 = red[1];
+    const int i0 = ri[0], i1 = ri[1];
+    __syncthreads();
+    return uniform(v1 > v0 ? i1 : i0);  // on a tie wavefront 0 holds the lower lane
+  }
+};
+
+template <class G>
+__device__ __forceinline__ double rms_g(const G &g, double v, int ns) {
+  return sqrt(g.sum(v * v) / ns);
 }
 
 // Copies the RHS stoichiometry lists into LDS (once per workgroup).
@@ -157,11 +242,13 @@ __device__ __forceinline__ double rate_coeff_f32(int t, double a, double b, doub
 }
 
 // Rate coefficients k_r (solver.py:100-133); f32: exponentials in FP32.
+// W: lanes of the workgroup.
+template <int W = 64>
 __device__ void rate_coeffs(const NetDev &N, const CellParams &p, const Lds &L, int lane,
                             bool f32 = false) {
   const double Tc = fmin(fmax(p.T, 10.0), 41000.0);
   const double lTc = log(Tc / 300.0), lT = log(p.T / 300.0);
-  for (int r = lane; r < N.nr; r += 64) {
+  for (int r = lane; r < N.nr; r += W) {
     const double a = N.alpha[r], b = N.beta[r], g = N.gamma[r];
     if (f32) {
       L.kco[r] = rate_coeff_f32(N.rtype[r], a, b, g, p);
@@ -205,12 +292,13 @@ __device__ __forceinline__ void publish_y(const NetDev &N, const Lds &L, double 
 }
 
 // Reaction rates into part[2r] (want_jac: partials into part[2r], part[2r+1]
-// instead).  Requires ys published.
+// instead).  Requires ys published.  W: lanes of the workgroup.
+template <int W = 64>
 __device__ void eval_reactions(const NetDev &N, const CellParams &p, const Lds &L, int lane,
                                bool want_jac) {
   double pm, dpm;
   pah_minus(L.ys[N.e_idx], p, pm, dpm);
-  for (int r = lane; r < N.nr; r += 64) {
+  for (int r = lane; r < N.nr; r += W) {
     const double k = L.kco[r];
     const int t = N.rtype[r];
     const int a = N.react[2 * r], b = N.react[2 * r + 1];
@@ -301,8 +389,9 @@ __device__ __forceinline__ double rhs_cached(const NetDev &N, const CellParams &
 }
 
 // dy/dt of species `lane` (upstream accumulation order).  Requires y published.
+template <int W = 64>
 __device__ double rhs_lane(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
-  eval_reactions(N, p, L, lane, false);
+  eval_reactions<W>(N, p, L, lane, false);
   double acc = 0.0;
   if (lane < N.ns) {
     const int e0 = N.rhs_off[lane], e1 = N.rhs_off[lane + 1];
@@ -324,6 +413,27 @@ __device__ void jacobian(const NetDev &N, const CellParams &p, const Lds &L, int
     L.J[N.jrow[z] * L.ld + N.jcol[z]] = acc;
   }
   __syncthreads();
+}
+
+// The Jacobian's structural entries into L.J[z] (z in the order of
+// jrow/jcol), the same sums as the dense form.  Requires y published.
+template <int W>
+__device__ void jacobian_sparse(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
+  eval_reactions<W>(N, p, L, lane, true);
+  for (int z = lane; z < N.nnz; z += W) {
+    double acc = 0.0;
+    for (int t = N.joff[z]; t < N.joff[z + 1]; ++t)
+      acc += N.jt_coef[t] * L.part[2 * N.jt_rxn[t] + N.jt_slot[t]];
+    L.J[z] = acc;
+  }
+  __syncthreads();
+}
+
+template <class G>
+__device__ __forceinline__ void jac_eval(const NetDev &N, const CellParams &p, const Lds &L,
+                                         int lane) {
+  if constexpr (G::W == 64) jacobian(N, p, L, lane);
+  else jacobian_sparse<G::W>(N, p, L, lane);
 }
 
 __device__ void lu_factor_inplace(const NetDev &N, const Lds &L, int lane);
@@ -389,6 +499,45 @@ __device__ double lu_solve(const NetDev &N, const Lds &L, double b, int lane) {
   return b;
 }
 
+// lu_solve for the two-wavefront form: the row interchanges are applied by
+// one lane to an LDS copy of b, then each substitution step publishes one
+// solved component through its own LDS slot (one barrier per step).  Same
+// operations in the same order as lu_solve.
+__device__ double lu_solve_wide(const NetDev &N, const Lds &L, double b, int lane) {
+  const int n = N.ns, ld = L.ld;
+  double *x = L.xs;
+  if (lane < n) x[lane] = b;
+  __syncthreads();
+  if (lane == 0)
+    for (int k = 0; k < n; ++k) {
+      const int pk = L.piv[k];
+      if (pk != k) {
+        const double t = x[k];
+        x[k] = x[pk];
+        x[pk] = t;
+      }
+    }
+  __syncthreads();
+  b = lane < n ? x[lane] : 0.0;
+  __syncthreads();
+  for (int k = 0; k < n - 1; ++k) {
+    if (lane == k) x[k] = b;
+    __syncthreads();
+    if (lane > k && lane < n) b -= L.M[lane * ld + k] * x[k];
+  }
+  __syncthreads();
+  for (int k = n - 1; k >= 0; --k) {
+    if (lane == k) {
+      b /= L.M[k * ld + k];
+      x[k] = b;
+    }
+    __syncthreads();
+    if (lane < k) b -= L.M[lane * ld + k] * x[k];
+  }
+  __syncthreads();
+  return b;
+}
+
 // Linear-algebra policies of the Newton iteration matrix M = I - c J.
 //  LinLds: J and M in LDS, any n <= 64 (lane = column in the elimination).
 //  LinReg: n <= 32; lane i keeps row i of J and of M in registers.  The
@@ -416,27 +565,44 @@ struct LinLds {
 // B[l / 16][l % 16]; accumulator element r of lane l is D[4r + l / 16][l % 16].
 // The factors have the layout of lu_factor's (unit L below the diagonal, U
 // on and above it, interchanges in L.piv), so lu_solve applies.
+// G: the workgroup (one wavefront, or two for up to 128 species, which
+// share the trailing tiles by checkerboard).  SparseJ: J holds the
+// structural entries only (two-wavefront LDS layout).
 typedef double p3d_f64x4 __attribute__((ext_vector_type(4)));
 
-struct LinMfma {
+template <class G, bool SparseJ>
+struct LinMfmaG {
   __device__ void load_jac(const NetDev &, const Lds &, int) {}
 
   __device__ void factor(const NetDev &N, const Lds &L, double c, int lane) {
     const int n = N.ns, ld = L.ld;
-    for (int idx = lane; idx < n * ld; idx += 64) {
-      const int i = idx / ld, j = idx - i * ld;
-      L.M[idx] = (i == j ? 1.0 : 0.0) - c * L.J[idx];
+    const G g{L.red};
+    if constexpr (SparseJ) {
+      for (int idx = lane; idx < n * ld; idx += G::W) {
+        const int i = idx / ld, j = idx - i * ld;
+        L.M[idx] = i == j ? 1.0 : 0.0;
+      }
+      __syncthreads();
+      for (int z = lane; z < N.nnz; z += G::W) {
+        const int i = N.jrow[z], j = N.jcol[z];
+        L.M[i * ld + j] = (i == j ? 1.0 : 0.0) - c * L.J[z];
+      }
+    } else {
+      for (int idx = lane; idx < n * ld; idx += G::W) {
+        const int i = idx / ld, j = idx - i * ld;
+        L.M[idx] = (i == j ? 1.0 : 0.0) - c * L.J[idx];
+      }
     }
     __syncthreads();
     const int nt = (n + 15) / 16;
-    const int lr = lane & 15, lq = lane >> 4;
+    const int lr = lane & 15, lq = (lane & 63) >> 4, wv = lane >> 6;
     for (int k0 = 0; k0 < n; k0 += 4) {
       const int k1 = min(k0 + 4, n);
       // panel columns k0..k1-1, unblocked; lane = row for the column work,
       // lane = column for the interchanges
       for (int k = k0; k < k1; ++k) {
         const double v = (lane >= k && lane < n) ? fabs(L.M[lane * ld + k]) : -1.0;
-        const int pk = wave_argmax(v, lane);
+        const int pk = g.argmax(v, lane);
         if (pk != k && lane < n) {
           const double a = L.M[k * ld + lane];
           L.M[k * ld + lane] = L.M[pk * ld + lane];
@@ -469,6 +635,7 @@ struct LinMfma {
         const int ia = ti * 16 + lr;
         const double a = (ia >= k1 && ia < n && q < k1) ? -L.M[ia * ld + q] : 0.0;
         for (int tj = k1 / 16; tj < nt; ++tj) {
+          if (G::W > 64 && ((ti + tj) & 1) != wv) continue;
           const int jb = tj * 16 + lr;
           const double b = (jb >= k1 && jb < n && q < k1) ? L.M[q * ld + jb] : 0.0;
           p3d_f64x4 acc;
@@ -490,9 +657,11 @@ struct LinMfma {
   }
 
   __device__ double solve(const NetDev &N, const Lds &L, double b, int lane) {
-    return lu_solve(N, L, b, lane);
+    if constexpr (G::W == 64) return lu_solve(N, L, b, lane);
+    else return lu_solve_wide(N, L, b, lane);
   }
 };
+using LinMfma = LinMfmaG<Wave1, false>;
 
 // LinLdsU: M in LDS for n <= NP (32), every loop fully unrolled so that the
 // LDS loads of a step issue together, ahead of the dependency chain.  Lane L
@@ -683,13 +852,19 @@ struct BdfStats {
 
 // Variable-order BDF from t0 to t1 for the wave's cell.  y: lane's species.
 // Returns 0 on success, -1 step underflow, -2 step limit.
-template <class Lin>
+template <class Lin, class G = Wave1>
 __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
                                              Lin &lin, const RxCache &rc, int lane, double &y,
                                              double t0, double t1, double rtol, double atol,
                                              double max_step, int max_steps, BdfStats &st) {
-  const bool cached = N.nr <= 64 * kRxPerLane;
-  auto rhs = [&]() { return cached ? rhs_cached(N, p, L, rc, lane) : rhs_lane(N, p, L, lane); };
+  const G g{L.red};
+  const bool cached = G::W == 64 && N.nr <= 64 * kRxPerLane;
+  auto rhs = [&]() {
+    if constexpr (G::W == 64)
+      return cached ? rhs_cached(N, p, L, rc, lane) : rhs_lane(N, p, L, lane);
+    else
+      return rhs_lane<G::W>(N, p, L, lane);
+  };
   const int ns = N.ns;
   const bool act = lane < ns;
   // coefficients indexed by the (wave-uniform) order: scalar loads
@@ -707,13 +882,13 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
   st.rhs++;
   // initial step (Hairer, Norsett & Wanner 1993, II.4), order 1
   double scale = atol + rtol * fabs(y);
-  double d0 = rms(act ? y / scale : 0.0, ns), d1 = rms(act ? f0 / scale : 0.0, ns);
+  double d0 = rms_g(g, act ? y / scale : 0.0, ns), d1 = rms_g(g, act ? f0 / scale : 0.0, ns);
   double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
   h0 = fmin(h0, span);
   publish_y(N, L, y + h0 * f0, lane);
   double f1 = rhs();
   st.rhs++;
-  double d2 = rms(act ? (f1 - f0) / scale : 0.0, ns) / h0;
+  double d2 = rms_g(g, act ? (f1 - f0) / scale : 0.0, ns) / h0;
   double dm = fmax(d1, d2);
   double h1 = (dm <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : sqrt(0.01 / dm);
   double h = fmin(fmin(100.0 * h0, h1), fmin(max_step, span));
@@ -726,7 +901,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
   int order = 1, n_equal = 0;
   double t = t0;
   publish_y(N, L, y, lane);
-  jacobian(N, p, L, lane);
+  jac_eval<G>(N, p, L, lane);
   lin.load_jac(N, L, lane);
   st.jac++;
   bool jac_current = true, lu_valid = false;
@@ -771,11 +946,11 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
           publish_y(N, L, ynew, lane);
           double f = rhs();
           st.rhs++;
-          if (!isfinite(wave_sum(act ? f : 0.0))) break;
+          if (!isfinite(g.sum(act ? f : 0.0))) break;
           double dy = act ? c * f - psi - d : 0.0;
           dy = lin.solve(N, L, dy, lane);
           if (!act) dy = 0.0;
-          const double dyn = rms(act ? dy / scale : 0.0, ns);
+          const double dyn = rms_g(g, act ? dy / scale : 0.0, ns);
           if (dy_old >= 0.0) rate = dyn / dy_old;
           if (rate >= 0.0 && (rate >= 1.0 || pow(rate, (double)(4 - it)) / (1.0 - rate) * dyn > newton_tol))
             break;
@@ -790,7 +965,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
         if (converged) break;
         if (!jac_current) {
           publish_y(N, L, ypred, lane);
-          jacobian(N, p, L, lane);
+          jac_eval<G>(N, p, L, lane);
           lin.load_jac(N, L, lane);
           st.jac++;
           jac_current = true;
@@ -810,7 +985,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
       }
       scale = atol + rtol * fabs(ynew);
       err = act ? ec(order) * d : 0.0;
-      const double en = rms(act ? err / scale : 0.0, ns);
+      const double en = rms_g(g, act ? err / scale : 0.0, ns);
       if (en > 1.0) {
         double factor = fmax(0.2, 0.9 * pow(en, -1.0 / (order + 1)));
         h *= factor;
@@ -841,29 +1016,29 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
     if (n_equal < order + 1) continue;
     // order selection
     scale = atol + rtol * fabs(D[0]);
-    const double e0 = rms(act ? err / scale : 0.0, ns);
+    const double e0 = rms_g(g, act ? err / scale : 0.0, ns);
     double em = INFINITY, ep = INFINITY;
     if (order > 1) {
       double dm1 = 0.0;
 #pragma unroll
       for (int j = 0; j < kNumD; ++j)
         if (j == order) dm1 = D[j];
-      em = rms(act ? ec(order - 1) * dm1 / scale : 0.0, ns);
+      em = rms_g(g, act ? ec(order - 1) * dm1 / scale : 0.0, ns);
     }
     if (order < kMaxOrder) {
       double dp2 = 0.0;
 #pragma unroll
       for (int j = 0; j < kNumD; ++j)
         if (j == order + 2) dp2 = D[j];
-      ep = rms(act ? ec(order + 1) * dp2 / scale : 0.0, ns);
+      ep = rms_g(g, act ? ec(order + 1) * dp2 / scale : 0.0, ns);
     }
     // the three candidate factors e^(-1/(q+1)) in one pow: lanes 0, 1, 2
     const double ev = lane == 0 ? em : (lane == 1 ? e0 : ep);
     const double ex = -1.0 / (double)(order + (lane == 0 ? 0 : (lane == 1 ? 1 : 2)));
     const double fv = ev > 0.0 ? pow(ev, ex) : INFINITY;
-    double fm = order > 1 ? bcast(fv, 0) : 0.0;
-    double fz = bcast(fv, 1);
-    double fp = order < kMaxOrder ? bcast(fv, 2) : 0.0;
+    double fm = order > 1 ? g.bc(fv, 0) : 0.0;
+    double fz = g.bc(fv, 1);
+    double fp = order < kMaxOrder ? g.bc(fv, 2) : 0.0;
     int dord = 0;
     double best = fz;
     if (fm > best) { best = fm; dord = -1; }
@@ -881,8 +1056,10 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
 
 // H+/e- steady-state correction (solver.py:227-281) and conservation
 // (solver.py:345-388); mode 0 = upstream clipped scales, 1 = exact projection.
+template <class G = Wave1>
 __device__ void post_solve(const NetDev &N, const CellParams &p, int lane, double &y, int mode,
-                           double x_c_total, double x_o_total) {
+                           double x_c_total, double x_o_total, double *red = nullptr) {
+  const G g{red};
   const int ns = N.ns;
   const bool act = lane < ns;
   const int iH = N.roles[0], iH2 = N.roles[1], iHp = N.roles[2], iCp = N.roles[3],
@@ -890,7 +1067,7 @@ __device__ void post_solve(const NetDev &N, const CellParams &p, int lane, doubl
   const double *wH = N.post_w, *wC = N.post_w + ns, *wO = N.post_w + 2 * ns,
                *wI = N.post_w + 3 * ns;
   {
-    const double xH = bcast(y, iH), xCp = bcast(y, iCp);
+    const double xH = g.bc(y, iH), xCp = g.bc(y, iCp);
     const double R = p.zeta * 0.46 * xH;
     const double Tc = fmax(p.T, 10.0);
     const double ae = 2.753e-14 * pow(Tc / 300.0, -0.745) + 1.0e-14 * sqrt(Tc / 100.0);
@@ -900,29 +1077,29 @@ __device__ void post_solve(const NetDev &N, const CellParams &p, int lane, doubl
       if (disc >= 0.0 && xCp < 1e-6) {
         const double xhp = fmax((-b + sqrt(disc)) / (2.0 * a), 1e-30);
         if (lane == iHp) y = xhp;
-        const double npos = wave_sum(act ? wI[lane] * y : 0.0);
+        const double npos = g.sum(act ? wI[lane] * y : 0.0);
         if (lane == ie) y = fmax(npos, 1e-30);
       }
     }
   }
   {
-    const double Hmaj = bcast(y, iH) + 2.0 * bcast(y, iH2);
-    const double Hmin = wave_sum(act ? wH[lane] * y : 0.0);
+    const double Hmaj = g.bc(y, iH) + 2.0 * g.bc(y, iH2);
+    const double Hmin = g.sum(act ? wH[lane] * y : 0.0);
     if (Hmaj > 0.0) {
       double s = mode == 0 ? fmin(fmax(fmax(1.0 - Hmin, 0.9) / Hmaj, 0.8), 1.2) : (1.0 - Hmin) / Hmaj;
       if (lane == iH || lane == iH2) y *= s;
     }
-    const double Cmaj = bcast(y, iCp) + bcast(y, iC) + bcast(y, iCO);
-    const double Cmin = wave_sum(act ? wC[lane] * y : 0.0);
+    const double Cmaj = g.bc(y, iCp) + g.bc(y, iC) + g.bc(y, iCO);
+    const double Cmin = g.sum(act ? wC[lane] * y : 0.0);
     if (Cmaj > 0.0) {
       double s = mode == 0
                      ? fmin(fmax(fmax(x_c_total - Cmin, x_c_total * 0.9) / Cmaj, 0.8), 1.2)
                      : (x_c_total - Cmin) / Cmaj;
       if (lane == iCp || lane == iC || lane == iCO) y *= s;
     }
-    const double Omol = wave_sum(act ? wO[lane] * y : 0.0);
+    const double Omol = g.sum(act ? wO[lane] * y : 0.0);
     if (lane == iO) y = fmax(x_o_total - Omol, 1e-30);
-    const double npos = wave_sum(act ? wI[lane] * y : 0.0);
+    const double npos = g.sum(act ? wI[lane] * y : 0.0);
     if (lane == ie) y = fmax(npos, 1e-30);
     y = fmax(y, 1e-30);
   }
@@ -938,8 +1115,8 @@ __device__ __forceinline__ CellParams load_params(int64_t c, const double *nH, c
   return p;
 }
 
-template <class Lin>
-__global__ void __launch_bounds__(64)
+template <class Lin, class G = Wave1>
+__global__ void __launch_bounds__(G::W)
 chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
                   const double *__restrict__ T, const double *__restrict__ G0,
                   const double *__restrict__ AV, const double *__restrict__ zeta,
@@ -953,15 +1130,18 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
   if (cell_list && blockIdx.x >= *list_len) return;
   const int64_t cell = cell_list ? cell_list[blockIdx.x] : (int64_t)blockIdx.x;
   const int lane = threadIdx.x;
-  const Lds L = carve(smem, N);
+  Lds L;
+  if constexpr (G::W == 64) L = carve(smem, N);
+  else L = carve_wide(smem, N);
+  const G g{L.red};
   const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
   const bool act = lane < N.ns;
   double yc = act ? y[(int64_t)lane * n + cell] : 0.0;
-  rate_coeffs(N, p, L, lane, o.rate_fp32 != 0);
-  stage_tables(N, L, lane);
+  rate_coeffs<G::W>(N, p, L, lane, o.rate_fp32 != 0);
+  if constexpr (G::W == 64) stage_tables(N, L, lane);
   __syncthreads();
   RxCache rc;
-  rc.load(N, L, lane);
+  if constexpr (G::W == 64) rc.load(N, L, lane);
 
   double tmax = o.t_max;
   if (tmax <= 0.0) tmax = fmin(3e17, fmax(1e16, 3e17 / fmax(p.nH, 1.0)));
@@ -974,8 +1154,8 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
     double ta = (i == 0) ? o.t_start : pow(10.0, l0 + (l1 - l0) * i / (nk - 1));
     double tb = (i + 1 == nk - 1) ? tmax : pow(10.0, l0 + (l1 - l0) * (i + 1) / (nk - 1));
     double yn = yc;
-    int status = bdf_integrate(N, p, L, lin, rc, lane, yn, ta, tb, o.rtol, o.atol, tb - ta,
-                               o.max_steps, st);
+    int status = bdf_integrate<Lin, G>(N, p, L, lin, rc, lane, yn, ta, tb, o.rtol, o.atol,
+                                       tb - ta, o.max_steps, st);
     if (status != 0) continue;  // a failed span leaves the state unchanged (solver.py:216-217)
     yn = fmax(yn, 1e-30);
     if (o.early_stop) {
@@ -983,7 +1163,7 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         const int k = N.roles[q == 0 ? 0 : q == 1 ? 1 : q == 2 ? 3 : q == 3 ? 4 : q == 4 ? 5 : 6];
-        const double old = bcast(yc, k), nw = bcast(yn, k);
+        const double old = g.bc(yc, k), nw = g.bc(yn, k);
         if (old > 1e-12) rel = fmax(rel, fabs(nw - old) / fmax(old, 1e-20));
       }
       if (rel >= 0.0 && rel < o.stop_rtol && i > 3) {
@@ -994,7 +1174,8 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
     }
     yc = yn;
   }
-  if (o.post_mode >= 0) post_solve(N, p, lane, yc, o.post_mode, o.x_c_total, o.x_o_total);
+  if (o.post_mode >= 0)
+    post_solve<G>(N, p, lane, yc, o.post_mode, o.x_c_total, o.x_o_total, L.red);
   if (act) y[(int64_t)lane * n + cell] = yc;
   if (lane == 0) {
     if (conv_out) conv_out[cell] = conv;
@@ -1370,8 +1551,8 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
                        const int32_t *products, int32_t e_index, const int32_t *roles7,
                        const double *post_w4, int64_t *handle) {
   P3D_REQUIRE(handle, "handle pointer is required");
-  P3D_REQUIRE(ns >= 1 && ns <= kMaxSpecies, "n_species must be in [1, %d] (got %d)",
-              kMaxSpecies, ns);
+  P3D_REQUIRE(ns >= 1 && ns <= kMaxSpeciesWide, "n_species must be in [1, %d] (got %d)",
+              kMaxSpeciesWide, ns);
   P3D_REQUIRE(nr >= 1 && nr <= 16384, "n_reactions must be in [1, 16384] (got %d)", nr);
   P3D_REQUIRE(rtype && alpha && beta && gamma && reactants && products, "null table");
   P3D_REQUIRE(e_index >= 0 && e_index < ns, "e_index out of range");
@@ -1524,6 +1705,13 @@ int p3d_network_destroy(int64_t handle) {
   return P3D_OK;
 }
 
+// Entry points whose kernels map one species to one lane of a wavefront.
+static int check_narrow(const Network *net, const char *what) {
+  P3D_REQUIRE(net->d.ns <= kMaxSpecies, "%s supports networks of at most %d species (got %d)",
+              what, kMaxSpecies, net->d.ns);
+  return P3D_OK;
+}
+
 static int check_cells(const Network *net, int64_t n, const void *y, const double *nH,
                        const double *T, const double *G0, const double *AV, const double *zeta,
                        const double *fH2, const double *fCO) {
@@ -1540,6 +1728,7 @@ int p3d_chem_rhs(int64_t handle, int64_t n, const double *y, const double *n_h,
   Network *net = find_net(handle);
   int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
   if (rc != P3D_OK || n == 0) return rc;
+  if ((rc = check_narrow(net, "p3d_chem_rhs")) != P3D_OK) return rc;
   P3D_REQUIRE(dydt, "null output");
   const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
@@ -1555,6 +1744,7 @@ int p3d_chem_jacobian(int64_t handle, int64_t n, const double *y, const double *
   Network *net = find_net(handle);
   int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
   if (rc != P3D_OK || n == 0) return rc;
+  if ((rc = check_narrow(net, "p3d_chem_jacobian")) != P3D_OK) return rc;
   P3D_REQUIRE(jac, "null output");
   const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
@@ -1581,6 +1771,7 @@ int p3d_chem_polish(int64_t handle, int64_t n, double *y, const double *n_h, con
   Network *net = find_net(handle);
   int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
   if (rc != P3D_OK || n == 0) return rc;
+  if ((rc = check_narrow(net, "p3d_chem_polish")) != P3D_OK) return rc;
   SteadyCons K;
   if ((rc = steady_cons_from(cons, net->d, K)) != P3D_OK) return rc;
   P3D_REQUIRE(max_iter >= 1 && max_iter <= 1000, "max_iter must be in [1, 1000]");
@@ -1600,6 +1791,7 @@ int p3d_chem_adjoint(int64_t handle, int64_t n, const double *y, const double *n
   Network *net = find_net(handle);
   int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
   if (rc != P3D_OK || n == 0) return rc;
+  if ((rc = check_narrow(net, "p3d_chem_adjoint")) != P3D_OK) return rc;
   SteadyCons K;
   if ((rc = steady_cons_from(cons, net->d, K)) != P3D_OK) return rc;
   P3D_REQUIRE(cii5, "[CII] line parameters (host, 5 doubles) are required");
@@ -1624,6 +1816,7 @@ int p3d_chem_post(int64_t handle, int64_t n, double *y, const double *n_h,
   P3D_REQUIRE(net, "unknown network handle");
   P3D_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return P3D_OK;
+  if (int rc = check_narrow(net, "p3d_chem_post")) return rc;
   P3D_REQUIRE(y && n_h && t_gas && zeta, "null device array");
   P3D_REQUIRE(post_mode == 0 || post_mode == 1, "post_mode must be 0 or 1");
   for (int k = 0; k < 7; ++k)
@@ -1662,6 +1855,18 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
   P3D_REQUIRE(o.rescue_frac >= 0.0 && o.rescue_frac <= 1.0, "rescue_frac must be in [0, 1]");
   if (o.variant == 4) return lane_solve(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o,
                                         converged, stats, stream);
+  if (net->d.ns > kMaxSpecies) {
+    // two wavefronts per cell, lane = species, MFMA panel LU
+    P3D_REQUIRE(o.variant == 0 || o.variant == 5,
+                "networks of more than %d species use the two-wavefront MFMA form "
+                "(variant 0 or 5; got %d)", kMaxSpecies, o.variant);
+    const size_t shw = lds_bytes_wide(net->d.ns, net->d.nr, net->d.nnz);
+    P3D_REQUIRE(shw <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", shw);
+    hipLaunchKernelGGL((chem_solve_kernel<LinMfmaG<Wave2, true>, Wave2>), dim3((unsigned)n),
+                       dim3(Wave2::W), shw, as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v,
+                       zeta, fsh_h2, fsh_co, o, converged, stats, nullptr, nullptr);
+    return launch_status("chem_solve_kernel");
+  }
   const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
   int var = o.variant;

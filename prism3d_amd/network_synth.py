@@ -23,6 +23,14 @@ EXTRA = ['N', 'N+', 'NH', 'NH+', 'NH2', 'NH2+', 'NH3', 'NH3+', 'NH4+', 'N2', 'N2
          'CN', 'CN+', 'HCN', 'HCN+', 'HCNH+', 'NO', 'NO+', 'HNO+',
          'CH2', 'CH3', 'CH3+', 'CH4', 'CH4+', 'CH5+', 'C2', 'C2+', 'C2H', 'C2H+', 'C2H2',
          'C2H2+', 'O2+']
+# the further 39 species of the 103-species network: nitrogen oxides and
+# isomers, sulphur, silicon and more carbon-oxygen species
+EXTRA_WIDE = ['HNC', 'H2CN', 'HNO', 'NO2', 'N2O', 'OCN',
+              'SH', 'SH+', 'H2S', 'H2S+', 'H3S+', 'CS', 'CS+', 'HCS+', 'SO', 'SO+', 'SO2', 'OCS',
+              'OCS+', 'NS', 'NS+',
+              'SiH', 'SiH+', 'SiO', 'SiO+', 'SiOH+', 'SiH2+',
+              'HCO', 'HOC+', 'H2CO', 'H2CO+', 'H3CO+', 'CO2', 'HCO2+', 'O2H+', 'C3', 'C3H', 'C3H+',
+              'C2H3+']
 
 _TOKEN = re.compile(r'(He|Si|Fe|H|C|N|O|S)(\d*)')
 
@@ -63,11 +71,14 @@ def balanced(reactants, products):
     return _key(r) == _key(products) if len(r) == len(reactants) else True
 
 
-def synthetic_network(seed=2024, n_extra=None, max_channels=3):
+def synthetic_network(seed=2024, n_extra=None, max_channels=3, max_exchange=2, extra=None):
     """(species, reactions) of the 64-species synthetic network (31 bundled
-    + 33 extra, or the first n_extra of the extra list).  Reactions are the
-    bundled 75 first, then the generated ones in a fixed order."""
-    extra = EXTRA if n_extra is None else EXTRA[:n_extra]
+    + 33 extra, or the first n_extra of the extra list, or the given extra
+    species).  Reactions are the bundled 75 first, then the generated ones in
+    a fixed order: up to max_channels product channels per recombination and
+    max_exchange per two-body reactant pair."""
+    if extra is None:
+        extra = EXTRA if n_extra is None else EXTRA[:n_extra]
     species = list(TRACKED) + list(extra)
     rng = np.random.RandomState(seed)
     comp = {s: formula(s) for s in species}
@@ -145,7 +156,7 @@ def synthetic_network(seed=2024, n_extra=None, max_channels=3):
             if not cands:
                 continue
             rng.shuffle(cands)
-            for p in cands[:2]:
+            for p in cands[:max_exchange]:
                 if qa + qb == 0:
                     typ, al, be, ga = 'two_body', 10 ** rng.uniform(-11.5, -10), \
                         rng.uniform(-0.3, 0.5), rng.choice([0.0, rng.uniform(100, 3000)])
@@ -160,6 +171,13 @@ def synthetic_network(seed=2024, n_extra=None, max_channels=3):
                 add(f'{a}_{b}_to_{"_".join(p)}', typ, [a, b], list(p), al, be, ga)
     reactions = list(REACTIONS) + out
     return species, reactions
+
+
+def synthetic_network_wide(seed=2024):
+    """(species, reactions) of the 103-species synthetic network (the 64 of
+    synthetic_network plus EXTRA_WIDE), one product channel per two-body
+    reactant pair: 1349 reactions, 3830 structural Jacobian nonzeros."""
+    return synthetic_network(seed=seed, max_exchange=1, extra=EXTRA + EXTRA_WIDE)
 
 
 def element_totals_check(species, reactions):

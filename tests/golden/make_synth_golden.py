@@ -1,10 +1,11 @@
 """Write synth_golden.npz: steady states of the 64-species synthetic network
 (prism3d_amd.network_synth) on six seeded cells, from scipy's BDF on the
 oracle's restatement of the upstream RHS (oracle/generic_network.py) at
-rtol 1e-10, atol 1e-24, with the analytic Jacobian.  No upstream code runs;
-this checks the GPU solve on a network the reference does not ship.
+rtol 1e-10, atol 1e-24, with the analytic Jacobian.  With --wide, the same
+for the 103-species network into synth_wide_golden.npz.  No upstream code
+runs; this checks the GPU solve on networks the reference does not ship.
 
-    python tests/golden/make_synth_golden.py
+    python tests/golden/make_synth_golden.py [--wide]
 """
 import os
 import sys
@@ -16,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from oracle.generic_network import GenericNetwork, integrate  # noqa: E402
-from prism3d_amd.network_synth import synthetic_network  # noqa: E402
+from prism3d_amd.network_synth import synthetic_network, synthetic_network_wide  # noqa: E402
 
 INIT = (('H', 1.0), ('C+', 1.4e-4), ('O', 3e-4), ('N', 7.5e-5), ('He', 0.1), ('S+', 1e-6),
         ('Si+', 1e-7), ('Fe+', 1e-8))
@@ -30,8 +31,8 @@ def initial(species):
     return y0
 
 
-def main():
-    sp, rx = synthetic_network()
+def main(wide=False):
+    sp, rx = synthetic_network_wide() if wide else synthetic_network()
     net = GenericNetwork(sp, rx)
     y0 = initial(sp)
     t_end = 1e7 * 3.15576e7
@@ -44,9 +45,10 @@ def main():
     for c, p in enumerate(par):
         out.append(integrate(net, y0, *p, t_end, rtol=1e-10, atol=1e-24))
         print(f"cell {c} done", flush=True)
-    np.savez_compressed(os.path.join(HERE, "synth_golden.npz"), params=par, y0=y0,
+    name = "synth_wide_golden.npz" if wide else "synth_golden.npz"
+    np.savez_compressed(os.path.join(HERE, name), params=par, y0=y0,
                         t_end=t_end, y_end=np.array(out), rtol=1e-10, atol=1e-24)
 
 
 if __name__ == "__main__":
-    main()
+    main(wide="--wide" in sys.argv[1:])

@@ -67,11 +67,11 @@ def test_network_validation_without_gpu():
                               _lib.i32ptr(bad), _lib.i32ptr(n.products), n.e_index,
                               _lib.i32ptr(n.roles), _lib.dptr(n.post_w), ctypes.byref(h))
     assert rc == -1 and b"reaction 0" in L.p3d_last_error()
-    rc = L.p3d_network_create(65, n.n_reactions, _lib.i32ptr(n.rtype), _lib.dptr(n.alpha),
+    rc = L.p3d_network_create(129, n.n_reactions, _lib.i32ptr(n.rtype), _lib.dptr(n.alpha),
                               _lib.dptr(n.beta), _lib.dptr(n.gamma), _lib.i32ptr(n.reactants),
                               _lib.i32ptr(n.products), n.e_index, _lib.i32ptr(n.roles),
                               _lib.dptr(n.post_w), ctypes.byref(h))
-    assert rc == -1
+    assert rc == -1 and b"n_species must be in [1, 128]" in L.p3d_last_error()
 
 
 def test_network_tables_match_oracle_stoichiometry():

@@ -103,12 +103,104 @@ def test_mfma_and_lds_lu_agree_on_random_cells(gpu):
     assert rel.max() < 1e-4
 
 
-def test_variant_5_rejects_oversized_networks(gpu):
-    """The MFMA LU variant, like every workgroup form, takes at most 64
-    species; a larger network is refused at creation."""
+def test_oversized_networks_are_refused(gpu):
+    """Networks of up to 128 species are taken at creation, larger ones
+    refused.  Above 64 species only the solve runs (two wavefronts per
+    cell, MFMA LU); the entry points that map a species to a lane of one
+    wavefront, and the one-wavefront solve variants, refuse such a
+    network."""
+    import torch
+    from prism3d_amd.chemistry import ChemistrySolver, solve_opts
     from prism3d_amd.network import DeviceNetwork, Network
-    from prism3d_amd.network_synth import synthetic_network
-    sp, rx = synthetic_network()
-    sp2 = sp + ['X1']
+    from prism3d_amd.network_synth import synthetic_network_wide
+    print("case: species-count limits", flush=True)
+    sp, rx = synthetic_network_wide()
     with pytest.raises(ValueError):
-        DeviceNetwork(Network(sp2, rx))
+        DeviceNetwork(Network(sp + [f'X{i}' for i in range(129 - len(sp))], rx))
+    cs = ChemistrySolver(Network(sp, rx))
+    n = 4
+    y = torch.full((len(sp), n), 1e-10, dtype=torch.float64, device=gpu)
+    par = [np.full(n, v) for v in (1e3, 50.0, 10.0, 1.0, 2e-16, 1.0, 1.0)]
+    with pytest.raises(ValueError):
+        cs.rhs_batch(y, *par)
+    with pytest.raises(ValueError):
+        cs.solve_steady_batch(y, *par, opts=solve_opts(variant=1, post_mode=-1))
+
+
+def _wide_solver():
+    from prism3d_amd.chemistry import ChemistrySolver
+    from prism3d_amd.network import Network
+    from prism3d_amd.network_synth import synthetic_network_wide
+    sp, rx = synthetic_network_wide()
+    return sp, rx, ChemistrySolver(Network(sp, rx))
+
+
+def test_wide_network_against_scipy(gpu):
+    """Six cells of the 103-species network (two wavefronts per cell, sparse
+    J and the MFMA panel LU in LDS) integrated to 1e7 yr against scipy's BDF
+    (rtol 1e-10) on the restated RHS, species above 1e-14; the auto variant
+    picks the same form."""
+    import os
+    import torch
+    from conftest import GOLDEN
+    from prism3d_amd.chemistry import solve_opts
+    if not os.path.exists(os.path.join(GOLDEN, "synth_wide_golden.npz")):
+        pytest.skip("synth_wide_golden.npz not generated (tests/golden/make_synth_golden.py --wide)")
+    g = golden("synth_wide_golden.npz")
+    sp, rx, cs = _wide_solver()
+    par = g['params']
+    n = par.shape[0]
+    ref = g['y_end'].T
+    m = ref > 1e-14
+    got = {}
+    for var in (5, 0):
+        print(f"case: 103-species network, variant {var}", flush=True)
+        y = torch.tensor(np.repeat(g['y0'][:, None], n, axis=1), device=gpu)
+        opts = solve_opts(t_max=float(g['t_end']), n_checkpoints=2, early_stop=False, rtol=1e-9,
+                          atol=1e-24, post_mode=-1, variant=var)
+        _, st = cs.solve_steady_batch(y, *[par[:, k] for k in range(7)], opts=opts)
+        got[var] = y.cpu().numpy()
+        rel = np.abs(got[var][m] - ref[m]) / ref[m]
+        print(f"variant {var} vs scipy: max {rel.max():.2e}, median {np.median(rel):.2e}, "
+              f"steps {st.cpu().numpy()[:, 0].mean():.0f}", flush=True)
+        assert rel.max() < 1e-5
+        assert np.median(rel) < 1e-8
+    assert np.array_equal(got[5], got[0])
+
+
+def test_wide_network_conserves_nuclei(gpu):
+    """1024 cells over PDR conditions on the 103-species network: every
+    element's nuclei are conserved by the BDF/Newton solve (each Newton
+    correction lies in the null space of the conservation laws, up to the
+    rounding of the LU solves), here to 1e-7 of the element total (scipy's
+    BDF drifts by up to 4e-9, in nitrogen, on the 64-species fixture cells);
+    every state finite and above the floor."""
+    import torch
+    from prism3d_amd.chemistry import solve_opts
+    from prism3d_amd.network_synth import formula
+    print("case: 103-species network, nuclei on 1024 cells", flush=True)
+    sp, rx, cs = _wide_solver()
+    y0 = np.full(len(sp), 1e-30)
+    for s_, v in (('H', 1.0), ('C+', 1.4e-4), ('O', 3e-4), ('N', 7.5e-5), ('He', 0.1),
+                  ('S+', 1e-6), ('Si+', 1e-7), ('Fe+', 1e-8)):
+        y0[sp.index(s_)] = v
+    y0[sp.index('e-')] = 1.4e-4 + 1e-6 + 1e-7 + 1e-8
+    rng = np.random.RandomState(11)
+    n = 1024
+    par = [10 ** rng.uniform(2, 5, n), 10 ** rng.uniform(1.2, 2.5, n),
+           10 ** rng.uniform(0, 3.5, n), rng.uniform(0, 6, n), 10 ** rng.uniform(-16.5, -15.5, n),
+           10 ** rng.uniform(-4, 0, n), 10 ** rng.uniform(-4, 0, n)]
+    y = torch.tensor(np.repeat(y0[:, None], n, axis=1), device=gpu)
+    opts = solve_opts(t_max=1e7 * 3.15576e7, n_checkpoints=2, early_stop=False, rtol=1e-9,
+                      atol=1e-24, post_mode=-1, variant=0)
+    _, st = cs.solve_steady_batch(y, *par, opts=opts)
+    got = y.cpu().numpy()
+    assert np.isfinite(got).all() and (got >= 1e-30).all()
+    elems = sorted({e for s in sp for e in formula(s)[0]})
+    C = np.array([[formula(s)[0].get(e, 0) for s in sp] for e in elems], dtype=np.float64)
+    tot0 = C @ y0
+    tot = C @ got
+    rel = np.abs(tot - tot0[:, None]) / tot0[:, None]
+    print(f"nuclei drift by element {dict(zip(elems, rel.max(axis=1)))}; "
+          f"steps {st.cpu().numpy()[:, 0].mean():.0f}", flush=True)
+    assert rel.max() < 1e-7

@@ -27,6 +27,27 @@ def test_synthetic_network_is_balanced_and_reproducible():
     assert (net.roles >= 0).all()
 
 
+def test_wide_synthetic_network_is_balanced_and_reproducible():
+    """The 103-species network: reproducible, every reaction balancing
+    nuclei and charge, every added species formed and destroyed, and its
+    reaction list conserving each element (C S = 0)."""
+    from prism3d_amd.network import Network
+    from prism3d_amd.network_synth import (EXTRA, EXTRA_WIDE, element_totals_check, formula,
+                                           synthetic_network_wide)
+    sp, rx = synthetic_network_wide()
+    assert (sp, rx) == synthetic_network_wide()
+    assert len(sp) == 103 and len(set(sp)) == 103 and len(rx) == 1349
+    assert sp[64:] == EXTRA_WIDE and sp[31:64] == EXTRA
+    assert element_totals_check(sp, rx) == []
+    for s in EXTRA_WIDE:
+        assert any(s in r[3] for r in rx), s
+        assert any(s in r[2] for r in rx), s
+    net = Network(sp, rx)
+    elems = sorted({e for x in sp for e in formula(x)[0]})
+    C = np.array([[formula(x)[0].get(e, 0) for x in sp] for e in elems], dtype=np.float64)
+    assert np.abs(C @ net.stoichiometry()).max() == 0.0
+
+
 def test_generic_oracle_equals_the_bundled_oracle():
     """The reaction-list restatement against the 31-species C oracle (pinned
     by the upstream golden vectors) on random states: rates to one rounding,
