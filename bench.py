@@ -93,7 +93,9 @@ def synth_line(args):
         stn = st.cpu().numpy()
         fl = float(flops_per_cell(stn, len(sp), len(rx), n_terms).sum())
         res[var] = dict(s=sec, y=y.cpu().numpy(), flops=fl, lu=float(stn[:, 1].sum()),
-                        steps_mean=float(stn[:, 0].mean()))
+                        steps_mean=float(stn[:, 0].mean()), steps_max=int(stn[:, 0].max()),
+                        steps_p99=float(np.percentile(stn[:, 0], 99)),
+                        lu_mean=float(stn[:, 1].mean()))
         print(f"[bench] synthetic network, variant {var}: {sec:.2f} s", file=sys.stderr,
               flush=True)
     r5 = res[5]
@@ -108,7 +110,9 @@ def synth_line(args):
            "config": {"workload": f"{n} cells, {len(sp)} species / {len(rx)} reactions "
                                   "(prism3d_amd.network_synth), BDF to 1e7 yr at rtol 1e-9",
                       "kernel": kern},
-           "s_mfma_lu": r5["s"], "bdf_steps_per_cell": r5["steps_mean"]}
+           "s_mfma_lu": r5["s"], "bdf_steps_per_cell": r5["steps_mean"],
+           "bdf_steps_p99": r5["steps_p99"], "bdf_steps_max": r5["steps_max"],
+           "lu_per_cell": r5["lu_mean"]}
     if not wide:
         a, b = res[5]["y"], res[1]["y"]
         m = b > 1e-14

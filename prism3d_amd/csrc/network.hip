@@ -33,6 +33,7 @@ namespace p3d {
 
 constexpr int kMaxSpecies = 64;       // one-wavefront forms: lane = species
 constexpr int kMaxSpeciesWide = 128;  // two-wavefront form of the solve
+constexpr int kMaxPieces = kMaxSpeciesWide + 128;  // RHS pieces: rows + lane cuts
 constexpr int kMaxOrder = 5;
 constexpr int kNumD = kMaxOrder + 3;
 
@@ -51,6 +52,13 @@ struct NetDev {
   const int *jt_rxn, *jt_slot;
   const double *jt_coef;
   const double *post_w;  // 4 * ns
+  // balanced RHS gather of the two-wavefront form: the packed entries
+  // (rhs_pk, species by species) cut into pieces at every species boundary
+  // and at 128 equal shares, one share per lane
+  int n_pc;
+  const int *pc_beg;   // n_pc + 1 entry offsets
+  const int *pc_lane;  // 129: pieces of lane l are pc_lane[l] .. pc_lane[l+1]-1
+  const int *pc_row;   // ns + 1: pieces of species s
 };
 
 struct Network {
@@ -94,6 +102,7 @@ struct Lds {
   int *csr;      // n_rhs_e packed RHS entries (LDS copy of rhs_pk)
   double *red;   // two-wavefront form: 4 doubles of cross-wave reduction slots
   double *xs;    // two-wavefront form: 128-entry vector of the triangular solves
+  double *psum;  // two-wavefront form: partial sums of the RHS pieces
   int ld;
 };
 
@@ -111,6 +120,7 @@ __device__ __forceinline__ Lds carve(double *base, const NetDev &N) {
   L.csr = L.csr_off + kMaxSpecies + 1;
   L.red = nullptr;
   L.xs = nullptr;
+  L.psum = nullptr;
   return L;
 }
 
@@ -127,7 +137,8 @@ __device__ __forceinline__ Lds carve_wide(double *base, const NetDev &N) {
   L.M = L.J + N.nnz;
   L.red = L.M + N.ns * L.ld;
   L.xs = L.red + 4;
-  L.ru = L.xs + kMaxSpeciesWide;
+  L.psum = L.xs + kMaxSpeciesWide;
+  L.ru = L.psum + kMaxPieces;
   L.piv = reinterpret_cast<int *>(L.ru + 36);
   L.csr_off = nullptr;
   L.csr = nullptr;
@@ -142,7 +153,8 @@ static size_t lds_bytes(int ns, int nr, int n_rhs_e) {
 
 static size_t lds_bytes_wide(int ns, int nr, int nnz) {
   const int ld = ns | 1;
-  return sizeof(double) * (size_t)(3 * nr + 2 * kMaxSpeciesWide + nnz + ns * ld + 4 + 36) +
+  return sizeof(double) * (size_t)(3 * nr + 2 * kMaxSpeciesWide + kMaxPieces + nnz + ns * ld +
+                                   4 + 36) +
          sizeof(int) * (size_t)kMaxSpeciesWide;
 }
 
@@ -429,6 +441,44 @@ __device__ void jacobian_sparse(const NetDev &N, const CellParams &p, const Lds 
   __syncthreads();
 }
 
+// dy/dt for the two-wavefront form: lane l sums the entries of its share
+// piece by piece (each piece in the upstream order) into psum, then species
+// s adds its pieces in order.  A species whose row lies in one piece gets
+// the bits of rhs_lane; a row split over lanes differs by the rounding of
+// the split.  Requires y published.
+template <int W>
+__device__ double rhs_balanced(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
+  eval_reactions<W>(N, p, L, lane, false);
+  const int q0 = N.pc_lane[lane], q1 = N.pc_lane[lane + 1];
+  for (int q = q0; q < q1; ++q) {
+    const int e1 = N.pc_beg[q + 1];
+    int e = N.pc_beg[q];
+    double acc = 0.0;
+    for (; e + 4 <= e1; e += 4) {
+      const int c0 = N.rhs_pk[e], c1 = N.rhs_pk[e + 1], c2 = N.rhs_pk[e + 2],
+                c3 = N.rhs_pk[e + 3];
+      const double v0 = L.part[2 * (c0 >> 1)], v1 = L.part[2 * (c1 >> 1)],
+                   v2 = L.part[2 * (c2 >> 1)], v3 = L.part[2 * (c3 >> 1)];
+      acc = (c0 & 1) ? acc + v0 : acc - v0;
+      acc = (c1 & 1) ? acc + v1 : acc - v1;
+      acc = (c2 & 1) ? acc + v2 : acc - v2;
+      acc = (c3 & 1) ? acc + v3 : acc - v3;
+    }
+    for (; e < e1; ++e) {
+      const int c0 = N.rhs_pk[e];
+      const double v0 = L.part[2 * (c0 >> 1)];
+      acc = (c0 & 1) ? acc + v0 : acc - v0;
+    }
+    L.psum[q] = acc;
+  }
+  __syncthreads();
+  double acc = 0.0;
+  if (lane < N.ns)
+    for (int q = N.pc_row[lane]; q < N.pc_row[lane + 1]; ++q) acc += L.psum[q];
+  __syncthreads();  // part[] and psum[] are reused by the next evaluation
+  return acc;
+}
+
 template <class G>
 __device__ __forceinline__ void jac_eval(const NetDev &N, const CellParams &p, const Lds &L,
                                          int lane) {
@@ -499,42 +549,53 @@ __device__ double lu_solve(const NetDev &N, const Lds &L, double b, int lane) {
   return b;
 }
 
-// lu_solve for the two-wavefront form: the row interchanges are applied by
-// one lane to an LDS copy of b, then each substitution step publishes one
-// solved component through its own LDS slot (one barrier per step).  Same
-// operations in the same order as lu_solve.
+// lu_solve for the two-wavefront form, run by wavefront 0 alone: lane l
+// holds rows l and l + 64 and the solved components travel by readlane, as
+// in lu_solve (same operations in the same order); b enters and leaves
+// through an LDS vector, two barriers per solve.
 __device__ double lu_solve_wide(const NetDev &N, const Lds &L, double b, int lane) {
   const int n = N.ns, ld = L.ld;
   double *x = L.xs;
   if (lane < n) x[lane] = b;
   __syncthreads();
-  if (lane == 0)
+  if (lane < 64) {
+    const int r0 = lane, r1 = lane + 64;
+    double b0 = r0 < n ? x[r0] : 0.0, b1 = r1 < n ? x[r1] : 0.0;
     for (int k = 0; k < n; ++k) {
-      const int pk = L.piv[k];
+      const int pk = uniform(L.piv[k]);
       if (pk != k) {
-        const double t = x[k];
-        x[k] = x[pk];
-        x[pk] = t;
+        const double bk = bcast(k < 64 ? b0 : b1, k & 63);
+        const double bp = bcast(pk < 64 ? b0 : b1, pk & 63);
+        if (lane == (k & 63)) {
+          if (k < 64) b0 = bp;
+          else b1 = bp;
+        }
+        if (lane == (pk & 63)) {
+          if (pk < 64) b0 = bk;
+          else b1 = bk;
+        }
       }
     }
+    for (int k = 0; k < n - 1; ++k) {
+      const double bk = bcast(k < 64 ? b0 : b1, k & 63);
+      if (r0 > k && r0 < n) b0 -= L.M[r0 * ld + k] * bk;
+      if (r1 > k && r1 < n) b1 -= L.M[r1 * ld + k] * bk;
+    }
+    for (int k = n - 1; k >= 0; --k) {
+      if (lane == (k & 63)) {
+        if (k < 64) b0 /= L.M[k * ld + k];
+        else b1 /= L.M[k * ld + k];
+      }
+      const double bk = bcast(k < 64 ? b0 : b1, k & 63);
+      if (r0 < k) b0 -= L.M[r0 * ld + k] * bk;
+      if (r1 < k) b1 -= L.M[r1 * ld + k] * bk;
+    }
+    x[r0] = b0;
+    x[r1] = b1;
+  }
   __syncthreads();
   b = lane < n ? x[lane] : 0.0;
-  __syncthreads();
-  for (int k = 0; k < n - 1; ++k) {
-    if (lane == k) x[k] = b;
-    __syncthreads();
-    if (lane > k && lane < n) b -= L.M[lane * ld + k] * x[k];
-  }
-  __syncthreads();
-  for (int k = n - 1; k >= 0; --k) {
-    if (lane == k) {
-      b /= L.M[k * ld + k];
-      x[k] = b;
-    }
-    __syncthreads();
-    if (lane < k) b -= L.M[lane * ld + k] * x[k];
-  }
-  __syncthreads();
+  __syncthreads();  // x is reused by the next solve
   return b;
 }
 
@@ -863,7 +924,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
     if constexpr (G::W == 64)
       return cached ? rhs_cached(N, p, L, rc, lane) : rhs_lane(N, p, L, lane);
     else
-      return rhs_lane<G::W>(N, p, L, lane);
+      return rhs_balanced<G::W>(N, p, L, lane);
   };
   const int ns = N.ns;
   const bool act = lane < ns;
@@ -1634,6 +1695,26 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
     joff.push_back((int)jt_rxn.size());
   }
   const int nnz = (int)jrow.size();
+  // pieces of the balanced RHS gather: cuts at every species row start and
+  // at 128 equal shares of the entries
+  const int n_e = (int)rhs_pk.size();
+  std::vector<int> lane_cut(kMaxSpeciesWide + 1);
+  for (int l = 0; l <= kMaxSpeciesWide; ++l)
+    lane_cut[l] = (int)((int64_t)n_e * l / kMaxSpeciesWide);
+  std::vector<int> pc_beg(rhs_off.begin(), rhs_off.end() - 1);
+  pc_beg.insert(pc_beg.end(), lane_cut.begin(), lane_cut.end() - 1);
+  std::sort(pc_beg.begin(), pc_beg.end());
+  pc_beg.erase(std::unique(pc_beg.begin(), pc_beg.end()), pc_beg.end());
+  while (!pc_beg.empty() && pc_beg.back() >= n_e) pc_beg.pop_back();
+  const int n_pc = (int)pc_beg.size();
+  pc_beg.push_back(n_e);
+  auto first_piece = [&](int e) {
+    return (int)(std::lower_bound(pc_beg.begin(), pc_beg.end() - 1, e) - pc_beg.begin());
+  };
+  std::vector<int> pc_lane(kMaxSpeciesWide + 1), pc_row(ns + 1);
+  for (int l = 0; l <= kMaxSpeciesWide; ++l) pc_lane[l] = first_piece(lane_cut[l]);
+  for (int r = 0; r <= ns; ++r) pc_row[r] = first_piece(rhs_off[r]);
+  P3D_REQUIRE(n_pc <= kMaxPieces, "internal: %d RHS pieces", n_pc);
   std::vector<double> post(4 * (size_t)ns, 0.0);
   if (post_w4) post.assign(post_w4, post_w4 + 4 * (size_t)ns);
   int roles[7] = {-1, -1, -1, -1, -1, -1, -1};
@@ -1660,7 +1741,9 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
          o_jr = put(jrow.data(), 4 * nnz),
          o_jc = put(jcol.data(), 4 * nnz), o_jo = put(joff.data(), 4 * (nnz + 1)),
          o_tr = put(jt_rxn.data(), 4 * jt_rxn.size()), o_ts = put(jt_slot.data(), 4 * jt_slot.size()),
-         o_tc = put(jt_coef.data(), 8 * jt_coef.size()), o_pw = put(post.data(), 8 * post.size());
+         o_tc = put(jt_coef.data(), 8 * jt_coef.size()), o_pw = put(post.data(), 8 * post.size()),
+         o_pb = put(pc_beg.data(), 4 * pc_beg.size()), o_pl = put(pc_lane.data(), 4 * pc_lane.size()),
+         o_pr = put(pc_row.data(), 4 * pc_row.size());
   auto net = std::make_unique<Network>();
   if (hipMalloc(&net->blob, host.size()) != hipSuccess) {
     set_error("hipMalloc of %zu bytes for network tables failed", host.size());
@@ -1688,6 +1771,10 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   d.jt_slot = (const int *)(b + o_ts);
   d.jt_coef = (const double *)(b + o_tc);
   d.post_w = (const double *)(b + o_pw);
+  d.n_pc = n_pc;
+  d.pc_beg = (const int *)(b + o_pb);
+  d.pc_lane = (const int *)(b + o_pl);
+  d.pc_row = (const int *)(b + o_pr);
   net->lane_ok = roles7 != nullptr &&
                  lane_network_matches(ns, nr, rtype, reactants, products, e_index, roles);
   std::lock_guard<std::mutex> lk(g_net_mu);
