@@ -136,16 +136,13 @@ def _wide_solver():
 
 
 def test_wide_network_against_scipy(gpu):
-    """Six cells of the 103-species network (two wavefronts per cell, sparse
-    J and the MFMA panel LU in LDS) integrated to 1e7 yr against scipy's BDF
-    (rtol 1e-10) on the restated RHS, species above 1e-14; the auto variant
-    picks the same form."""
-    import os
+    """The fixture cells of the 103-species network (two wavefronts per cell,
+    sparse J and the MFMA panel LU in LDS) integrated to 1e7 yr against
+    scipy's BDF (rtol 1e-10) on the restated RHS, species above 1e-14; the
+    auto variant picks the same form.  (Of the six drawn cells, those scipy
+    did not finish within the fixture script's time limit are not here.)"""
     import torch
-    from conftest import GOLDEN
     from prism3d_amd.chemistry import solve_opts
-    if not os.path.exists(os.path.join(GOLDEN, "synth_wide_golden.npz")):
-        pytest.skip("synth_wide_golden.npz not generated (tests/golden/make_synth_golden.py --wide)")
     g = golden("synth_wide_golden.npz")
     sp, rx, cs = _wide_solver()
     par = g['params']

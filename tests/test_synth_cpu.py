@@ -95,14 +95,17 @@ def test_generic_jacobian_directional_derivatives_on_the_synthetic_network():
 
 
 def test_synthetic_fixture_conserves_nuclei():
-    """The scipy reference states of the synthetic network keep every
+    """The scipy reference states of both synthetic networks keep every
     element's nuclei (the reactions all balance; PAH recombinations move
     only charge), to the integration tolerance."""
-    from prism3d_amd.network_synth import formula, synthetic_network
-    g = golden("synth_golden.npz")
-    sp, _ = synthetic_network()
-    elems = sorted({e for s in sp for e in formula(s)[0]})
-    C = np.array([[formula(s)[0].get(e, 0) for s in sp] for e in elems], dtype=np.float64)
-    t0 = C @ g['y0']
-    for y in g['y_end']:
-        assert np.max(np.abs(C @ y - t0) / t0) < 1e-8
+    from prism3d_amd.network_synth import formula, synthetic_network, synthetic_network_wide
+    for name, make in (("synth_golden.npz", synthetic_network),
+                       ("synth_wide_golden.npz", synthetic_network_wide)):
+        g = golden(name)
+        sp, _ = make()
+        elems = sorted({e for s in sp for e in formula(s)[0]})
+        C = np.array([[formula(s)[0].get(e, 0) for s in sp] for e in elems], dtype=np.float64)
+        t0 = C @ g['y0']
+        assert len(g['y_end']) >= 5
+        for y in g['y_end']:
+            assert np.max(np.abs(C @ y - t0) / t0) < 1e-8
