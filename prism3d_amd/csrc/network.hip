@@ -527,32 +527,60 @@ __device__ void lu_factor_inplace(const NetDev &N, const Lds &L, int lane) {
 }
 
 // Solve M x = b with the factors (b and x live in registers, lane = row).
+// The pivots and the lane's row entries are read four per LDS round trip,
+// ahead of the readlane chain of the substitutions.
+constexpr int kSolveChunk = 4;
 __device__ double lu_solve(const NetDev &N, const Lds &L, double b, int lane) {
   const int n = N.ns, ld = L.ld;
-  for (int k = 0; k < n; ++k) {
-    const int pk = uniform(L.piv[k]);
-    if (pk != k) {
-      double bk = bcast(b, k), bp = bcast(b, pk);
-      if (lane == k) b = bp;
-      if (lane == pk) b = bk;
+  const double *Mr = L.M + min(lane, n - 1) * ld;  // rows past n: entries unused
+  for (int k0 = 0; k0 < n; k0 += kSolveChunk) {
+    int pv[kSolveChunk];
+#pragma unroll
+    for (int q = 0; q < kSolveChunk; ++q) pv[q] = L.piv[min(k0 + q, n - 1)];
+#pragma unroll
+    for (int q = 0; q < kSolveChunk; ++q) {
+      const int k = k0 + q, pk = uniform(pv[q]);
+      if (k < n && pk != k) {
+        double bk = bcast(b, k), bp = bcast(b, pk);
+        if (lane == k) b = bp;
+        if (lane == pk) b = bk;
+      }
     }
   }
-  for (int k = 0; k < n - 1; ++k) {
-    double bk = bcast(b, k);
-    if (lane > k && lane < n) b -= L.M[lane * ld + k] * bk;
+  for (int k0 = 0; k0 < n - 1; k0 += kSolveChunk) {
+    double a[kSolveChunk];
+#pragma unroll
+    for (int q = 0; q < kSolveChunk; ++q) a[q] = Mr[min(k0 + q, n - 1)];
+#pragma unroll
+    for (int q = 0; q < kSolveChunk; ++q) {
+      const int k = k0 + q;
+      if (k < n - 1) {
+        double bk = bcast(b, k);
+        if (lane > k && lane < n) b -= a[q] * bk;
+      }
+    }
   }
-  for (int k = n - 1; k >= 0; --k) {
-    if (lane == k) b /= L.M[k * ld + k];
-    double bk = bcast(b, k);
-    if (lane < k) b -= L.M[lane * ld + k] * bk;
+  for (int k0 = n - 1; k0 >= 0; k0 -= kSolveChunk) {
+    double a[kSolveChunk];
+#pragma unroll
+    for (int q = 0; q < kSolveChunk; ++q) a[q] = Mr[max(k0 - q, 0)];
+#pragma unroll
+    for (int q = 0; q < kSolveChunk; ++q) {
+      const int k = k0 - q;
+      if (k >= 0) {
+        if (lane == k) b /= a[q];  // the lane's own diagonal entry
+        double bk = bcast(b, k);
+        if (lane < k) b -= a[q] * bk;
+      }
+    }
   }
   return b;
 }
 
 // lu_solve for the two-wavefront form, run by wavefront 0 alone: lane l
 // holds rows l and l + 64 and the solved components travel by readlane, as
-// in lu_solve (same operations in the same order); b enters and leaves
-// through an LDS vector, two barriers per solve.
+// in lu_solve (same operations in the same order, the same chunked reads);
+// b enters and leaves through an LDS vector, two barriers per solve.
 __device__ double lu_solve_wide(const NetDev &N, const Lds &L, double b, int lane) {
   const int n = N.ns, ld = L.ld;
   double *x = L.xs;
@@ -560,35 +588,68 @@ __device__ double lu_solve_wide(const NetDev &N, const Lds &L, double b, int lan
   __syncthreads();
   if (lane < 64) {
     const int r0 = lane, r1 = lane + 64;
+    const double *M0 = L.M + min(r0, n - 1) * ld, *M1 = L.M + min(r1, n - 1) * ld;
     double b0 = r0 < n ? x[r0] : 0.0, b1 = r1 < n ? x[r1] : 0.0;
-    for (int k = 0; k < n; ++k) {
-      const int pk = uniform(L.piv[k]);
-      if (pk != k) {
-        const double bk = bcast(k < 64 ? b0 : b1, k & 63);
-        const double bp = bcast(pk < 64 ? b0 : b1, pk & 63);
-        if (lane == (k & 63)) {
-          if (k < 64) b0 = bp;
-          else b1 = bp;
-        }
-        if (lane == (pk & 63)) {
-          if (pk < 64) b0 = bk;
-          else b1 = bk;
+    for (int k0 = 0; k0 < n; k0 += kSolveChunk) {
+      int pv[kSolveChunk];
+#pragma unroll
+      for (int q = 0; q < kSolveChunk; ++q) pv[q] = L.piv[min(k0 + q, n - 1)];
+#pragma unroll
+      for (int q = 0; q < kSolveChunk; ++q) {
+        const int k = k0 + q, pk = uniform(pv[q]);
+        if (k < n && pk != k) {
+          const double bk = bcast(k < 64 ? b0 : b1, k & 63);
+          const double bp = bcast(pk < 64 ? b0 : b1, pk & 63);
+          if (lane == (k & 63)) {
+            if (k < 64) b0 = bp;
+            else b1 = bp;
+          }
+          if (lane == (pk & 63)) {
+            if (pk < 64) b0 = bk;
+            else b1 = bk;
+          }
         }
       }
     }
-    for (int k = 0; k < n - 1; ++k) {
-      const double bk = bcast(k < 64 ? b0 : b1, k & 63);
-      if (r0 > k && r0 < n) b0 -= L.M[r0 * ld + k] * bk;
-      if (r1 > k && r1 < n) b1 -= L.M[r1 * ld + k] * bk;
-    }
-    for (int k = n - 1; k >= 0; --k) {
-      if (lane == (k & 63)) {
-        if (k < 64) b0 /= L.M[k * ld + k];
-        else b1 /= L.M[k * ld + k];
+    for (int k0 = 0; k0 < n - 1; k0 += kSolveChunk) {
+      double a0[kSolveChunk], a1[kSolveChunk];
+#pragma unroll
+      for (int q = 0; q < kSolveChunk; ++q) {
+        const int k = min(k0 + q, n - 1);
+        a0[q] = M0[k];
+        a1[q] = M1[k];
       }
-      const double bk = bcast(k < 64 ? b0 : b1, k & 63);
-      if (r0 < k) b0 -= L.M[r0 * ld + k] * bk;
-      if (r1 < k) b1 -= L.M[r1 * ld + k] * bk;
+#pragma unroll
+      for (int q = 0; q < kSolveChunk; ++q) {
+        const int k = k0 + q;
+        if (k < n - 1) {
+          const double bk = bcast(k < 64 ? b0 : b1, k & 63);
+          if (r0 > k && r0 < n) b0 -= a0[q] * bk;
+          if (r1 > k && r1 < n) b1 -= a1[q] * bk;
+        }
+      }
+    }
+    for (int k0 = n - 1; k0 >= 0; k0 -= kSolveChunk) {
+      double a0[kSolveChunk], a1[kSolveChunk];
+#pragma unroll
+      for (int q = 0; q < kSolveChunk; ++q) {
+        const int k = max(k0 - q, 0);
+        a0[q] = M0[k];
+        a1[q] = M1[k];
+      }
+#pragma unroll
+      for (int q = 0; q < kSolveChunk; ++q) {
+        const int k = k0 - q;
+        if (k >= 0) {
+          if (lane == (k & 63)) {  // the lane's own diagonal entry
+            if (k < 64) b0 /= a0[q];
+            else b1 /= a1[q];
+          }
+          const double bk = bcast(k < 64 ? b0 : b1, k & 63);
+          if (r0 < k) b0 -= a0[q] * bk;
+          if (r1 < k) b1 -= a1[q] * bk;
+        }
+      }
     }
     x[r0] = b0;
     x[r1] = b1;
