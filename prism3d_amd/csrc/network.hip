@@ -34,6 +34,7 @@ namespace p3d {
 constexpr int kMaxSpecies = 64;       // one-wavefront forms: lane = species
 constexpr int kMaxSpeciesWide = 128;  // two-wavefront form of the solve
 constexpr int kMaxPieces = kMaxSpeciesWide + 128;  // RHS pieces: rows + lane cuts
+constexpr int kRxPerLaneCap = 2;  // reactions per lane of the register cache (kRxPerLane)
 constexpr int kMaxOrder = 5;
 constexpr int kNumD = kMaxOrder + 3;
 
@@ -116,11 +117,13 @@ __device__ __forceinline__ Lds carve(double *base, const NetDev &N) {
   L.M = L.J + N.ns * L.ld;
   L.piv = reinterpret_cast<int *>(L.M + N.ns * L.ld);
   L.ru = reinterpret_cast<double *>(L.piv + kMaxSpecies);
-  L.csr_off = reinterpret_cast<int *>(L.ru + 36);
+  // piece sums of the balanced RHS, for networks past the register cache
+  const bool pieces = N.nr > 64 * kRxPerLaneCap;
+  L.psum = pieces ? L.ru + 36 : nullptr;
+  L.csr_off = reinterpret_cast<int *>(L.ru + 36 + (pieces ? kMaxPieces : 0));
   L.csr = L.csr_off + kMaxSpecies + 1;
   L.red = nullptr;
   L.xs = nullptr;
-  L.psum = nullptr;
   return L;
 }
 
@@ -147,7 +150,8 @@ __device__ __forceinline__ Lds carve_wide(double *base, const NetDev &N) {
 
 static size_t lds_bytes(int ns, int nr, int n_rhs_e) {
   int ld = ns | 1;
-  return sizeof(double) * (size_t)(3 * nr + kMaxSpecies + 2 * ns * ld + 36) +
+  const int pieces = nr > 64 * kRxPerLaneCap ? kMaxPieces : 0;
+  return sizeof(double) * (size_t)(3 * nr + kMaxSpecies + 2 * ns * ld + 36 + pieces) +
          sizeof(int) * (size_t)(2 * kMaxSpecies + 1 + n_rhs_e);
 }
 
@@ -219,7 +223,7 @@ __device__ __forceinline__ void stage_tables(const NetDev &N, const Lds &L, int 
 }
 
 // Reactions evaluated by one lane (r = lane + 64 s), kept in registers.
-constexpr int kRxPerLane = 2;
+constexpr int kRxPerLane = kRxPerLaneCap;
 struct RxCache {
   int t[kRxPerLane], a[kRxPerLane], b[kRxPerLane];
   double k[kRxPerLane];
@@ -441,16 +445,16 @@ __device__ void jacobian_sparse(const NetDev &N, const CellParams &p, const Lds 
   __syncthreads();
 }
 
-// dy/dt for the two-wavefront form: lane l sums the entries of its share
-// piece by piece (each piece in the upstream order) into psum, then species
-// s adds its pieces in order.  A species whose row lies in one piece gets
-// the bits of rhs_lane; a row split over lanes differs by the rounding of
-// the split.  Requires y published.
+// dy/dt, balanced over the lanes: the entries are cut into 128 shares; lane
+// l sums shares l, l + W, ... piece by piece (each piece in the upstream
+// order) into psum, then species s adds its pieces in order.  A species
+// whose row lies in one piece gets the bits of rhs_lane; a row split over
+// shares differs by the rounding of the split.  Requires y published.
 template <int W>
 __device__ double rhs_balanced(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
   eval_reactions<W>(N, p, L, lane, false);
-  const int q0 = N.pc_lane[lane], q1 = N.pc_lane[lane + 1];
-  for (int q = q0; q < q1; ++q) {
+  for (int sh = lane; sh < kMaxSpeciesWide; sh += W)
+  for (int q = N.pc_lane[sh]; q < N.pc_lane[sh + 1]; ++q) {
     const int e1 = N.pc_beg[q + 1];
     int e = N.pc_beg[q];
     double acc = 0.0;
@@ -983,7 +987,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
   const bool cached = G::W == 64 && N.nr <= 64 * kRxPerLane;
   auto rhs = [&]() {
     if constexpr (G::W == 64)
-      return cached ? rhs_cached(N, p, L, rc, lane) : rhs_lane(N, p, L, lane);
+      return cached ? rhs_cached(N, p, L, rc, lane) : rhs_balanced<64>(N, p, L, lane);
     else
       return rhs_balanced<G::W>(N, p, L, lane);
   };
