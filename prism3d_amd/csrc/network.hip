@@ -453,27 +453,28 @@ __device__ void jacobian_sparse(const NetDev &N, const CellParams &p, const Lds 
 template <int W>
 __device__ double rhs_balanced(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
   eval_reactions<W>(N, p, L, lane, false);
-  for (int sh = lane; sh < kMaxSpeciesWide; sh += W)
-  for (int q = N.pc_lane[sh]; q < N.pc_lane[sh + 1]; ++q) {
-    const int e1 = N.pc_beg[q + 1];
-    int e = N.pc_beg[q];
-    double acc = 0.0;
-    for (; e + 4 <= e1; e += 4) {
-      const int c0 = N.rhs_pk[e], c1 = N.rhs_pk[e + 1], c2 = N.rhs_pk[e + 2],
-                c3 = N.rhs_pk[e + 3];
-      const double v0 = L.part[2 * (c0 >> 1)], v1 = L.part[2 * (c1 >> 1)],
-                   v2 = L.part[2 * (c2 >> 1)], v3 = L.part[2 * (c3 >> 1)];
-      acc = (c0 & 1) ? acc + v0 : acc - v0;
-      acc = (c1 & 1) ? acc + v1 : acc - v1;
-      acc = (c2 & 1) ? acc + v2 : acc - v2;
-      acc = (c3 & 1) ? acc + v3 : acc - v3;
+  for (int sh = lane; sh < kMaxSpeciesWide; sh += W) {
+    for (int q = N.pc_lane[sh]; q < N.pc_lane[sh + 1]; ++q) {
+      const int e1 = N.pc_beg[q + 1];
+      int e = N.pc_beg[q];
+      double acc = 0.0;
+      for (; e + 4 <= e1; e += 4) {
+        const int c0 = N.rhs_pk[e], c1 = N.rhs_pk[e + 1], c2 = N.rhs_pk[e + 2],
+                  c3 = N.rhs_pk[e + 3];
+        const double v0 = L.part[2 * (c0 >> 1)], v1 = L.part[2 * (c1 >> 1)],
+                     v2 = L.part[2 * (c2 >> 1)], v3 = L.part[2 * (c3 >> 1)];
+        acc = (c0 & 1) ? acc + v0 : acc - v0;
+        acc = (c1 & 1) ? acc + v1 : acc - v1;
+        acc = (c2 & 1) ? acc + v2 : acc - v2;
+        acc = (c3 & 1) ? acc + v3 : acc - v3;
+      }
+      for (; e < e1; ++e) {
+        const int c0 = N.rhs_pk[e];
+        const double v0 = L.part[2 * (c0 >> 1)];
+        acc = (c0 & 1) ? acc + v0 : acc - v0;
+      }
+      L.psum[q] = acc;
     }
-    for (; e < e1; ++e) {
-      const int c0 = N.rhs_pk[e];
-      const double v0 = L.part[2 * (c0 >> 1)];
-      acc = (c0 & 1) ? acc + v0 : acc - v0;
-    }
-    L.psum[q] = acc;
   }
   __syncthreads();
   double acc = 0.0;
