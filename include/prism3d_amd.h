@@ -34,6 +34,9 @@
  *   p3d_euler_rates            prism3d/solver_3d.py:549-557
  *   p3d_thermal_ctx            prism3d/solver_3d.py:693-731
  *   p3d_heating_cooling        prism3d/solver_3d.py:898-1050
+ *   p3d_loop_snapshot, p3d_damp_abundances, p3d_damp_temperature,
+ *   p3d_convergence            prism3d/solver_3d.py:262-436 (the damping
+ *                              and percentile test of PDRSolver3D.run)
  */
 #ifndef PRISM3D_AMD_H
 #define PRISM3D_AMD_H
@@ -339,6 +342,60 @@ int p3d_heating_cooling(int64_t n, const double *t_gas, const double *n_h,
                         const double *oi63_5, const double *oi145_5,
                         const double *ci10, double *gamma_out,
                         double *lambda_out, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* The tail of an inner fixed-point iteration (solver_3d.py:262-436)        */
+/* ---------------------------------------------------------------------- */
+
+/* State at the start of an iteration: t_old <- t_gas and y_old <- y
+ * ((n_species, n) abundances; n_species may be 0).  The reference keeps
+ * old_T, old_x_H2 and old_x_CO (solver_3d.py:265-267); the whole abundance
+ * state is kept here so that every damping mode can read it. */
+int p3d_loop_snapshot(int64_t n, int32_t n_species, const double *t_gas,
+                      const double *y, double *t_old, double *y_old,
+                      void *stream);
+
+/* Abundance damping after the chemistry of an iteration (iterations > 0),
+ * in place on y (n_species, n), from y_old of p3d_loop_snapshot.
+ * rows5 (HOST) = row indices of H, H2, C+, C, CO (unused by UNIFORM). */
+#define P3D_DAMP_REFERENCE 0   /* x_H2, x_CO 50/50 with the previous state;
+                                  x_H = max(1 - 2 x_H2, 1e-30),
+                                  x_C+ = max(x_c_total - x_CO - x_C, 1e-30)
+                                  (solver_3d.py:345-354)                   */
+#define P3D_DAMP_UNIFORM 1     /* every abundance 50/50 (keeps each element's
+                                  nuclei and the charge of the state)      */
+#define P3D_DAMP_OSCILLATING 2 /* cells with w_damp > 0.55 only, weight w_damp,
+                                  same closures (solver_3d.py:330-345)     */
+int p3d_damp_abundances(int64_t n, int32_t mode, int32_t n_species, double *y,
+                        const double *y_old, const double *w_damp,
+                        const int32_t *rows5, double x_c_total, void *stream);
+
+/* Adaptive temperature damping (solver_3d.py:357-375): with
+ * r = |T - T_old| / max(T_old, 10), w_damp += 0.15 where r > 0.5, -= 0.02
+ * where r < 0.1, clipped to [0.5, 0.98] (in place), then
+ * t_gas <- w_damp t_old + (1 - w_damp) t_gas. */
+int p3d_damp_temperature(int64_t n, const double *t_old, double *t_gas,
+                         double *w_damp, void *stream);
+
+/* Convergence figures of an iteration (solver_3d.py:381-432), computed on
+ * the device: rel_T = |T - T_old| / T_old over T_old > 10,
+ * rel_H2 = |dx_H2| / max(x_H2_old, 0.01) over x_H2_old > 1e-6,
+ * rel_CO = |dx_CO| / max(x_CO_old, 1e-6) over x_CO_old > 1e-8 (an empty
+ * mask gives the single value 0).  record (DEVICE, P3D_CONV_FIELDS doubles):
+ * [0] p95 rel_T (dT), [1] p99 rel_T, [2] p99 rel_H2 (dH2), [3] p99 rel_CO
+ * (dCO), [4] max rel_T, [5] max rel_H2, [6] cells with rel_T > tol,
+ * [7] max(dT, dH2, dCO) (delta_max), [8..10] values in each mask,
+ * [11] 1.0 when delta_max < tol.  Percentiles are exact order statistics
+ * interpolated as numpy's default ('linear') method, so they equal
+ * np.percentile bit for bit.  workspace: DEVICE buffer of
+ * P3D_CONV_WORKSPACE_BYTES (8-byte aligned); calls sharing one workspace
+ * must be ordered on one stream. */
+#define P3D_CONV_FIELDS 12
+#define P3D_CONV_WORKSPACE_BYTES 16384
+int p3d_convergence(int64_t n, const double *t_old, const double *t_gas,
+                    const double *h2_old, const double *x_h2,
+                    const double *co_old, const double *x_co, double tol,
+                    double *record, void *workspace, void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Tree-ensemble chemistry surrogate                                         */

@@ -14,6 +14,9 @@ P3D_OK = 0
 P3D_RX_TWO_BODY, P3D_RX_PHOTO, P3D_RX_PHOTO_FH2, P3D_RX_PHOTO_FCO, P3D_RX_CR, P3D_RX_GRAIN = range(6)
 P3D_SP_NONE = -1
 P3D_SP_PAH_MINUS = -2
+P3D_DAMP_REFERENCE, P3D_DAMP_UNIFORM, P3D_DAMP_OSCILLATING = range(3)
+P3D_CONV_FIELDS = 12
+P3D_CONV_WORKSPACE_BYTES = 16384
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I32P = ctypes.POINTER(ctypes.c_int32)
@@ -74,6 +77,10 @@ SIGNATURES = {
     "p3d_euler_rates": (_i32, [_i64] + [_VP] * 8 + [_VP] * 6 + [_VP]),
     "p3d_thermal_ctx": (_i32, [_i64] + [_VP] * 15 + [_VP] * 9 + [_VP]),
     "p3d_heating_cooling": (_i32, [_i64] + [_VP] * 16 + [_D] * 4 + [_VP, _VP, _VP]),
+    "p3d_loop_snapshot": (_i32, [_i64, _i32, _VP, _VP, _VP, _VP, _VP]),
+    "p3d_damp_abundances": (_i32, [_i64, _i32, _i32, _VP, _VP, _VP, _I32P, _dbl, _VP]),
+    "p3d_damp_temperature": (_i32, [_i64, _VP, _VP, _VP, _VP]),
+    "p3d_convergence": (_i32, [_i64] + [_VP] * 6 + [_dbl, _VP, _VP, _VP]),
 }
 
 _lib = None

@@ -106,7 +106,7 @@ class CloudBatch:
                     c._resolve_timing()
                     rec['dust_step'] = dust_step
                     rec['iteration'] = it
-                    rec['f_nano_mean'] = float(c.f_nano.mean())
+                    rec['f_nano_mean'] = c.f_nano_mean()
                     c.convergence_history.append(rec)
                     if verbose:
                         print(f"  cloud {i} {dust_step}.{it:2d} dT={rec['dT']:.2e} "

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libp3d_hip.so")
 SOURCES = ["core_kernels.hip", "network.hip", "grid_kernels.hip", "lane_kernels.hip",
-           "surrogate.hip"]
+           "surrogate.hip", "loop_kernels.hip"]
 ARCH = os.environ.get("P3D_OFFLOAD_ARCH", "gfx950")
 
 

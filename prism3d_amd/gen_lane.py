@@ -351,6 +351,47 @@ def generate(net=None):
     return "\n".join(L) + "\n", dict(NJ=NJ, NLU=NLU, perm=perm, jcells=jcells)
 
 
+def op_counts(net=None):
+    """FP64 operations of one evaluation of each block of the sparse solve
+    the generated code runs (a multiply-add counts 2): the RHS (rate products
+    and stoichiometric updates), the analytic Jacobian (partials and their
+    row updates), the no-pivot LU of M = I - c J in the minimum-Markowitz
+    order (matrix build, multipliers, row updates, reciprocal pivots) and
+    the two triangular solves.  These are the work the algorithm needs,
+    independent of how a kernel stores or pivots the matrix."""
+    from .network import Network
+    net = net or Network()
+    ns, nr = net.n_species, net.n_reactions
+    rt, re, pr = net.rtype, net.reactants, net.products
+    dep, nets, terms = structure(net)
+    pat = np.zeros((ns, ns), bool)
+    for (i, j) in terms:
+        pat[i, j] = True
+    for i in range(ns):
+        pat[i, i] = True
+    perm = markowitz(pat, ns)
+    F = fill(pat[np.ix_(perm, perm)], ns)
+    rhs = 0
+    for r in range(nr):
+        t = int(rt[r])
+        rhs += 3 if t in (TWO_BODY, GRAIN) else 1
+        slots = 2 if t in (TWO_BODY, GRAIN) else 1
+        rhs += sum(1 for k in range(slots) if re[2 * r + k] >= 0)
+        rhs += sum(1 for k in range(3) if pr[3 * r + k] >= 0)
+    jac = sum(3 for _ in {(r, sl) for v in terms.values() for (r, sl, _c) in v})
+    jac += 2 * sum(len(v) for v in terms.values())
+    nj = int(pat.sum())
+    lu = 2 * nj + ns  # M entries, reciprocal pivots
+    solve = ns
+    for pi in range(ns):
+        for pk in range(pi):
+            if F[pi, pk]:
+                lu += 1 + 2 * int(F[pk, pk + 1:].sum())
+                solve += 2
+        solve += 2 * int(F[pi, pi + 1:].sum())
+    return dict(rhs=rhs, jacobian=jac, lu=lu, solve=solve, n_species=ns)
+
+
 def write(path=OUT):
     text, info = generate()
     old = open(path).read() if os.path.exists(path) else None

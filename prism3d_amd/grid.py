@@ -215,3 +215,82 @@ def solve_thermal(gamma_fixed, gg_coeff, t_dust, n_e, n_hi, n_h2, n_co, n_cp, n_
                                           int(n_newton), _t(t_gas).data_ptr(), stream_handle())
     _lib.check(rc, "p3d_solve_thermal_vec")
     return t_gas
+
+
+# --- the tail of an inner iteration (solver_3d.py:262-436) --------------------
+CONV_KEYS = ('dT', 'dT_p99', 'dH2', 'dCO', 'dT_max', 'dH2_max', 'n_oscillating', 'delta_max')
+DAMP_MODES = {'reference': _lib.P3D_DAMP_REFERENCE, 'uniform': _lib.P3D_DAMP_UNIFORM,
+              'oscillating': _lib.P3D_DAMP_OSCILLATING}
+
+
+def conv_workspace(dev=None):
+    """Device workspace of one convergence() call sequence."""
+    return torch.empty(_lib.P3D_CONV_WORKSPACE_BYTES // 8, dtype=torch.float64,
+                       device=dev if dev is not None else device())
+
+
+def loop_snapshot(t_gas, y, t_old, y_old):
+    """t_old <- t_gas, y_old <- y (the state at the start of an iteration)."""
+    t_gas, t_old = _t(t_gas), _t(t_old)
+    y, y_old = _t(y), _t(y_old)
+    n = t_gas.numel()
+    if t_old.numel() != n or y.shape != y_old.shape or y.numel() != y.shape[0] * n:
+        raise ValueError("snapshot buffers do not match the state")
+    _lib.check(_lib.lib().p3d_loop_snapshot(n, y.shape[0], t_gas.data_ptr(), y.data_ptr(),
+                                            t_old.data_ptr(), y_old.data_ptr(), stream_handle()),
+               "p3d_loop_snapshot")
+
+
+def damp_abundances(y, y_old, mode, rows5, x_c_total, w_damp=None):
+    """Abundance damping of solver_3d.py:330-354 in place on y (n_species, n):
+    mode 'reference', 'uniform' or 'oscillating'; rows5 = rows of H, H2, C+,
+    C, CO."""
+    y, y_old = _t(y), _t(y_old)
+    if y.shape != y_old.shape or y.ndim != 2:
+        raise ValueError("y and y_old must share one (n_species, n) shape")
+    n = y.shape[1]
+    w = None
+    if w_damp is not None:
+        w = _t(w_damp)
+        if w.numel() != n:
+            raise ValueError("w_damp must hold one weight per cell")
+    r = np.ascontiguousarray(rows5, dtype=np.int32)
+    _lib.check(_lib.lib().p3d_damp_abundances(n, DAMP_MODES[mode], y.shape[0], y.data_ptr(),
+                                              y_old.data_ptr(),
+                                              w.data_ptr() if w is not None else None,
+                                              _lib.i32ptr(r), float(x_c_total), stream_handle()),
+               "p3d_damp_abundances")
+
+
+def damp_temperature(t_old, t_gas, w_damp):
+    """Adaptive T damping of solver_3d.py:357-375, in place on t_gas and w_damp."""
+    t_old, t_gas, w = _t(t_old), _t(t_gas), _t(w_damp)
+    n = t_gas.numel()
+    if t_old.numel() != n or w.numel() != n:
+        raise ValueError("t_old, t_gas and w_damp must have one value per cell")
+    _lib.check(_lib.lib().p3d_damp_temperature(n, t_old.data_ptr(), t_gas.data_ptr(),
+                                               w.data_ptr(), stream_handle()),
+               "p3d_damp_temperature")
+
+
+def convergence(t_old, t_gas, h2_old, x_h2, co_old, x_co, tol, record, workspace):
+    """Convergence figures of solver_3d.py:381-432 into the device record
+    (P3D_CONV_FIELDS doubles), enqueued on the current stream."""
+    a = [_t(v) for v in (t_old, t_gas, h2_old, x_h2, co_old, x_co)]
+    n = a[0].numel()
+    if any(v.numel() != n for v in a):
+        raise ValueError("the six state arrays must have one value per cell")
+    if record.numel() < _lib.P3D_CONV_FIELDS or workspace.numel() * workspace.element_size() \
+            < _lib.P3D_CONV_WORKSPACE_BYTES:
+        raise ValueError("record or workspace too small")
+    _lib.check(_lib.lib().p3d_convergence(n, *[v.data_ptr() for v in a], float(tol),
+                                          record.data_ptr(), workspace.data_ptr(),
+                                          stream_handle()), "p3d_convergence")
+
+
+def convergence_record(record):
+    """The device record as the reference's convergence-history fields."""
+    r = record.cpu().numpy()
+    rec = {k: float(r[i]) for i, k in enumerate(CONV_KEYS)}
+    rec['n_oscillating'] = int(r[6])
+    return rec

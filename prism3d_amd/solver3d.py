@@ -50,9 +50,9 @@ class PDRSolverGPU:
     def __init__(self, density, box_size, G0_external=1.0, zeta_CR_0=2e-16, nside_rt=1,
                  fixed_T=None, chemistry='euler', x_c_total=GAS_C, x_o_total=GAS_O,
                  av_per_nh=grid.AV_PER_NH, t_chem=1e7 * YR, chem_rtol=1e-9, chem_atol=1e-20,
-                 initial_abundances=None, T_init=50.0, chem_mode='steady', chem_variant=0,
+                 initial_abundances=None, T_init=50.0, chem_mode='integrate', chem_variant=0,
                  chem_waves_per_cu=0, chem_rescue_frac=0.0, chem_autotune=False, star=None,
-                 velocity_field=None):
+                 velocity_field=None, abundance_damping='reference', chem_post_mode=0):
         self.dev = device()
         dens = np.asarray(density, dtype=np.float64)
         self.shape = dens.shape
@@ -76,14 +76,40 @@ class PDRSolverGPU:
         self.x_c_total, self.x_o_total = float(x_c_total), float(x_o_total)
         self.av_per_nh = float(av_per_nh)
         self.t_chem, self.chem_rtol, self.chem_atol = float(t_chem), chem_rtol, chem_atol
-        # 'steady': every iteration continues each cell from its current
-        # abundances to the upstream steady-state horizon
-        # t_max = min(3e17, max(1e16, 3e17 / n_H)) s (chemistry/solver.py:150-154),
-        # as refine() does; 'integrate': every iteration restarts from the
-        # initial abundances and integrates to t_chem.
+        # 'integrate' (default): every iteration restarts each cell from the
+        # initial abundances and integrates to t_chem, so the chemistry of an
+        # iteration is a fixed function of that iteration's T, radiation and
+        # shielding, and the loop has a fixed point.  'steady': every iteration
+        # continues each cell from its current abundances to the upstream
+        # steady-state horizon t_max = min(3e17, max(1e16, 3e17 / n_H)) s
+        # (chemistry/solver.py:150-154), as refine() does once; the network's
+        # PAH recombinations move charge into the untracked PAH-, so x_e grows
+        # with every further span and that loop drifts instead of converging.
         if chem_mode not in ('steady', 'integrate'):
             raise ValueError("chem_mode must be 'steady' or 'integrate'")
         self.chem_mode = chem_mode
+        # abundance damping after the chemistry of every iteration but the
+        # first, for the explicit and the network chemistry (the surrogate
+        # damps oscillating cells only, solver_3d.py:330-345):
+        # 'reference': x_H2 and x_CO averaged 50/50 with the previous
+        #   iteration, x_HI = 1 - 2 x_H2 and x_C+ = C - x_CO - x_C
+        #   (solver_3d.py:345-354);
+        # 'uniform': every tracked abundance averaged 50/50 (keeps each
+        #   element's nuclei and the charge of the network state);
+        # 'none': no abundance damping.
+        if abundance_damping not in ('reference', 'uniform', 'none'):
+            raise ValueError("abundance_damping must be 'reference', 'uniform' or 'none'")
+        self.abundance_damping = abundance_damping
+        # post-solve corrections of the in-loop network chemistry: 0 the
+        # upstream H+/e- correction and clipped conservation that end every
+        # solve_steady_state (chemistry/solver.py:219-223, 227-388), 1 the
+        # exact projection onto the element totals, -1 none.  The raw network
+        # moves charge into the untracked PAH- (its PAH recombinations), so
+        # without the electron closure every further span of integration
+        # raises x_e and the fixed point drifts.
+        if chem_post_mode not in (-1, 0, 1):
+            raise ValueError("chem_post_mode must be -1, 0 or 1")
+        self.chem_post_mode = int(chem_post_mode)
         self.chem_variant = int(chem_variant)
         self.chem_waves_per_cu = int(chem_waves_per_cu)
         self.chem_rescue_frac = float(chem_rescue_frac)
@@ -121,6 +147,15 @@ class PDRSolverGPU:
         for s, v in initial_abundances.items():
             self.y[self.sidx[s]] = float(v)
         self.y_init = self.y.clone()
+        # the state at the start of an inner iteration, the damping weights
+        # and the device convergence record (fixed buffers, so that a captured
+        # iteration reads and writes the same memory on every replay)
+        self._T_old = torch.empty_like(self.T_gas)
+        self._y_old = torch.empty_like(self.y)
+        self._conv_rec = torch.zeros(_lib.P3D_CONV_FIELDS, dtype=torch.float64, device=self.dev)
+        self._conv_ws = grid.conv_workspace(self.dev)
+        self._fn_mean = torch.zeros(1, dtype=torch.float64, device=self.dev)
+        self._damp_rows = [self.sidx[s] for s in ('H', 'H2', 'C+', 'C', 'CO')]
         self.pe = PETable()
         self._pe_table = torch.tensor(self.pe.table, device=self.dev)
         self._pe_bins = [torch.tensor(a, device=self.dev) for a in
@@ -219,7 +254,8 @@ class PDRSolverGPU:
                     y=self.y[:, idx].cpu().numpy(), n_H=flat(self.density), T=flat(self.T_gas),
                     G0=flat(self.G0), A_V=flat(self.A_V), zeta=flat(self.zeta_CR),
                     fsh_H2=flat(self.fsh_H2), fsh_CO=flat(self.fsh_CO),
-                    t_max=o.t_max))
+                    t_max=o.t_max, post_mode=o.post_mode, x_c_total=o.x_c_total,
+                    x_o_total=o.x_o_total))
             timed = self.record_chem or (self.tuner is not None and self.tuner.tuning)
             if timed:
                 ev0 = torch.cuda.Event(enable_timing=True)
@@ -231,6 +267,8 @@ class PDRSolverGPU:
                 self.fsh_CO.reshape(-1), opts=o, sync=False)
             if timed:
                 ev1.record()
+            if self.record_inputs is not None:  # the GPU result at the same cells
+                self.recorded[-1]['y_out'] = self.y[:, self.record_inputs].cpu().numpy()
             if self.record_chem:
                 self.chem_records.append((ev0, ev1, self.chem_stats))
             if self.tuner is not None and self.tuner.tuning:
@@ -248,13 +286,16 @@ class PDRSolverGPU:
 
     def chem_opts(self, cfg=None):
         """Solve options of the in-loop chemistry: one span from t = 1e4 s to
-        t_chem ('integrate') or to the steady-state horizon ('steady'), no
-        post-solve corrections; cfg = (variant, waves_per_cu, rescue_frac)."""
+        t_chem ('integrate') or to the steady-state horizon ('steady'), then
+        the post-solve corrections of chem_post_mode; cfg = (variant,
+        waves_per_cu, rescue_frac)."""
         if cfg is None:
             cfg = (self.chem_variant, self.chem_waves_per_cu, self.chem_rescue_frac)
         t_max = self.t_chem if self.chem_mode == 'integrate' else None
         return solve_opts(t_max=t_max, t_start=1e4, n_checkpoints=2, early_stop=False,
-                          rtol=self.chem_rtol, atol=self.chem_atol, post_mode=-1,
+                          rtol=self.chem_rtol, atol=self.chem_atol,
+                          post_mode=self.chem_post_mode, x_c_total=self.x_c_total,
+                          x_o_total=self.x_o_total,
                           variant=cfg[0], waves_per_cu=cfg[1], rescue_frac=cfg[2])
 
     def thermal_step(self):
@@ -262,9 +303,9 @@ class PDRSolverGPU:
                                self.T_dust, self.Gamma_PE, self._R_H2, self.x_HI, self.x_H2,
                                self.x_e, self.x_CO, self.x_Cp, self.x_O, self.x_C)
         gf, gg, ne, nhi, nh2, nco, ncp, no, nc = ctx
-        T = torch.empty_like(self.T_gas)
-        grid.solve_thermal(gf, gg, self.T_dust, ne, nhi, nh2, nco, ncp, no, nc, T)
-        self.T_gas = T
+        # the root-find reads no incoming T (thermal.rs:21-360), so T_gas is
+        # overwritten in place and stays one buffer over the whole loop
+        grid.solve_thermal(gf, gg, self.T_dust, ne, nhi, nh2, nco, ncp, no, nc, self.T_gas)
 
     def diagnostics(self):
         self.Gamma, self.Lambda = grid.heating_cooling(
@@ -334,9 +375,10 @@ class PDRSolverGPU:
     def _before_chemistry(self, first):
         """Radiation, shielding and dust of an inner iteration; returns the
         previous T, x_H2 and x_CO."""
-        old_T = self.T_gas.clone()
-        old_H2 = self.x_H2.clone()
-        old_CO = self.x_CO.clone()
+        grid.loop_snapshot(self.T_gas, self.y, self._T_old, self._y_old)
+        old_T = self._T_old
+        old_H2 = self._y_old[self.sidx['H2']].view(self.shape)
+        old_CO = self._y_old[self.sidx['CO']].view(self.shape)
         self._phase('RT', self.rt_step)
         if first and (self.chemistry == 'euler' or self.accelerator is not None):
             self._set_analytical_ic()
@@ -346,34 +388,20 @@ class PDRSolverGPU:
 
     def _after_chemistry(self, iteration, w_damp, old_T, old_H2, old_CO):
         """Damping and the thermal balance after the chemistry of an inner
-        iteration; returns the previous state and the new damping weights."""
-        if iteration > 0 and self.accelerator is not None:
-            # the surrogate damps only oscillating cells (solver_3d.py:330-345)
-            osc = w_damp > 0.55
-            w = w_damp
-            x_h2 = torch.where(osc, w * old_H2 + (1 - w) * self.x_H2, self.x_H2)
-            self._set_row('x_H2', x_h2)
-            self._set_row('x_HI', torch.where(osc, torch.clamp(1.0 - 2.0 * x_h2, min=1e-30),
-                                              self.x_HI))
-            x_co = torch.where(osc, w * old_CO + (1 - w) * self.x_CO, self.x_CO)
-            self._set_row('x_CO', x_co)
-            self._set_row('x_Cp', torch.where(
-                osc, torch.clamp(self.x_c_total - x_co - self.x_C, min=1e-30), self.x_Cp))
-        elif iteration > 0 and self.chemistry == 'euler':
-            # Euler mode damps the abundances 50/50 (solver_3d.py:345-354)
-            self._set_row('x_H2', 0.5 * old_H2 + 0.5 * self.x_H2)
-            self._set_row('x_HI', torch.clamp(1.0 - 2.0 * self.x_H2, min=1e-30))
-            self._set_row('x_CO', 0.5 * old_CO + 0.5 * self.x_CO)
-            self._set_row('x_Cp', torch.clamp(self.x_c_total - self.x_CO - self.x_C, min=1e-30))
+        iteration (solver_3d.py:324-375); returns the previous state and the
+        damping weights (updated in place)."""
+        if iteration > 0:
+            # the surrogate damps only oscillating cells (solver_3d.py:330-345),
+            # the other chemistries every cell (solver_3d.py:345-354)
+            mode = 'oscillating' if self.accelerator is not None else self.abundance_damping
+            if mode != 'none':
+                grid.damp_abundances(self.y, self._y_old, mode, self._damp_rows,
+                                     self.x_c_total, w_damp)
         if self.fixed_T is None:
             self._phase('thermal', self.thermal_step)
             self.diagnostics()
             if iteration > 0:
-                rel = torch.abs(self.T_gas - old_T) / torch.clamp(old_T, min=10.0)
-                w_damp = torch.clamp(torch.where(rel > 0.5, w_damp + 0.15,
-                                                 torch.where(rel < 0.1, w_damp - 0.02, w_damp)),
-                                     0.5, 0.98)
-                self.T_gas = w_damp * old_T + (1.0 - w_damp) * self.T_gas
+                grid.damp_temperature(old_T, self.T_gas, w_damp)
         return old_T, old_H2, old_CO, w_damp
 
     def iterate(self, iteration, w_damp, first=False, convergence_tol=0.05):
@@ -383,52 +411,45 @@ class PDRSolverGPU:
         self._resolve_timing()
         return rec, w_damp
 
-    def _capture(self, w_damp):
-        """Records one (non-first) inner iteration as a HIP graph.  The state
-        carried between iterations -- the abundances and the grain state
-        (updated in place), T_gas and the damping weights -- lives in fixed
-        buffers, so every replay advances the solver by one iteration; the
-        returned previous-state tensors feed the host-side convergence test
-        after each replay."""
+    def _capture(self, w_damp, convergence_tol):
+        """Records one (non-first) inner iteration as a HIP graph, from the
+        state snapshot to the convergence figures.  The state carried between
+        iterations -- the abundances, the grain state, T_gas and the damping
+        weights -- lives in fixed buffers updated in place, so every replay
+        advances the solver by one iteration and leaves its convergence record
+        in the device record buffer; only that record (12 doubles) is read back
+        after a replay.  Every kernel in the graph is one of this library's."""
         if self.record_chem or self.record_inputs is not None:
             raise ValueError("chemistry timing / input recording cannot run inside a graph")
         if self.tuner is not None and self.tuner.tuning:
             raise ValueError("finish autotuning before capturing the iteration")
-        self._T_state = self.T_gas.clone()
         self._w_state = w_damp.clone()
-        self.T_gas = self._T_state
         torch.cuda.synchronize(self.dev)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            old_T, old_H2, old_CO, w_new = self._advance(1, self._w_state, False)
-            self._T_state.copy_(self.T_gas)
-            self._w_state.copy_(w_new)
-        self.T_gas = self._T_state
+            old_T, old_H2, old_CO, _ = self._advance(1, self._w_state, False)
+            self._enqueue_convergence(old_T, old_H2, old_CO, convergence_tol)
+        self._graph_tol = convergence_tol
         self._graph = g
-        self._graph_prev = (old_T, old_H2, old_CO)
         # the graph writes the tensors bound at capture; eager iterations
         # rebind fresh ones, so every replay restores these bindings
         self._graph_bind = {k: v for k, v in self.__dict__.items()
                             if isinstance(v, torch.Tensor)}
 
+    def f_nano_mean(self):
+        """Grid mean of the nano-grain fraction (the history's f_nano_mean)."""
+        return float(grid.mean(self.f_nano, out=self._fn_mean, workspace=self._mean_ws).item())
+
+    def _enqueue_convergence(self, old_T, old_H2, old_CO, convergence_tol):
+        grid.convergence(old_T, self.T_gas, old_H2, self.x_H2, old_CO, self.x_CO,
+                         convergence_tol, self._conv_rec, self._conv_ws)
+
     def _convergence(self, old_T, old_H2, old_CO, convergence_tol=0.05):
         """p95 (and p99, max) of |dT|/T, p99 of |dH2| and |dCO|, and the count
-        of cells whose |dT|/T exceeds the tolerance (solver_3d.py:381-432)."""
-        def pct(x, q):
-            return float(np.percentile(x, q)) if x.size else 0.0
-        mT = old_T > 10
-        rel_T = (torch.abs(self.T_gas - old_T) / old_T)[mT].cpu().numpy()
-        mH = old_H2 > 1e-6
-        rel_H2 = (torch.abs(self.x_H2 - old_H2) / torch.clamp(old_H2, min=0.01))[mH].cpu().numpy()
-        mC = old_CO > 1e-8
-        rel_CO = (torch.abs(self.x_CO - old_CO) / torch.clamp(old_CO, min=1e-6))[mC].cpu().numpy()
-        rec = {'dT': pct(rel_T, 95), 'dT_p99': pct(rel_T, 99), 'dH2': pct(rel_H2, 99),
-               'dCO': pct(rel_CO, 99)}
-        rec['dT_max'] = float(rel_T.max()) if rel_T.size else 0.0
-        rec['dH2_max'] = float(rel_H2.max()) if rel_H2.size else 0.0
-        rec['n_oscillating'] = int(np.sum(rel_T > convergence_tol))
-        rec['delta_max'] = max(rec['dT'], rec['dH2'], rec['dCO'])
-        return rec
+        of cells whose |dT|/T exceeds the tolerance (solver_3d.py:381-432),
+        computed on the device (p3d_convergence)."""
+        self._enqueue_convergence(old_T, old_H2, old_CO, convergence_tol)
+        return grid.convergence_record(self._conv_rec)
 
     def run(self, max_iterations=20, convergence_tol=0.05, dust_steps=1, verbose=False,
             graph=False):
@@ -451,21 +472,22 @@ class PDRSolverGPU:
             w_damp = torch.full(self.shape, 0.5, dtype=torch.float64, device=self.dev)
             for it in range(max_iterations):
                 if graph and it >= 2:
+                    if self._graph is not None and self._graph_tol != convergence_tol:
+                        self._graph = None
                     if self._graph is None:
-                        self._capture(w_damp)
+                        self._capture(w_damp, convergence_tol)
                     elif it == 2:  # a later dust step: load the graph's state buffers
-                        self._T_state.copy_(self.T_gas)
                         self._w_state.copy_(w_damp)
                         self.__dict__.update(self._graph_bind)
                     self._graph.replay()
-                    rec = self._convergence(*self._graph_prev, convergence_tol)
+                    rec = grid.convergence_record(self._conv_rec)
                     w_damp = self._w_state
                 else:
                     rec, w_damp = self.iterate(it, w_damp, first=(dust_step == 0 and it == 0),
                                                convergence_tol=convergence_tol)
                 rec['dust_step'] = dust_step
                 rec['iteration'] = it
-                rec['f_nano_mean'] = float(self.f_nano.mean())
+                rec['f_nano_mean'] = self.f_nano_mean()
                 self.convergence_history.append(rec)
                 if verbose:
                     tm = self.timing

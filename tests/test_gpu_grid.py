@@ -214,10 +214,11 @@ def test_driver_network_mode_small_grid(gpu):
     assert len(rec) == 3
     last = rec[-1]
     ref, _, _ = O.solve_steady_batch(last['y'], last['n_H'], last['T'], last['G0'], last['A_V'],
-                                     last['zeta'], last['fsh_H2'], last['fsh_CO'], t_max=0.0,
-                                     n_checkpoints=2, rtol=1e-9, atol=1e-20, early_stop=False,
-                                     post_mode=-1)
-    got = sv.y[:, sv.record_inputs].cpu().numpy()
+                                     last['zeta'], last['fsh_H2'], last['fsh_CO'],
+                                     t_max=last['t_max'], n_checkpoints=2, rtol=1e-9, atol=1e-20,
+                                     early_stop=False, post_mode=last['post_mode'],
+                                     x_c_total=last['x_c_total'], x_o_total=last['x_o_total'])
+    got = last['y_out']
     m = ref > 1e-14
     assert np.max(np.abs(got[m] - ref[m]) / ref[m]) < 1e-6
 
