@@ -995,7 +995,7 @@ void orc_solve_steady_batch(int64_t n, double *y, const double *nH, const double
       if (i == 0) ta = t_start;
       double yn[NSP];
       memcpy(yn, yc, sizeof(yn));
-      int rc = bdf_integrate(yn, &p, ta, tb, rtol, atol, tb - ta, 100000, &st);
+      int rc = bdf_integrate(yn, &p, ta, tb, rtol, atol, tb - ta, 1000000, &st);
       if (rc != 0) continue; /* upstream: failed span leaves the state as is */
       for (int s = 0; s < NSP; ++s) yn[s] = yn[s] > 1e-30 ? yn[s] : 1e-30;
       if (early_stop) {

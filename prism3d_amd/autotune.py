@@ -14,15 +14,18 @@ so the tuner picks both the cells per workgroup and the LU shape: variants
 row) or in LDS with unrolled elimination steps; variant 4 is the persistent
 one-cell-per-lane kernel (64 cells per wavefront, statically ordered sparse
 LU) with `waves_per_cu` resident wavefronts per CU that hands its long-tail
-cells over once `rescue_frac` of its lanes are idle.
+cells over once `rescue_frac` of its lanes are idle; variant 6 is the
+persistent one-cell-per-wavefront kernel (lane = species, the same sparse LU
+with its rows in registers), `waves_per_cu` capping its resident wavefronts
+(0: as many as fit).
 """
 
 import json
 import math
 import os
 
-DEFAULT_CANDIDATES = ((4, 8, 0.95), (4, 8, 0.98), (4, 8, 0.99), (4, 12, 0.95), (2, 0, 0.0),
-                      (3, 0, 0.0))
+DEFAULT_CANDIDATES = ((6, 0, 0.0), (6, 2, 0.0), (4, 8, 0.95), (4, 8, 0.99), (4, 12, 0.95),
+                      (2, 0, 0.0))
 
 
 def _cache_path():

@@ -54,7 +54,7 @@ GAS_HE = 8.51e-2
 
 
 def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=1e-4,
-               atol=1e-22, stop_rtol=1e-3, post_mode=0, max_steps=100000, x_c_total=GAS_C,
+               atol=1e-22, stop_rtol=1e-3, post_mode=0, max_steps=1000000, x_c_total=GAS_C,
                x_o_total=GAS_O, variant=0, waves_per_cu=0, rescue_frac=0.0, rate_fp32=False):
     """p3d_solve_opts; the defaults are the upstream refine-path settings
     (solver.py:150-223).  variant: 0 auto, 1-3 one cell per 64-lane
@@ -62,7 +62,11 @@ def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=
     one-cell-per-lane kernel; waves_per_cu sizes variant 4's resident grid and
     rescue_frac sets when its long-tail cells move to the workgroup kernel;
     rate_fp32 evaluates the rate coefficients' exp/log in FP32 (the solve
-    itself stays FP64)."""
+    itself stays FP64).  max_steps caps the BDF steps of one span (a span
+    that reaches it fails and leaves the state unchanged, as a failed
+    upstream span does); the upstream integrator has no cap, and 1e6 is far
+    above the ~1e5 steps the slowest bundled-network cells take at rtol 1e-9,
+    atol 1e-24."""
     o = _lib.SolveOpts()
     o.t_max = -1.0 if t_max is None else float(t_max)
     o.t_start = float(t_start)

@@ -24,10 +24,12 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "lane_launch.h"
 #include "p3d_common.h"
+#include "wave_net.h"
 
 namespace p3d {
 
@@ -70,9 +72,12 @@ struct Network {
   // work counter; calls on one handle share it, so they run in stream order
   void *lane_ws = nullptr;
   size_t lane_ws_bytes = 0;
+  // work counter of the persistent one-cell-per-wavefront kernel (variant 6)
+  unsigned long long *queue_ctr = nullptr;
   ~Network() {
     if (blob) (void)hipFree(blob);
     if (lane_ws) (void)hipFree(lane_ws);
+    if (queue_ctr) (void)hipFree(queue_ctr);
   }
 };
 
@@ -107,20 +112,24 @@ struct Lds {
   int ld;
 };
 
-__device__ __forceinline__ Lds carve(double *base, const NetDev &N) {
+// with_m = false: no dense M (linear-algebra forms that keep M in registers);
+// wave_pieces: the 64 piece sums of rhs_pieces.
+__device__ __forceinline__ Lds carve(double *base, const NetDev &N, bool with_m = true,
+                                     bool wave_pieces = false) {
   Lds L;
   L.ld = N.ns | 1;
   L.kco = base;
   L.part = L.kco + N.nr;
   L.ys = L.part + 2 * N.nr;
   L.J = L.ys + kMaxSpecies;
-  L.M = L.J + N.ns * L.ld;
-  L.piv = reinterpret_cast<int *>(L.M + N.ns * L.ld);
+  L.M = with_m ? L.J + N.ns * L.ld : nullptr;
+  L.piv = reinterpret_cast<int *>(L.J + (with_m ? 2 : 1) * N.ns * L.ld);
   L.ru = reinterpret_cast<double *>(L.piv + kMaxSpecies);
-  // piece sums of the balanced RHS, for networks past the register cache
+  // piece sums of the balanced RHS, for networks past the register cache,
+  // and of the register-LU form's RHS pieces (one per lane)
   const bool pieces = N.nr > 64 * kRxPerLaneCap;
-  L.psum = pieces ? L.ru + 36 : nullptr;
-  L.csr_off = reinterpret_cast<int *>(L.ru + 36 + (pieces ? kMaxPieces : 0));
+  L.psum = (pieces || wave_pieces) ? L.ru + 36 : nullptr;
+  L.csr_off = reinterpret_cast<int *>(L.ru + 36 + (pieces ? kMaxPieces : (wave_pieces ? 64 : 0)));
   L.csr = L.csr_off + kMaxSpecies + 1;
   L.red = nullptr;
   L.xs = nullptr;
@@ -148,10 +157,12 @@ __device__ __forceinline__ Lds carve_wide(double *base, const NetDev &N) {
   return L;
 }
 
-static size_t lds_bytes(int ns, int nr, int n_rhs_e) {
+static size_t lds_bytes(int ns, int nr, int n_rhs_e, bool with_m = true,
+                        bool wave_pieces = false) {
   int ld = ns | 1;
   const int pieces = nr > 64 * kRxPerLaneCap ? kMaxPieces : 0;
-  return sizeof(double) * (size_t)(3 * nr + kMaxSpecies + 2 * ns * ld + 36 + pieces) +
+  return sizeof(double) * (size_t)(3 * nr + kMaxSpecies + (with_m ? 2 : 1) * ns * ld + 36 +
+                                   (pieces ? pieces : (wave_pieces ? 64 : 0))) +
          sizeof(int) * (size_t)(2 * kMaxSpecies + 1 + n_rhs_e);
 }
 
@@ -402,6 +413,54 @@ __device__ __forceinline__ double rhs_cached(const NetDev &N, const CellParams &
   }
   __syncthreads();  // part[] is reused by the next evaluation
   return acc;
+}
+
+// RHS pieces of the register-LU form (bundled network, wave_net.h): lane l
+// sums piece l (at most kPieceLen entries of one species' row, in the
+// upstream order, their indices held in two registers), then species s adds
+// its pieces in order.  A row in one piece gets the bits of rhs_cached.
+__constant__ unsigned c_wave_pw0[64] = P3D_WAVE_PIECE_W0;
+__constant__ unsigned c_wave_pw1[64] = P3D_WAVE_PIECE_W1;
+__constant__ int c_wave_rowpc[wave::kNS + 1] = P3D_WAVE_ROW_PIECE;
+
+__device__ __forceinline__ double rhs_pieces(const NetDev &N, const CellParams &p, const Lds &L,
+                                             const RxCache &rc, unsigned pw0, unsigned pw1,
+                                             int lane) {
+  double pm, dpm;
+  pah_minus(L.ys[N.e_idx], p, pm, dpm);
+#pragma unroll
+  for (int q = 0; q < kRxPerLane; ++q) {
+    const int t = rc.t[q];
+    if (t < 0) continue;
+    const double ya = L.ys[rc.a[q]];
+    double rate;
+    if (t == P3D_RX_TWO_BODY) {
+      const double yb = (rc.b[q] == P3D_SP_PAH_MINUS) ? pm : L.ys[rc.b[q]];
+      rate = rc.k[q] * ya * yb * p.nH;
+    } else if (t == P3D_RX_GRAIN) {
+      rate = rc.k[q] * ya * p.nH * 0.5;
+    } else {
+      rate = rc.k[q] * ya;
+    }
+    L.part[2 * (lane + 64 * q)] = rate;
+  }
+  __syncthreads();
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < wave::kPieceLen; ++q) {
+    const unsigned e = ((q < 4 ? pw0 : pw1) >> (8 * (q & 3))) & 0xFFu;
+    if (e != 0xFFu) {
+      const double v = L.part[2 * (e >> 1)];
+      acc = (e & 1u) ? acc + v : acc - v;
+    }
+  }
+  L.psum[lane] = acc;
+  __syncthreads();
+  double f = 0.0;
+  if (lane < N.ns)
+    for (int q = c_wave_rowpc[lane]; q < c_wave_rowpc[lane + 1]; ++q) f += L.psum[q];
+  __syncthreads();  // part[] and psum[] are reused by the next evaluation
+  return f;
 }
 
 // dy/dt of species `lane` (upstream accumulation order).  Requires y published.
@@ -929,6 +988,68 @@ struct LinReg {
   }
 };
 
+// The bundled network's no-pivot LU in registers (wave_net.h): lane =
+// species holds its row of M = I - c J by elimination position; the
+// factorisation broadcasts only the structural entries of each pivot row
+// (readlane), in the minimum-Markowitz order of the one-cell-per-lane code,
+// so no pivot search and no LDS traffic.  Needs the network structure the
+// code was generated for (Network::lane_ok).
+__constant__ unsigned c_wave_lmask[64] = P3D_WAVE_LMASK;
+__constant__ unsigned c_wave_umask[64] = P3D_WAVE_UMASK;
+
+struct WaveBcast {
+  __device__ __forceinline__ double operator()(double v, int src) const { return bcast(v, src); }
+};
+
+struct LinStatic {
+  // The factors are kept in LDS between the factorisation and the solves
+  // (the lane's row, by position, in L.M), so the 31 row registers are live
+  // only inside each, not across the whole step: that keeps the kernel at two
+  // wavefronts per SIMD.
+  static constexpr bool kNeedsM = true;
+  unsigned pw0, pw1;  // this lane's RHS piece (rhs_pieces)
+  __device__ LinStatic() {
+    pw0 = c_wave_pw0[threadIdx.x & 63];
+    pw1 = c_wave_pw1[threadIdx.x & 63];
+  }
+
+  __device__ void load_jac(const NetDev &, const Lds &, int) {}
+
+  __device__ __forceinline__ void factor(const NetDev &N, const Lds &L, double c, int lane) {
+    const bool row = lane < wave::kNS;
+    const unsigned lm = row ? c_wave_lmask[lane] : 0u;
+    const double *Jrow = L.J + (row ? lane : 0) * L.ld;
+    double a[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+      if (q < wave::kNS) {
+        const double jv = row ? Jrow[wave::kPerm[q]] : 0.0;
+        a[q] = (wave::kPerm[q] == lane ? 1.0 : 0.0) - c * jv;
+      } else {
+        a[q] = 0.0;
+      }
+    }
+    wave::lu_factor(a, lm, lane, WaveBcast{});
+    if (row) {
+      double *Mrow = L.M + lane * L.ld;
+#pragma unroll
+      for (int q = 0; q < wave::kNS; ++q) Mrow[q] = a[q];
+    }
+  }
+
+  __device__ __forceinline__ double solve(const NetDev &N, const Lds &L, double b, int lane) {
+    const bool row = lane < wave::kNS;
+    const unsigned lm = row ? c_wave_lmask[lane] : 0u;
+    const unsigned um = row ? c_wave_umask[lane] : 0u;
+    const double *Mrow = L.M + (row ? lane : 0) * L.ld;
+    double a[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) a[q] = (row && q < wave::kNS) ? Mrow[q] : 0.0;
+    if (lane >= N.ns) b = 0.0;
+    return wave::lu_solve(a, lm, um, lane, b, WaveBcast{});
+  }
+};
+
 __device__ __forceinline__ double rms(double v, int ns) { return sqrt(wave_sum(v * v) / ns); }
 
 // Step-size change of the backward differences D[0..order] by `factor`:
@@ -987,7 +1108,9 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
   const G g{L.red};
   const bool cached = G::W == 64 && N.nr <= 64 * kRxPerLane;
   auto rhs = [&]() {
-    if constexpr (G::W == 64)
+    if constexpr (std::is_same<Lin, LinStatic>::value)
+      return rhs_pieces(N, p, L, rc, lin.pw0, lin.pw1, lane);
+    else if constexpr (G::W == 64)
       return cached ? rhs_cached(N, p, L, rc, lane) : rhs_balanced<64>(N, p, L, lane);
     else
       return rhs_balanced<G::W>(N, p, L, lane);
@@ -1242,30 +1365,28 @@ __device__ __forceinline__ CellParams load_params(int64_t c, const double *nH, c
   return p;
 }
 
-template <class Lin, class G = Wave1>
-__global__ void __launch_bounds__(G::W)
-chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
-                  const double *__restrict__ T, const double *__restrict__ G0,
-                  const double *__restrict__ AV, const double *__restrict__ zeta,
-                  const double *__restrict__ fH2, const double *__restrict__ fCO,
-                  p3d_solve_opts o, int32_t *__restrict__ conv_out,
-                  int32_t *__restrict__ stats_out, const int64_t *__restrict__ cell_list,
-                  const unsigned *__restrict__ list_len) {
-  extern __shared__ double smem[];
-  // with a cell list (cells handed over by the lane kernel), workgroup b
-  // solves cell_list[b] if b < *list_len
-  if (cell_list && blockIdx.x >= *list_len) return;
-  const int64_t cell = cell_list ? cell_list[blockIdx.x] : (int64_t)blockIdx.x;
+// The steady-state solve of one cell by one workgroup (solver.py:145-225):
+// log-spaced checkpoint spans of the BDF, the early-stop test, the post-solve
+// corrections, then the outputs.  The RHS stoichiometry lists must be staged
+// in LDS (one-wavefront forms).
+template <class Lin, class G>
+__device__ __forceinline__ void solve_cell(const NetDev &N, const Lds &L, int64_t cell,
+                                           int64_t n, double *__restrict__ y,
+                                           const double *__restrict__ nH,
+                                           const double *__restrict__ T,
+                                           const double *__restrict__ G0,
+                                           const double *__restrict__ AV,
+                                           const double *__restrict__ zeta,
+                                           const double *__restrict__ fH2,
+                                           const double *__restrict__ fCO,
+                                           const p3d_solve_opts &o, int32_t *__restrict__ conv_out,
+                                           int32_t *__restrict__ stats_out) {
   const int lane = threadIdx.x;
-  Lds L;
-  if constexpr (G::W == 64) L = carve(smem, N);
-  else L = carve_wide(smem, N);
   const G g{L.red};
   const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
   const bool act = lane < N.ns;
   double yc = act ? y[(int64_t)lane * n + cell] : 0.0;
   rate_coeffs<G::W>(N, p, L, lane, o.rate_fp32 != 0);
-  if constexpr (G::W == 64) stage_tables(N, L, lane);
   __syncthreads();
   RxCache rc;
   if constexpr (G::W == 64) rc.load(N, L, lane);
@@ -1312,6 +1433,60 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
       stats_out[4 * cell + 2] = st.rhs;
       stats_out[4 * cell + 3] = st.jac;
     }
+  }
+}
+
+template <class Lin, class G = Wave1>
+__global__ void __launch_bounds__(G::W)
+chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
+                  const double *__restrict__ T, const double *__restrict__ G0,
+                  const double *__restrict__ AV, const double *__restrict__ zeta,
+                  const double *__restrict__ fH2, const double *__restrict__ fCO,
+                  p3d_solve_opts o, int32_t *__restrict__ conv_out,
+                  int32_t *__restrict__ stats_out, const int64_t *__restrict__ cell_list,
+                  const unsigned *__restrict__ list_len) {
+  extern __shared__ double smem[];
+  // with a cell list (cells handed over by the lane kernel), workgroup b
+  // solves cell_list[b] if b < *list_len
+  if (cell_list && blockIdx.x >= *list_len) return;
+  const int64_t cell = cell_list ? cell_list[blockIdx.x] : (int64_t)blockIdx.x;
+  Lds L;
+  if constexpr (G::W == 64) L = carve(smem, N);
+  else L = carve_wide(smem, N);
+  if constexpr (G::W == 64) stage_tables(N, L, threadIdx.x);
+  solve_cell<Lin, G>(N, L, cell, n, y, nH, T, G0, AV, zeta, fH2, fCO, o, conv_out, stats_out);
+}
+
+// Persistent form (variant 6): each resident wavefront takes cells from the
+// device work counter one at a time until the batch is exhausted, so cells
+// whose step counts differ by orders of magnitude share the chip without
+// idle workgroups.  One cell per wavefront, lane = species, M in registers.
+__global__ void __launch_bounds__(64) queue_reset(unsigned long long *ctr) {
+  if (threadIdx.x == 0) *ctr = 0ull;
+}
+
+template <class Lin>
+__global__ void __launch_bounds__(64)
+chem_queue_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
+                  const double *__restrict__ T, const double *__restrict__ G0,
+                  const double *__restrict__ AV, const double *__restrict__ zeta,
+                  const double *__restrict__ fH2, const double *__restrict__ fCO,
+                  p3d_solve_opts o, int32_t *__restrict__ conv_out,
+                  int32_t *__restrict__ stats_out, unsigned long long *__restrict__ ctr) {
+  extern __shared__ double smem[];
+  const Lds L = carve(smem, N, Lin::kNeedsM, true);
+  stage_tables(N, L, threadIdx.x);
+  for (;;) {
+    unsigned long long c = 0;
+    if (threadIdx.x == 0) c = atomicAdd(ctr, 1ull);
+    c = __shfl(c, 0, 64);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)c);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(c >> 32));
+    const int64_t cell = (int64_t)(((unsigned long long)hi << 32) | lo);
+    if (cell >= n) break;  // scalar branch: every lane leaves together
+    solve_cell<Lin, Wave1>(N, L, cell, n, y, nH, T, G0, AV, zeta, fH2, fCO, o, conv_out,
+                           stats_out);
+    __syncthreads();  // LDS of this cell is rewritten by the next
   }
 }
 
@@ -1667,6 +1842,40 @@ static int lane_solve(Network *net, int64_t n, double *y, const double *n_h, con
   return launch_status("chem_solve_kernel (lane rescue)");
 }
 
+// Persistent one-cell-per-wavefront solve (variant 6): as many resident
+// wavefronts as the occupancy allows (capped at waves_per_cu per CU when
+// given), each taking cells from the handle's work counter.
+static int queue_solve(Network *net, int64_t n, double *y, const double *n_h,
+                       const double *t_gas, const double *g0, const double *a_v,
+                       const double *zeta, const double *fsh_h2, const double *fsh_co,
+                       const p3d_solve_opts &o, int32_t *converged, int32_t *stats,
+                       void *stream) {
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e, LinStatic::kNeedsM, true);
+  P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
+  if (!net->queue_ctr) {
+    if (hipMalloc(&net->queue_ctr, 256) != hipSuccess) {
+      (void)hipGetLastError();
+      net->queue_ctr = nullptr;
+      set_error("hipMalloc of the work counter failed");
+      return P3D_ENOMEM;
+    }
+  }
+  int dev = 0, cus = 0, per_cu = 0;
+  P3D_CHECK_HIP(hipGetDevice(&dev));
+  P3D_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  P3D_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(chem_queue_kernel<LinStatic>), 64, sh));
+  if (o.waves_per_cu > 0) per_cu = std::min(per_cu, (int)o.waves_per_cu);
+  per_cu = std::max(per_cu, 1);
+  const int64_t waves = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)cus * per_cu));
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, net->queue_ctr);
+  hipLaunchKernelGGL(chem_queue_kernel<LinStatic>, dim3((unsigned)waves), dim3(64), sh, s,
+                     net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o, converged,
+                     stats, net->queue_ctr);
+  return launch_status("chem_queue_kernel");
+}
+
 }  // namespace p3d
 
 using namespace p3d;
@@ -1998,16 +2207,23 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
     for (int k = 0; k < 7; ++k)
       P3D_REQUIRE(net->d.roles[k] >= 0,
                   "post-solve corrections / early stop need the species roles of the network");
-  P3D_REQUIRE(o.variant >= 0 && o.variant <= 5, "variant must be 0 .. 5");
+  P3D_REQUIRE(o.variant >= 0 && o.variant <= 6, "variant must be 0 .. 6");
   P3D_REQUIRE((o.variant != 2 && o.variant != 3) || net->d.ns <= 32,
               "variants 2 and 3 need n_species <= 32");
   P3D_REQUIRE(o.variant != 4 || net->lane_ok,
               "variant 4 (one cell per lane) is generated for the bundled network structure; "
               "this network differs");
+  P3D_REQUIRE(o.variant != 6 || net->lane_ok,
+              "variant 6 (register LU, work queue) is generated for the bundled network "
+              "structure; this network differs");
   P3D_REQUIRE(o.waves_per_cu >= 0 && o.waves_per_cu <= 64, "waves_per_cu must be in [0, 64]");
   P3D_REQUIRE(o.rescue_frac >= 0.0 && o.rescue_frac <= 1.0, "rescue_frac must be in [0, 1]");
   if (o.variant == 4) return lane_solve(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o,
                                         converged, stats, stream);
+  // auto: for the bundled network structure the persistent register-LU form
+  if (o.variant == 6 || (o.variant == 0 && net->lane_ok))
+    return queue_solve(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o, converged,
+                       stats, stream);
   if (net->d.ns > kMaxSpecies) {
     // two wavefronts per cell, lane = species, MFMA panel LU
     P3D_REQUIRE(o.variant == 0 || o.variant == 5,
@@ -2026,17 +2242,6 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
   // auto: registers for small networks, the MFMA trailing update from 64
   // species (where the panel products are 16 x 16 x 4 tiles), else LDS
   if (var == 0) var = net->d.ns <= 32 ? 2 : (net->d.ns >= 64 ? 5 : 1);
-  // auto: the persistent lane kernel once the batch fills its resident lanes
-  // (below that the slowest cell's tail dominates and one cell per
-  // workgroup is faster; measured crossover ~1e5 cells on MI355X)
-  if (o.variant == 0 && net->lane_ok) {
-    int dev = 0, cus = 0;
-    P3D_CHECK_HIP(hipGetDevice(&dev));
-    P3D_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if (n >= (int64_t)cus * 8 * 64)
-      return lane_solve(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o, converged,
-                        stats, stream);
-  }
   if (var == 5)
     hipLaunchKernelGGL(chem_solve_kernel<LinMfma>, dim3((unsigned)n), dim3(64), sh,
                        as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,

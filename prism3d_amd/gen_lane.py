@@ -1,6 +1,6 @@
 """Generate the straight-line network code of the one-cell-per-lane solver.
 
-    python -m prism3d_amd.gen_lane            # writes prism3d_amd/csrc/lane_net.h
+    python -m prism3d_amd.gen_lane   # writes prism3d_amd/csrc/lane_net.h and wave_net.h
 
 The lane solver keeps one cell per GPU lane, so every per-species array of a
 cell must be indexed with compile-time constants (registers cannot be
@@ -31,6 +31,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "csrc", "lane_net.h")
+OUT_WAVE = os.path.join(HERE, "csrc", "wave_net.h")
 
 # reaction types (include/prism3d_amd.h)
 TWO_BODY, PHOTO, PHOTO_FH2, PHOTO_FCO, CR, GRAIN = 0, 1, 2, 3, 4, 5
@@ -392,6 +393,178 @@ def op_counts(net=None):
     return dict(rhs=rhs, jacobian=jac, lu=lu, solve=solve, n_species=ns)
 
 
+def generate_wave(net=None):
+    """Straight-line register code of the same no-pivot LU (minimum-Markowitz
+    order, filled pattern) for the one-cell-per-wavefront form, lane =
+    species: each lane holds its species' row of M in a[0..kNS) indexed by
+    elimination position.  Step k of the factorisation broadcasts (readlane)
+    the pivot lane's reciprocal diagonal and only the structural entries of
+    its U row; lanes whose row has a structural multiplier at k update (a
+    per-lane mask).  The solves broadcast one component per position.  Every
+    entry receives its updates in the order of the one-cell-per-lane code."""
+    from .network import Network
+    net = net or Network()
+    ns = net.n_species
+    if ns > 32:
+        raise ValueError("the register form holds at most 32 species")
+    dep, nets, terms = structure(net)
+    pat = np.zeros((ns, ns), bool)
+    for (i, j) in terms:
+        pat[i, j] = True
+    perm = markowitz(pat, ns)
+    pos = {s_: p_ for p_, s_ in enumerate(perm)}
+    F = fill(pat[np.ix_(perm, perm)], ns)
+    lmask = [0] * ns
+    umask = [0] * ns
+    for s_ in range(ns):
+        i = pos[s_]
+        for k in range(ns):
+            if F[i, k] and k < i:
+                lmask[s_] |= 1 << k
+            if F[i, k] and k > i:
+                umask[s_] |= 1 << k
+    L = []
+    w = L.append
+    w("// Generated by prism3d_amd/gen_lane.py from the network structure of")
+    w("// prism3d_amd/network.py.  Do not edit; rerun the generator.")
+    w("//")
+    w(f"// Register LU of M = I - c J for {ns} species, one cell per wavefront, lane =")
+    w("// species: the no-pivot factorisation in the minimum-Markowitz order of")
+    w("// lane_net.h, right-looking, broadcasting only structural U entries.")
+    w("#pragma once")
+    w("")
+    w("namespace p3d {")
+    w("namespace wave {")
+    w("")
+    w(f"constexpr int kNS = {ns};")
+    w("constexpr int kPerm[kNS] = {" + ", ".join(str(int(x)) for x in perm) + "};")
+    w("// per species (lane): bit k set where its row has a structural multiplier")
+    w("// (kLMask) or upper entry (kUMask) at elimination position k")
+    w("#define P3D_WAVE_LMASK {" + ", ".join(f"0x{m:08x}u" for m in lmask) + "}")
+    w("#define P3D_WAVE_UMASK {" + ", ".join(f"0x{m:08x}u" for m in umask) + "}")
+    w("")
+    # dependency levels.  Forward solve: component k is final once every
+    # earlier component its row multiplies is (row dependencies).  Factor:
+    # step k needs its pivot row final (row dependencies) and its column of
+    # multipliers final, i.e. every earlier step k' whose pivot row has an
+    # upper entry in column k (column dependencies); steps of one level touch
+    # disjoint data, so their broadcasts are issued together.
+    lev_r = [0] * ns
+    for k in range(ns):
+        lev_r[k] = 1 + max([lev_r[q] for q in range(k) if F[k, q]], default=0)
+    lev_f = [0] * ns
+    for k in range(ns):
+        lev_f[k] = 1 + max([lev_f[q] for q in range(k) if F[k, q] or F[q, k]], default=0)
+    lev_b = [0] * ns
+    for k in range(ns - 1, -1, -1):
+        lev_b[k] = 1 + max([lev_b[q] for q in range(k + 1, ns) if F[k, q]], default=0)
+    groups = lambda lev: [[k for k in range(ns) if lev[k] == v] for v in range(1, max(lev) + 1)]
+    has_l = [any((lmask[s_] >> k) & 1 for s_ in range(ns)) for k in range(ns)]
+    has_u = [any((umask[s_] >> k) & 1 for s_ in range(ns)) for k in range(ns)]
+    # RHS pieces: every species row (reaction order, reactant occurrences then
+    # product occurrences, as the library's CSR lists) cut into pieces of at
+    # most K entries, K the smallest that gives <= 64 pieces; lane l sums piece l
+    rt, re, pr = net.rtype, net.reactants, net.products
+    rows = [[] for _ in range(ns)]
+    for r in range(net.n_reactions):
+        slots = 2 if rt[r] in (TWO_BODY, GRAIN) else 1
+        for k in range(slots):
+            if re[2 * r + k] >= 0:
+                rows[re[2 * r + k]].append(r << 1)
+        for k in range(3):
+            if pr[3 * r + k] >= 0:
+                rows[pr[3 * r + k]].append((r << 1) | 1)
+    if max(e for row in rows for e in row) >= 255:
+        raise ValueError("RHS pieces hold reaction indices below 127")
+    K = next(k for k in range(1, 9) if sum(-(-len(x) // k) for x in rows) <= 64)
+    pieces, row_pc = [], [0]
+    for row in rows:
+        for c0 in range(0, len(row), K):
+            pieces.append(row[c0:c0 + K])
+        row_pc.append(len(pieces))
+    words = []
+    for pc in pieces + [[]] * (64 - len(pieces)):
+        b = pc + [0xFF] * (8 - len(pc))
+        words.append((b[0] | b[1] << 8 | b[2] << 16 | b[3] << 24, b[4] | b[5] << 8 | b[6] << 16 | b[7] << 24))
+    w(f"// RHS pieces: lane l sums piece l, at most kPieceLen entries of one species'")
+    w("// row (byte q of the two words: reaction << 1 | product, 0xff unused); the")
+    w("// pieces of species s are lanes kRowPiece[s] .. kRowPiece[s + 1] - 1")
+    w(f"constexpr int kPieceLen = {K};")
+    w("#define P3D_WAVE_PIECE_W0 {" + ", ".join(f"0x{a:08x}u" for a, _ in words) + "}")
+    w("#define P3D_WAVE_PIECE_W1 {" + ", ".join(f"0x{b_:08x}u" for _, b_ in words) + "}")
+    w("#define P3D_WAVE_ROW_PIECE {" + ", ".join(str(x) for x in row_pc) + "}")
+    w("")
+    w("// a: the lane's row of M (positions); lm: the lane's kLMask entry.")
+    w("// Afterwards a[k] holds the multiplier l_ik below the diagonal, 1 / u_ii")
+    w("// on it and u_ij above it (structural entries).  The pivot rows of one")
+    w("// dependency level are broadcast together, then applied in position order.")
+    w("template <class BC>")
+    w("__device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane, const BC &bc) {")
+    for g in groups(lev_f):
+        w(f"  {{  // level {lev_f[g[0]]}: {', '.join(net.species[perm[k]] for k in g)}")
+        for k in g:
+            P = perm[k]
+            w(f"    const double inv{k} = 1.0 / bc(a[{k}], {P});")
+            if has_l[k]:
+                for j in range(k + 1, ns):
+                    if F[k, j]:
+                        w(f"    const double u{k}_{j} = bc(a[{j}], {P});")
+        for k in g:
+            P = perm[k]
+            if has_l[k]:
+                w(f"    if ((lm >> {k}) & 1u) {{")
+                w(f"      const double m = a[{k}] * inv{k};")
+                for j in range(k + 1, ns):
+                    if F[k, j]:
+                        w(f"      a[{j}] -= m * u{k}_{j};")
+                w(f"      a[{k}] = m;")
+                w("    }")
+            w(f"    if (lane == {P}) a[{k}] = inv{k};")
+        w("  }")
+    w("}")
+    w("")
+    w("// Solve M x = b with the factors; b and x: the lane's species component.")
+    w("// Components of one dependency level are broadcast together.")
+    w("template <class BC>")
+    w("__device__ __forceinline__ double lu_solve(const double (&a)[32], unsigned lm, unsigned um, int lane,")
+    w("                                          double b, const BC &bc) {")
+    for g in groups(lev_r):
+        g = [k for k in g if has_l[k]]
+        if not g:
+            continue
+        w("  {")
+        for k in g:
+            w(f"    const double x{k} = bc(b, {perm[k]});")
+        for k in g:
+            w(f"    if ((lm >> {k}) & 1u) b -= a[{k}] * x{k};")
+        w("  }")
+    for g in groups(lev_b):
+        w("  {")
+        for k in sorted(g, reverse=True):
+            w(f"    if (lane == {perm[k]}) b *= a[{k}];")
+        gu = [k for k in sorted(g, reverse=True) if has_u[k]]
+        for k in gu:
+            w(f"    const double x{k} = bc(b, {perm[k]});")
+        for k in gu:
+            w(f"    if ((um >> {k}) & 1u) b -= a[{k}] * x{k};")
+        w("  }")
+    w("  return b;")
+    w("}")
+    w("")
+    w("}  // namespace wave")
+    w("}  // namespace p3d")
+    return "\n".join(L) + "\n"
+
+
+def write_wave(path=OUT_WAVE):
+    text = generate_wave()
+    old = open(path).read() if os.path.exists(path) else None
+    if old != text:
+        with open(path, "w") as f:
+            f.write(text)
+    return path
+
+
 def write(path=OUT):
     text, info = generate()
     old = open(path).read() if os.path.exists(path) else None
@@ -402,5 +575,6 @@ def write(path=OUT):
 
 
 if __name__ == "__main__":
+    write_wave()
     p, info = write()
     print(p, "J entries", info["NJ"], "LU entries", info["NLU"], file=sys.stderr)

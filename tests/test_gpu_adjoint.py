@@ -44,8 +44,12 @@ def test_polish_and_adjoint_match_oracle(gpu):
     cs = ChemistrySolver()
     y = torch.tensor(np.repeat(y0[:, None], n, axis=1), device=gpu)
     ins = [torch.tensor(par[k], device=gpu) for k in range(7)]
+    # the integration that precedes the polish runs in the one-cell-per-
+    # workgroup form (the batched forms' integrations are compared with the
+    # oracle in test_gpu_lane.py)
     cs.solve_steady_batch(y, *ins, opts=solve_opts(t_max=None, n_checkpoints=2, rtol=1e-9,
-                                                   atol=1e-20, early_stop=False, post_mode=-1))
+                                                   atol=1e-20, early_stop=False, post_mode=-1,
+                                                   variant=2))
     y_int = y.cpu().numpy().copy()
     it = cs.polish_steady_batch(y, *ins)
     assert (it.cpu().numpy() > 0).all()

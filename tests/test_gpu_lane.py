@@ -1,8 +1,10 @@
-"""GPU parity of the persistent one-cell-per-lane chemistry kernel
-(p3d_chem_solve variant 4) through the C ABI: against the oracle BDF and the
-upstream scipy fixture at tight tolerance, against the upstream steady-state
-fixture at its own settings, and against the one-cell-per-workgroup kernel
-on a grid of cells with mixed stiffness."""
+"""GPU parity of the two batched chemistry forms through the C ABI: the
+persistent one-cell-per-lane kernel (p3d_chem_solve variant 4) and the
+persistent one-cell-per-wavefront kernel with the register LU (variant 6,
+the default for the bundled network).  Against the oracle BDF and the
+upstream scipy fixture at tight tolerance, against the upstream
+steady-state fixture at its own settings, and against the oracle on a grid
+of cells with mixed stiffness, beside the one-cell-per-workgroup drop-in."""
 
 import numpy as np
 import pytest
@@ -27,10 +29,11 @@ def _tight(gpu, variant, wpc=0):
     return g, y0, y.cpu().numpy(), stats.cpu().numpy()
 
 
-def test_lane_tight_vs_oracle_and_upstream(gpu):
+@pytest.mark.parametrize("variant", [4, 6])
+def test_lane_tight_vs_oracle_and_upstream(gpu, variant):
     from oracle import oracle as O
-    print("case: lane kernel, tight fixture", flush=True)
-    g, y0, got, st = _tight(gpu, 4)
+    print(f"case: batched form {variant}, tight fixture", flush=True)
+    g, y0, got, st = _tight(gpu, variant)
     t_end = float(g['t_end'])
     orc, _, _ = O.solve_steady_batch(y0, g['n_H'], g['T'], g['G0'], g['A_V'], g['zeta'],
                                      g['fsh_H2'], g['fsh_CO'], t_max=t_end, n_checkpoints=2,
@@ -47,11 +50,12 @@ def test_lane_tight_vs_oracle_and_upstream(gpu):
     assert np.all(st[:, 0] > 0)
 
 
-def test_lane_upstream_mode_vs_golden(gpu):
+@pytest.mark.parametrize("variant", [4, 6])
+def test_lane_upstream_mode_vs_golden(gpu, variant):
     import torch
     from oracle import oracle as O
     from prism3d_amd.chemistry import ChemistrySolver, solve_opts
-    print("case: lane kernel, upstream-mode steady fixture", flush=True)
+    print(f"case: batched form {variant}, upstream-mode steady fixture", flush=True)
     g = golden("steady_golden.npz")
     n = g['n_H'].size
     y0 = np.full((O.NSP, n), 1e-30)
@@ -60,7 +64,7 @@ def test_lane_upstream_mode_vs_golden(gpu):
     cs = ChemistrySolver()
     y = torch.tensor(y0, device=gpu)
     conv, _ = cs.solve_steady_batch(y, g['n_H'], g['T'], g['G0'], g['A_V'], g['zeta'],
-                                    g['fsh_H2'], g['fsh_CO'], opts=solve_opts(variant=4))
+                                    g['fsh_H2'], g['fsh_CO'], opts=solve_opts(variant=variant))
     got = y.cpu().numpy()
     ref = g['x_out'].T
     m = ref > 1e-12
@@ -69,14 +73,24 @@ def test_lane_upstream_mode_vs_golden(gpu):
     assert np.mean(rel < 1e-3) > 0.99
 
 
-def test_lane_matches_workgroup_kernel_on_mixed_grid(gpu):
+@pytest.mark.parametrize("atol", [1e-24, 1e-20])
+def test_forms_match_oracle_on_mixed_grid(gpu, atol):
     """4096 cells spanning n_H 1e2-1e6, G0 1e0-1e5, A_V 0-20, T 10-3000 K,
-    solved at rtol 1e-9 by both kernels: the same steady states (species
-    above 1e-14) within 1e-6, every cell through the work queue once
-    (counters > 0), with waves_per_cu chosen so that lanes refill."""
+    to the upstream steady-state horizon at rtol 1e-9: the drop-in (2) and
+    both batched forms (4, 6) against the oracle at the same tolerances,
+    every abundance above 1e-14 within 1e-6.  (Each form's difference from
+    the oracle, not from another form, is what is bounded.)
+
+    A few hot, irradiated cells are ill-conditioned for this integration:
+    the oracle itself moves by up to 2e-5 there when n_H changes by one ulp
+    (its step sequence changes and the global error of rtol 1e-9 over
+    thousands of steps shows); a different RHS summation order is a larger
+    perturbation of the same kind.  Cells whose 1-ulp spread exceeds 1e-7 are
+    named, their number bounded, and they are held to 1e-4 instead."""
     import torch
+    from oracle import oracle as O
     from prism3d_amd.chemistry import ChemistrySolver, solve_opts
-    print("case: lane vs workgroup kernel, 4096 mixed cells", flush=True)
+    print(f"case: forms vs oracle, 4096 mixed cells, atol {atol:g}", flush=True)
     rng = np.random.RandomState(5)
     n = 4096
     nH = 10 ** rng.uniform(2, 6, n)
@@ -89,21 +103,33 @@ def test_lane_matches_workgroup_kernel_on_mixed_grid(gpu):
     cs = ChemistrySolver()
     y0 = np.stack([cs._dict_to_array(cs._default_initial(G0[i], AV[i])) for i in range(n)],
                   axis=1)
-    outs = []
-    for var, wpc in ((2, 0), (4, 1)):
+    solve = lambda nh: O.solve_steady_batch(y0, nh, T, G0, AV, zeta, fh2, fco, t_max=0.0,
+                                            n_checkpoints=2, rtol=1e-9, atol=atol,
+                                            early_stop=False, post_mode=0)[0]
+    ref = solve(nH)
+    ulp = solve(nH * (1.0 + 2.220446049250313e-16))
+    m = ref > 1e-14
+    spread = np.where(m, np.abs(ulp - ref) / np.where(m, ref, 1.0), 0.0).max(axis=0)
+    sensitive = spread > 1e-7
+    print(f"oracle 1-ulp spread > 1e-7 in cells {np.where(sensitive)[0].tolist()} "
+          f"(max {spread.max():.2e})", flush=True)
+    assert sensitive.sum() <= 16
+    mc = m & ~sensitive[None, :]
+    for var, wpc in ((2, 0), (4, 1), (6, 0)):
         y = torch.tensor(y0, device=gpu)
-        opts = solve_opts(n_checkpoints=2, early_stop=False, rtol=1e-9, atol=1e-20,
+        opts = solve_opts(n_checkpoints=2, early_stop=False, rtol=1e-9, atol=atol,
                           post_mode=0, variant=var, waves_per_cu=wpc)
         _, st = cs.solve_steady_batch(y, nH, T, G0, AV, zeta, fh2, fco, opts=opts)
-        outs.append((y.cpu().numpy(), st.cpu().numpy()))
-    (a, sa), (b, sb) = outs
-    m = a > 1e-14
-    rel = np.abs(b[m] - a[m]) / a[m]
-    print(f"lane vs workgroup: max rel {rel.max():.2e}, p99.9 {np.percentile(rel, 99.9):.2e}; "
-          f"steps {sa[:, 0].mean():.0f} / {sb[:, 0].mean():.0f}")
-    assert np.all(sb[:, 0] > 0)
-    assert np.percentile(rel, 99.9) < 1e-6
-    assert rel.max() < 1e-4
+        got, st = y.cpu().numpy(), st.cpu().numpy()
+        rel = np.abs(got[mc] - ref[mc]) / ref[mc]
+        cell_err = np.where(m, np.abs(got - ref) / np.where(m, ref, 1.0), 0.0).max(axis=0)
+        print(f"variant {var} vs oracle: max rel {rel.max():.2e}, p99.9 "
+              f"{np.percentile(rel, 99.9):.2e}; sensitive cells max "
+              f"{cell_err[sensitive].max() if sensitive.any() else 0.0:.2e}; steps "
+              f"{st[:, 0].mean():.0f}", flush=True)
+        assert np.all(st[:, 0] > 0)
+        assert rel.max() < 1e-6, var
+        assert np.all(cell_err[sensitive] < 1e-4), var
 
 
 def test_driver_autotune_picks_a_config_and_keeps_results(gpu, tmp_path, monkeypatch):
@@ -129,7 +155,7 @@ def test_driver_autotune_picks_a_config_and_keeps_results(gpu, tmp_path, monkeyp
         assert np.max(np.abs(y[m] - x[m]) / np.abs(x[m])) < 1e-5, k
 
 
-@pytest.mark.parametrize("variant", [2, 4])
+@pytest.mark.parametrize("variant", [2, 4, 6])
 def test_fp32_rate_evaluation_error_against_fp64_oracle(gpu, variant):
     """Rate coefficients with FP32 exponentials (rate_fp32), the solve in
     FP64: the steady states of the tight fixture against the FP64 oracle.

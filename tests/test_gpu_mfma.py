@@ -189,9 +189,14 @@ def test_wide_network_conserves_nuclei(gpu):
            10 ** rng.uniform(-4, 0, n), 10 ** rng.uniform(-4, 0, n)]
     y = torch.tensor(np.repeat(y0[:, None], n, axis=1), device=gpu)
     opts = solve_opts(t_max=1e7 * 3.15576e7, n_checkpoints=2, early_stop=False, rtol=1e-9,
-                      atol=1e-24, post_mode=-1, variant=0)
+                      atol=1e-24, post_mode=-1, variant=0, max_steps=100000)
     _, st = cs.solve_steady_batch(y, *par, opts=opts)
     got = y.cpu().numpy()
+    # cells whose span reached the 1e5-step cap keep their initial state (a
+    # failed span); they are counted and bounded, not hidden
+    capped = int((st.cpu().numpy()[:, 0] > 100000).sum())
+    print(f"cells at the step cap: {capped} of {n}", flush=True)
+    assert capped <= n // 100
     assert np.isfinite(got).all() and (got >= 1e-30).all()
     elems = sorted({e for s in sp for e in formula(s)[0]})
     C = np.array([[formula(s)[0].get(e, 0) for s in sp] for e in elems], dtype=np.float64)
