@@ -1070,3 +1070,84 @@ def run_loop(density, box, G0_external, zeta0, nside, iterations, x_c_total=GAS_
                          f_nano_mean=float(np.mean(st['f_nano']))))
         st.update(T_dust=Td, Gamma_PE=gpe, R_H2=RH2, fsh_H2=fH2, fsh_CO=fCO, zeta_CR=zeta)
     return st, hist
+
+
+def run_loop_network(density, box, G0_external, zeta0, nside, iterations, y_init, T_init=50.0,
+                     x_c_total=GAS_C, x_o_total=GAS_O, t_chem=1e7 * YR, rtol=1e-9, atol=1e-20,
+                     post_mode=0, damping='reference', tol=0.05):
+    """The fixed-point loop of solver_3d.py:262-442 with the 31-species network
+    chemistry in every iteration: each cell integrated from the initial
+    abundances y_init (NSP,) to t_chem by the oracle BDF with the upstream
+    post-solve corrections (chemistry/solver.py:145-225), then the upstream
+    abundance damping (damping 'reference': x_H2 and x_CO 50/50, x_H and x_C+
+    from their closures, solver_3d.py:345-354; 'none'), the thermal balance
+    with the adaptive temperature damping (solver_3d.py:357-375) and the
+    percentile convergence figures (solver_3d.py:381-432); it stops as
+    upstream once delta_max < tol at the fourth iteration or later
+    (solver_3d.py:436).  Returns the final state (y (NSP, n) and the grid
+    fields) and the history."""
+    shape = density.shape
+    n = density.size
+    y0 = np.asarray(y_init, dtype=np.float64).reshape(NSP)
+    y = np.repeat(y0[:, None], n, axis=1)
+    T = np.full(shape, float(T_init))
+    f_nano, E_g = np.full(shape, 1.0), np.full(shape, 0.1)
+    row = lambda s_: y[IDX[s_]].reshape(shape)
+    pe = pe_table()
+    cell = box / shape[0]
+    flat = lambda a: np.ascontiguousarray(a, dtype=np.float64).ravel()
+    w_damp = np.full(shape, 0.5)
+    hist = []
+    st = {}
+    for it in range(iterations):
+        old_T, old_H2, old_CO = T.copy(), row('H2').copy(), row('CO').copy()
+        G0, AV, NH, NH2, NCO = fuv_field(density, G0_external, cell, row('H2'), row('CO'),
+                                         nside=nside)
+        fH2 = f_shield_H2(NH2, T=np.mean(T))
+        fCO = f_shield_CO(NCO, NH2)
+        zeta = cr_ionization_rate(NH, zeta0)
+        gpe = pe_heating_vec(pe['table'], pe['log_psi_min'], pe['log_psi_step'], pe['n_psi'],
+                             pe['eg_min'], pe['eg_step'], pe['n_eg'], pe['sigma_abs'],
+                             pe['dn_da_da'], pe['is_nano'], G0, T, row('e-'), density, f_nano,
+                             E_g).reshape(shape)
+        Td = dust_temperature(G0, AV, nH=density, T_gas=T)
+        RH2 = h2_formation_rate(T, Td, f_nano, pe['sigma_nano'], pe['sigma_largeC'],
+                                pe['sigma_Sil'])
+        y, _, _ = solve_steady_batch(np.repeat(y0[:, None], n, axis=1), flat(density), flat(T),
+                                     flat(G0), flat(AV), flat(zeta), flat(fH2), flat(fCO),
+                                     t_max=t_chem, n_checkpoints=2, rtol=rtol, atol=atol,
+                                     early_stop=False, post_mode=post_mode,
+                                     x_c_total=x_c_total, x_o_total=x_o_total)
+        if it > 0 and damping == 'reference':
+            h2 = 0.5 * old_H2 + 0.5 * row('H2')
+            y[IDX['H2']] = h2.ravel()
+            y[IDX['H']] = np.maximum(1.0 - 2.0 * h2, 1e-30).ravel()
+            co = 0.5 * old_CO + 0.5 * row('CO')
+            y[IDX['CO']] = co.ravel()
+            y[IDX['C+']] = np.maximum(x_c_total - co - row('C'), 1e-30).ravel()
+        ctx = thermal_context(density, G0, AV, fH2, zeta, Td, gpe, RH2, row('H'), row('H2'),
+                              row('e-'), row('CO'), row('C+'), row('O'), row('C'))
+        Tn = solve_thermal_vec(*ctx, *line_params()).reshape(shape)
+        if it > 0:
+            rel = np.abs(Tn - old_T) / np.maximum(old_T, 10.0)
+            w_damp = np.clip(np.where(rel > 0.5, w_damp + 0.15,
+                                      np.where(rel < 0.1, w_damp - 0.02, w_damp)), 0.5, 0.98)
+            Tn = w_damp * old_T + (1.0 - w_damp) * Tn
+        T = Tn
+        mT, mH, mC = old_T > 10, old_H2 > 1e-6, old_CO > 1e-8
+        relT = np.abs(T[mT] - old_T[mT]) / old_T[mT] if np.any(mT) else np.array([0.0])
+        relH = (np.abs(row('H2')[mH] - old_H2[mH]) / np.maximum(old_H2[mH], 0.01)
+                if np.any(mH) else np.array([0.0]))
+        relC = (np.abs(row('CO')[mC] - old_CO[mC]) / np.maximum(old_CO[mC], 1e-6)
+                if np.any(mC) else np.array([0.0]))
+        rec = dict(iteration=it, dT=float(np.percentile(relT, 95)),
+                   dT_p99=float(np.percentile(relT, 99)), dH2=float(np.percentile(relH, 99)),
+                   dCO=float(np.percentile(relC, 99)), dT_max=float(np.max(relT)),
+                   dH2_max=float(np.max(relH)), n_oscillating=int(np.sum(relT > tol)))
+        rec['delta_max'] = max(rec['dT'], rec['dH2'], rec['dCO'])
+        hist.append(rec)
+        st = dict(y=y, T_gas=T, G0=G0, A_V=AV, N_H=NH, N_H2=NH2, N_CO=NCO, fsh_H2=fH2,
+                  fsh_CO=fCO, zeta_CR=zeta, T_dust=Td, Gamma_PE=gpe, R_H2=RH2)
+        if rec['delta_max'] < tol and it > 2:
+            break
+    return st, hist
