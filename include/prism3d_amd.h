@@ -167,9 +167,9 @@ typedef struct p3d_solve_opts {
                           1 H+/e- correction then exact projection          */
   int32_t max_steps;   /* per span                                          */
   double x_c_total, x_o_total; /* element totals for the conservation step */
-  int32_t variant;     /* kernel variant: 0 auto (4 when the network is the
-                          bundled one and n >= resident lanes, else 2 for
-                          n_species <= 32, 5 for n_species >= 64, else 1),
+  int32_t variant;     /* kernel variant: 0 auto (6 when the network is the
+                          bundled one, else 2 for n_species <= 32, 5 for
+                          n_species >= 64, else 1),
                           1 LU in LDS, 2 LU in registers
                           (n_species <= 32), 3 LU in LDS with unrolled loops
                           (n_species <= 32), 5 LU in LDS with the panel
@@ -177,10 +177,16 @@ typedef struct p3d_solve_opts {
                           workgroup (above 64 species: one cell per two
                           wavefronts, J sparse in LDS); 4 persistent one-cell-per-lane kernel
                           with a device work queue (the bundled network
-                          structure only) */
+                          structure only); 6 persistent one-cell-per-
+                          wavefront kernel with a device work queue, lane =
+                          species, the network's static no-pivot sparse LU
+                          in registers (the bundled network structure
+                          only) */
   int32_t waves_per_cu; /* variant 4: resident wavefronts per compute unit
                           (0: library default 8); the kernel is compiled for
-                          waves_per_cu / 4 wavefronts per SIMD (2..4) */
+                          waves_per_cu / 4 wavefronts per SIMD (2..4);
+                          variant 6: at most this many per CU (0: as many as
+                          fit) */
   double rescue_frac;  /* variant 4: once this fraction of lanes found the work
                           queue empty, cells still integrating are solved by
                           the one-cell-per-workgroup kernel instead (0: library
@@ -188,6 +194,13 @@ typedef struct p3d_solve_opts {
   int32_t rate_fp32;   /* 1: evaluate the exp/log of the rate coefficients in
                           FP32 (once per cell; the integration stays FP64);
                           0: FP64 as upstream (solver.py:100-133)           */
+  int32_t fast;        /* 1: the upstream fast mode instead of the BDF
+                          (chemistry/solver.py:164-183): ten explicit steps
+                          y += dt f(y), dt from 1e12 s limited to a 30%
+                          largest relative change, then doubled up to 1e15 s;
+                          then the post-solve corrections of post_mode (0 as
+                          upstream); one cell per 64-lane workgroup, at most
+                          64 species; converged = 0 as upstream            */
 } p3d_solve_opts;
 
 /* Integrate every cell to its steady state / end time with the

@@ -277,6 +277,26 @@ def solve_steady_batch(y_soa, nH, T, G0, AV, zeta, fH2, fCO, t_max=0.0,
     return y, conv.astype(bool), steps
 
 
+def solve_fast(y0, nH, T, G0, AV, zeta, fH2=1.0, fCO=1.0, x_c_total=GAS_C, x_o_total=GAS_O):
+    """solve_steady_state(fast=True) of chemistry/solver.py:164-183 for one
+    cell: ten explicit steps y <- max(y + f(y) dt, 1e-30), dt starting at
+    1e12 s, cut so the largest relative change |f| dt / max(|y|, 1e-20) is
+    0.3, then doubled up to 1e15 s; then the H+/e- correction and the
+    upstream conservation step."""
+    y = _f64(y0).copy()
+    dt = 1e12
+    for _ in range(10):
+        f = rhs(y, nH, T, G0, AV, zeta, fH2, fCO)
+        max_change = np.max(np.abs(f) * dt / np.maximum(np.abs(y), 1e-20))
+        if max_change > 0.3:
+            dt *= 0.3 / max_change
+        y = np.maximum(y + f * dt, 1e-30)
+        dt *= 2
+        dt = min(dt, 1e15)
+    y = correct_hp_e(y, nH, T, zeta)
+    return enforce_conservation(y, x_c_total, x_o_total, 0)
+
+
 def integrate_batch(y_soa, nH, T, G0, AV, zeta, fH2, fCO, t0, t1, rtol, atol):
     y = np.ascontiguousarray(y_soa, dtype=np.float64).copy()
     n = y.shape[1]

@@ -31,7 +31,8 @@ class SolveOpts(ctypes.Structure):
                 ("early_stop", _i32), ("rtol", _dbl), ("atol", _dbl),
                 ("stop_rtol", _dbl), ("post_mode", _i32), ("max_steps", _i32),
                 ("x_c_total", _dbl), ("x_o_total", _dbl), ("variant", _i32),
-                ("waves_per_cu", _i32), ("rescue_frac", _dbl), ("rate_fp32", _i32)]
+                ("waves_per_cu", _i32), ("rescue_frac", _dbl), ("rate_fp32", _i32),
+                ("fast", _i32)]
 
 
 class SteadyCons(ctypes.Structure):

@@ -55,7 +55,8 @@ GAS_HE = 8.51e-2
 
 def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=1e-4,
                atol=1e-22, stop_rtol=1e-3, post_mode=0, max_steps=1000000, x_c_total=GAS_C,
-               x_o_total=GAS_O, variant=0, waves_per_cu=0, rescue_frac=0.0, rate_fp32=False):
+               x_o_total=GAS_O, variant=0, waves_per_cu=0, rescue_frac=0.0, rate_fp32=False,
+               fast=False):
     """p3d_solve_opts; the defaults are the upstream refine-path settings
     (solver.py:150-223).  variant: 0 auto, 1-3 one cell per 64-lane
     workgroup (LU in LDS / registers / unrolled LDS), 4 persistent
@@ -66,7 +67,8 @@ def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=
     that reaches it fails and leaves the state unchanged, as a failed
     upstream span does); the upstream integrator has no cap, and 1e6 is far
     above the ~1e5 steps the slowest bundled-network cells take at rtol 1e-9,
-    atol 1e-24."""
+    atol 1e-24.  fast: the upstream fast mode (ten explicit steps,
+    chemistry/solver.py:164-183) instead of the BDF."""
     o = _lib.SolveOpts()
     o.t_max = -1.0 if t_max is None else float(t_max)
     o.t_start = float(t_start)
@@ -80,6 +82,7 @@ def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=
     o.waves_per_cu = int(waves_per_cu)
     o.rescue_frac = float(rescue_frac)
     o.rate_fp32 = int(bool(rate_fp32))
+    o.fast = int(bool(fast))
     return o
 
 
@@ -167,16 +170,16 @@ class ChemistrySolver:
         """Steady-state abundances of one cell -> (dict, converged).
 
         ``rtol`` is the checkpoint stop criterion and ``atol`` is unused, as
-        upstream; the BDF itself runs at rtol 1e-4, atol 1e-22."""
-        if fast:
-            raise NotImplementedError("fast=True (the explicit 10-step mode) is not part of "
-                                      "the GPU solve; use fast=False")
+        upstream; the BDF itself runs at rtol 1e-4, atol 1e-22.  ``fast``
+        runs the upstream fast mode (ten explicit steps, then the
+        corrections; converged is False), as chemistry/solver.py:164-183."""
         if x_init is None:
             x_init = self._default_initial(G0, A_V)
         y = torch.tensor(self._dict_to_array(x_init), dtype=torch.float64,
                          device=device()).reshape(-1, 1)
         conv = self.solve_steady_batch(y, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO,
-                                       opts=solve_opts(t_max=t_max, stop_rtol=rtol))[0]
+                                       opts=solve_opts(t_max=t_max, stop_rtol=rtol,
+                                                       fast=fast))[0]
         return self._abundances_to_dict(y[:, 0].cpu().numpy()), bool(conv[0].item())
 
     # --- batched SoA API ----------------------------------------------------

@@ -1490,6 +1490,57 @@ chem_queue_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
   }
 }
 
+// The upstream fast mode (chemistry/solver.py:164-183), one cell per
+// workgroup, lane = species: ten explicit steps y <- max(y + dt f(y), 1e-30)
+// with dt limited so that the largest relative change |f| dt / max(|y|,
+// 1e-20) is 0.3, then doubled (at most 1e15 s), then the post-solve
+// corrections.  Uncontracted, so every step rounds as the numpy expression.
+__global__ void __launch_bounds__(64)
+chem_fast_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
+                 const double *__restrict__ T, const double *__restrict__ G0,
+                 const double *__restrict__ AV, const double *__restrict__ zeta,
+                 const double *__restrict__ fH2, const double *__restrict__ fCO,
+                 p3d_solve_opts o, int32_t *__restrict__ conv_out,
+                 int32_t *__restrict__ stats_out) {
+#pragma clang fp contract(off)
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  const Lds L = carve(smem, N);
+  stage_tables(N, L, lane);
+  const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
+  const bool act = lane < N.ns;
+  double yc = act ? y[(int64_t)lane * n + cell] : 0.0;
+  rate_coeffs<64>(N, p, L, lane, o.rate_fp32 != 0);
+  __syncthreads();
+  RxCache rc;
+  rc.load(N, L, lane);
+  const bool cached = N.nr <= 64 * kRxPerLane;
+  double dt = 1e12;
+  for (int step = 0; step < 10; ++step) {
+    publish_y(N, L, yc, lane);
+    const double f = cached ? rhs_cached(N, p, L, rc, lane) : rhs_balanced<64>(N, p, L, lane);
+    const double mc = act ? fabs(f) * dt / fmax(fabs(yc), 1e-20) : 0.0;
+    const double max_change = wave_max(mc);
+    if (max_change > 0.3) dt *= 0.3 / max_change;
+    if (act) yc = fmax(yc + f * dt, 1e-30);
+    dt *= 2.0;
+    dt = fmin(dt, 1e15);
+  }
+  if (o.post_mode >= 0)
+    post_solve<Wave1>(N, p, lane, yc, o.post_mode, o.x_c_total, o.x_o_total, L.red);
+  if (act) y[(int64_t)lane * n + cell] = yc;
+  if (lane == 0) {
+    if (conv_out) conv_out[cell] = 0;
+    if (stats_out) {
+      stats_out[4 * cell + 0] = 10;
+      stats_out[4 * cell + 1] = 0;
+      stats_out[4 * cell + 2] = 10;
+      stats_out[4 * cell + 3] = 0;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(64)
 chem_post_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
                  const double *__restrict__ T, const double *__restrict__ zeta, int mode,
@@ -2208,6 +2259,16 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
       P3D_REQUIRE(net->d.roles[k] >= 0,
                   "post-solve corrections / early stop need the species roles of the network");
   P3D_REQUIRE(o.variant >= 0 && o.variant <= 6, "variant must be 0 .. 6");
+  P3D_REQUIRE(o.fast == 0 || o.fast == 1, "fast must be 0 or 1");
+  if (o.fast) {
+    P3D_REQUIRE(net->d.ns <= kMaxSpecies, "the fast mode takes at most %d species", kMaxSpecies);
+    const size_t shf = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
+    P3D_REQUIRE(shf <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", shf);
+    hipLaunchKernelGGL(chem_fast_kernel, dim3((unsigned)n), dim3(64), shf, as_stream(stream),
+                       net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o, converged,
+                       stats);
+    return launch_status("chem_fast_kernel");
+  }
   P3D_REQUIRE((o.variant != 2 && o.variant != 3) || net->d.ns <= 32,
               "variants 2 and 3 need n_species <= 32");
   P3D_REQUIRE(o.variant != 4 || net->lane_ok,

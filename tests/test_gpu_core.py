@@ -241,3 +241,39 @@ def test_single_cell_api(gpu):
         ref = s['x_out'][i, k]
         if ref > 1e-14:
             assert abs(res[sp] - ref) / ref < 1e-2, sp
+
+
+def test_fast_mode_matches_oracle_and_upstream(gpu):
+    """ChemistrySolver.solve_steady_state(fast=True) and the batched fast mode
+    (p3d_solve_opts.fast): the upstream ten-step explicit mode
+    (chemistry/solver.py:164-183) on the 64 fixture cells, against the
+    oracle and the upstream outputs (species above 1e-20), and returns
+    converged False as upstream."""
+    import torch
+    from oracle import oracle as O
+    from prism3d_amd.chemistry import ChemistrySolver, solve_opts
+    print("case: fast mode, 64 fixture cells", flush=True)
+    g = golden("fast_golden.npz")
+    cs = ChemistrySolver()
+    n = g['n_H'].size
+    y = torch.tensor(g['x_init'].T.copy(), device=gpu)
+    conv, st = cs.solve_steady_batch(y, g['n_H'], g['T'], g['G0'], g['A_V'], g['zeta'],
+                                     g['fsh_H2'], g['fsh_CO'], opts=solve_opts(fast=True))
+    got = y.cpu().numpy()
+    assert not conv.cpu().numpy().any() and (st.cpu().numpy()[:, 0] == 10).all()
+    ref = g['x_out'].T
+    orc = np.stack([O.solve_fast(g['x_init'][i], g['n_H'][i], g['T'][i], g['G0'][i],
+                                 g['A_V'][i], g['zeta'][i], g['fsh_H2'][i], g['fsh_CO'][i])
+                    for i in range(n)], axis=1)
+    m = ref > 1e-20
+    e_up = np.max(np.abs(got[m] - ref[m]) / ref[m])
+    e_or = np.max(np.abs(got[m] - orc[m]) / orc[m])
+    print(f"fast mode: vs oracle {e_or:.2e}, vs upstream {e_up:.2e}", flush=True)
+    assert e_or < 1e-10 and e_up < 1e-10
+    d, c = cs.solve_steady_state(g['n_H'][0], g['T'][0], g['G0'][0], g['A_V'][0], g['zeta'][0],
+                                 x_init=dict(zip(g['species'], g['x_init'][0])),
+                                 f_shield_H2=g['fsh_H2'][0], f_shield_CO=g['fsh_CO'][0],
+                                 fast=True)
+    assert c is False
+    one = np.array([d[s] for s in g['species']])
+    assert np.max(np.abs(one[m[:, 0]] - ref[m[:, 0], 0]) / ref[m[:, 0], 0]) < 1e-10

@@ -604,3 +604,19 @@ def test_steady_adjoint_against_finite_differences_and_forward_sensitivities(cel
             Jm = J(*_forward(y0, par, dpar={ipar: np.exp(-e)}))
             fd = (Jp - Jm) / (2 * e)
         assert abs(fd - ga[j]) <= 1e-4 * abs(fd) + 1e-12 * abs(J0) / h, (j, fd, ga[j])
+
+
+def test_fast_mode_matches_upstream():
+    """solve_steady_state(fast=True) (chemistry/solver.py:164-183): the
+    oracle's ten explicit steps and corrections against the upstream outputs
+    of 64 cells (fast_golden.npz) for every species above 1e-20."""
+    g = golden("fast_golden.npz")
+    assert list(g['species']) == O.SPECIES
+    worst = 0.0
+    for i in range(g['n_H'].size):
+        y = O.solve_fast(g['x_init'][i], g['n_H'][i], g['T'][i], g['G0'][i], g['A_V'][i],
+                         g['zeta'][i], g['fsh_H2'][i], g['fsh_CO'][i])
+        ref = g['x_out'][i]
+        m = ref > 1e-20
+        worst = max(worst, float(np.max(np.abs(y[m] - ref[m]) / ref[m])))
+    assert worst < 1e-12
