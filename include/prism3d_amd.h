@@ -410,6 +410,21 @@ int p3d_convergence(int64_t n, const double *t_old, const double *t_gas,
                     const double *co_old, const double *x_co, double tol,
                     double *record, void *workspace, void *stream);
 
+/* p3d_convergence without a caller workspace and without any allocation.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The
+ * workspace is a file-scope device array of P3D_CONV_WORKSPACE_BYTES (16 KiB)
+ * per GPU, held until the process ends; its device address is looked up on
+ * the first call on each device (make that first call outside a graph
+ * capture).  The figures equal p3d_convergence's bit for bit; n up to 2^40
+ * (larger n is rejected).  Every call on one GPU shares that GPU's array and
+ * clears it first, so calls that overlap on one GPU (two streams, two
+ * threads) race and must be serialised by the caller. */
+int p3d_convergence_noalloc(int64_t n, const double *t_old, const double *t_gas,
+                            const double *h2_old, const double *x_h2,
+                            const double *co_old, const double *x_co,
+                            double tol, double *record, void *stream);
+
 /* ---------------------------------------------------------------------- */
 /* Tree-ensemble chemistry surrogate                                         */
 /* ---------------------------------------------------------------------- */

@@ -82,6 +82,7 @@ SIGNATURES = {
     "p3d_damp_abundances": (_i32, [_i64, _i32, _i32, _VP, _VP, _VP, _I32P, _dbl, _VP]),
     "p3d_damp_temperature": (_i32, [_i64, _VP, _VP, _VP, _VP]),
     "p3d_convergence": (_i32, [_i64] + [_VP] * 6 + [_dbl, _VP, _VP, _VP]),
+    "p3d_convergence_noalloc": (_i32, [_i64] + [_VP] * 6 + [_dbl, _VP, _VP]),
 }
 
 _lib = None

@@ -13,6 +13,8 @@
 //
 // Elementwise arithmetic is uncontracted, so it rounds as the reference's
 // numpy expressions do.
+#include <mutex>
+
 #include "p3d_common.h"
 
 namespace p3d {
@@ -367,6 +369,38 @@ damp_temperature_kernel(int64_t n, const double *__restrict__ t_old, double *__r
 }
 
 }  // namespace
+
+// Workspace of p3d_convergence_noalloc: one copy per GPU, held until the
+// process ends.  Host code never passes this name to a kernel (on the host it
+// is only a stand-in for the symbol); it uses the device address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[P3D_CONV_WORKSPACE_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch (looked up once per device).
+static int scratch_address(void **out) {
+  int dev = 0;
+  P3D_CHECK_HIP(hipGetDevice(&dev));
+  P3D_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the supported %d", dev,
+              kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress of the convergence scratch failed: %s",
+                hipGetErrorString(e));
+      return P3D_EHIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return P3D_OK;
+}
+
 }  // namespace p3d
 
 using namespace p3d;
@@ -441,6 +475,15 @@ int p3d_convergence(int64_t n, const double *t_old, const double *t_gas, const d
   }
   hipLaunchKernelGGL(conv_final, dim3(1), dim3(64), 0, s, tol, ws, record);
   return launch_status("convergence kernels");
+}
+
+int p3d_convergence_noalloc(int64_t n, const double *t_old, const double *t_gas,
+                            const double *h2_old, const double *x_h2, const double *co_old,
+                            const double *x_co, double tol, double *record, void *stream) {
+  void *ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc != P3D_OK) return rc;
+  return p3d_convergence(n, t_old, t_gas, h2_old, x_h2, co_old, x_co, tol, record, ws, stream);
 }
 
 }  // extern "C"
