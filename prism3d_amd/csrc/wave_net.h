@@ -460,11 +460,24 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
   }
 }
 
+// per lane: elimination position (31: no row) and the forward | backward << 8
+// dependency levels of the solves
+#define P3D_WAVE_POS {24, 25, 26, 13, 7, 0, 20, 22, 27, 28, 19, 17, 11, 29, 10, 23, 16, 21, 14, 12, 15, 30, 18, 1, 2, 3, 4, 5, 6, 8, 9, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31}
+#define P3D_WAVE_LEVELS {1798, 1543, 1288, 2562, 2049, 1537, 2050, 2308, 1033, 778, 2562, 2561, 2817, 523, 2817, 2053, 2305, 2563, 2561, 2817, 2561, 268, 2817, 2305, 2050, 2305, 2050, 2305, 2050, 2305, 2050, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}
+
 // Solve M x = b with the factors; b and x: the lane's species component.
-// Components of one dependency level are broadcast together.
+// Level by level: the components of one level are broadcast, each lane of
+// the level records its component in xs[pos], then every lane subtracts
+// a[k] x_k with no mask.  A lane reached by a component it has no entry
+// for in this triangle (an entry of the other triangle, or its own
+// diagonal) is always final by then, so only its register is spoilt and
+// the recorded xs stays exact; the few components for which that is not
+// so keep their mask.  dinv: the lane's 1 / u_ii.
 template <class BC>
 __device__ __forceinline__ double lu_solve(const double (&a)[32], unsigned lm, unsigned um, int lane,
-                                          double b, const BC &bc) {
+                                          double b, const BC &bc, double *xs, int pos,
+                                          int lev, double dinv) {
+  const int lvf = lev & 0xff, lvb = lev >> 8;
   {
     const double x0 = bc(b, 5);
     const double x1 = bc(b, 23);
@@ -480,20 +493,21 @@ __device__ __forceinline__ double lu_solve(const double (&a)[32], unsigned lm, u
     const double x16 = bc(b, 16);
     const double x17 = bc(b, 11);
     const double x18 = bc(b, 22);
-    if ((lm >> 0) & 1u) b -= a[0] * x0;
-    if ((lm >> 1) & 1u) b -= a[1] * x1;
-    if ((lm >> 3) & 1u) b -= a[3] * x3;
-    if ((lm >> 5) & 1u) b -= a[5] * x5;
-    if ((lm >> 7) & 1u) b -= a[7] * x7;
-    if ((lm >> 8) & 1u) b -= a[8] * x8;
-    if ((lm >> 10) & 1u) b -= a[10] * x10;
-    if ((lm >> 11) & 1u) b -= a[11] * x11;
-    if ((lm >> 12) & 1u) b -= a[12] * x12;
-    if ((lm >> 14) & 1u) b -= a[14] * x14;
-    if ((lm >> 15) & 1u) b -= a[15] * x15;
-    if ((lm >> 16) & 1u) b -= a[16] * x16;
-    if ((lm >> 17) & 1u) b -= a[17] * x17;
-    if ((lm >> 18) & 1u) b -= a[18] * x18;
+    if (lvf == 1) xs[pos] = b;
+    b -= a[0] * x0;
+    b -= a[1] * x1;
+    b -= a[3] * x3;
+    b -= a[5] * x5;
+    b -= a[7] * x7;
+    b -= a[8] * x8;
+    b -= a[10] * x10;
+    b -= a[11] * x11;
+    b -= a[12] * x12;
+    b -= a[14] * x14;
+    b -= a[15] * x15;
+    b -= a[16] * x16;
+    b -= a[17] * x17;
+    b -= a[18] * x18;
   }
   {
     const double x2 = bc(b, 24);
@@ -503,144 +517,149 @@ __device__ __forceinline__ double lu_solve(const double (&a)[32], unsigned lm, u
     const double x13 = bc(b, 3);
     const double x19 = bc(b, 10);
     const double x20 = bc(b, 6);
-    if ((lm >> 2) & 1u) b -= a[2] * x2;
-    if ((lm >> 4) & 1u) b -= a[4] * x4;
-    if ((lm >> 6) & 1u) b -= a[6] * x6;
-    if ((lm >> 9) & 1u) b -= a[9] * x9;
-    if ((lm >> 13) & 1u) b -= a[13] * x13;
-    if ((lm >> 19) & 1u) b -= a[19] * x19;
-    if ((lm >> 20) & 1u) b -= a[20] * x20;
+    if (lvf == 2) xs[pos] = b;
+    b -= a[2] * x2;
+    b -= a[4] * x4;
+    b -= a[6] * x6;
+    b -= a[9] * x9;
+    b -= a[13] * x13;
+    b -= a[19] * x19;
+    b -= a[20] * x20;
   }
   {
     const double x21 = bc(b, 17);
-    if ((lm >> 21) & 1u) b -= a[21] * x21;
+    if (lvf == 3) xs[pos] = b;
+    b -= a[21] * x21;
   }
   {
     const double x22 = bc(b, 7);
-    if ((lm >> 22) & 1u) b -= a[22] * x22;
+    if (lvf == 4) xs[pos] = b;
+    b -= a[22] * x22;
   }
   {
     const double x23 = bc(b, 15);
-    if ((lm >> 23) & 1u) b -= a[23] * x23;
+    if (lvf == 5) xs[pos] = b;
+    b -= a[23] * x23;
   }
   {
     const double x24 = bc(b, 0);
-    if ((lm >> 24) & 1u) b -= a[24] * x24;
+    if (lvf == 6) xs[pos] = b;
+    b -= a[24] * x24;
   }
   {
     const double x25 = bc(b, 1);
-    if ((lm >> 25) & 1u) b -= a[25] * x25;
+    if (lvf == 7) xs[pos] = b;
+    b -= a[25] * x25;
   }
   {
     const double x26 = bc(b, 2);
-    if ((lm >> 26) & 1u) b -= a[26] * x26;
+    if (lvf == 8) xs[pos] = b;
+    b -= a[26] * x26;
   }
   {
     const double x27 = bc(b, 8);
-    if ((lm >> 27) & 1u) b -= a[27] * x27;
+    if (lvf == 9) xs[pos] = b;
+    b -= a[27] * x27;
   }
   {
     const double x28 = bc(b, 9);
-    if ((lm >> 28) & 1u) b -= a[28] * x28;
+    if (lvf == 10) xs[pos] = b;
+    b -= a[28] * x28;
   }
   {
     const double x29 = bc(b, 13);
-    if ((lm >> 29) & 1u) b -= a[29] * x29;
+    if (lvf == 11) xs[pos] = b;
+    b -= a[29] * x29;
   }
   {
-    if (lane == 21) b *= a[30];
-    const double x30 = bc(b, 21);
-    if ((um >> 30) & 1u) b -= a[30] * x30;
+    if (lvf == 12) xs[pos] = b;
+  }
+  b = xs[pos];
+  {
+    const double bx = b * dinv;
+    if (lvb == 1) xs[pos] = bx;
+    const double x30 = bc(bx, 21);
+    b -= a[30] * x30;
   }
   {
-    if (lane == 13) b *= a[29];
-    const double x29 = bc(b, 13);
-    if ((um >> 29) & 1u) b -= a[29] * x29;
+    const double bx = b * dinv;
+    if (lvb == 2) xs[pos] = bx;
+    const double x29 = bc(bx, 13);
+    b -= a[29] * x29;
   }
   {
-    if (lane == 9) b *= a[28];
-    const double x28 = bc(b, 9);
-    if ((um >> 28) & 1u) b -= a[28] * x28;
+    const double bx = b * dinv;
+    if (lvb == 3) xs[pos] = bx;
+    const double x28 = bc(bx, 9);
+    b -= a[28] * x28;
   }
   {
-    if (lane == 8) b *= a[27];
-    const double x27 = bc(b, 8);
-    if ((um >> 27) & 1u) b -= a[27] * x27;
+    const double bx = b * dinv;
+    if (lvb == 4) xs[pos] = bx;
+    const double x27 = bc(bx, 8);
+    b -= a[27] * x27;
   }
   {
-    if (lane == 2) b *= a[26];
-    const double x26 = bc(b, 2);
-    if ((um >> 26) & 1u) b -= a[26] * x26;
+    const double bx = b * dinv;
+    if (lvb == 5) xs[pos] = bx;
+    const double x26 = bc(bx, 2);
+    b -= a[26] * x26;
   }
   {
-    if (lane == 1) b *= a[25];
-    if (lane == 5) b *= a[0];
-    const double x25 = bc(b, 1);
-    if ((um >> 25) & 1u) b -= a[25] * x25;
+    const double bx = b * dinv;
+    if (lvb == 6) xs[pos] = bx;
+    const double x25 = bc(bx, 1);
+    b -= a[25] * x25;
   }
   {
-    if (lane == 0) b *= a[24];
-    const double x24 = bc(b, 0);
-    if ((um >> 24) & 1u) b -= a[24] * x24;
+    const double bx = b * dinv;
+    if (lvb == 7) xs[pos] = bx;
+    const double x24 = bc(bx, 0);
+    b -= a[24] * x24;
   }
   {
-    if (lane == 15) b *= a[23];
-    if (lane == 6) b *= a[20];
-    if (lane == 30) b *= a[9];
-    if (lane == 4) b *= a[7];
-    if (lane == 28) b *= a[6];
-    if (lane == 26) b *= a[4];
-    if (lane == 24) b *= a[2];
-    const double x23 = bc(b, 15);
-    const double x20 = bc(b, 6);
-    const double x9 = bc(b, 30);
-    const double x6 = bc(b, 28);
-    const double x4 = bc(b, 26);
-    const double x2 = bc(b, 24);
-    if ((um >> 23) & 1u) b -= a[23] * x23;
+    const double bx = b * dinv;
+    if (lvb == 8) xs[pos] = bx;
+    const double x23 = bc(bx, 15);
+    const double x20 = bc(bx, 6);
+    const double x9 = bc(bx, 30);
+    const double x6 = bc(bx, 28);
+    const double x4 = bc(bx, 26);
+    const double x2 = bc(bx, 24);
+    b -= a[23] * x23;
     if ((um >> 20) & 1u) b -= a[20] * x20;
     if ((um >> 9) & 1u) b -= a[9] * x9;
-    if ((um >> 6) & 1u) b -= a[6] * x6;
-    if ((um >> 4) & 1u) b -= a[4] * x4;
-    if ((um >> 2) & 1u) b -= a[2] * x2;
+    b -= a[6] * x6;
+    b -= a[4] * x4;
+    b -= a[2] * x2;
   }
   {
-    if (lane == 7) b *= a[22];
-    if (lane == 16) b *= a[16];
-    if (lane == 29) b *= a[8];
-    if (lane == 27) b *= a[5];
-    if (lane == 25) b *= a[3];
-    if (lane == 23) b *= a[1];
-    const double x22 = bc(b, 7);
-    const double x16 = bc(b, 16);
-    if ((um >> 22) & 1u) b -= a[22] * x22;
+    const double bx = b * dinv;
+    if (lvb == 9) xs[pos] = bx;
+    const double x22 = bc(bx, 7);
+    const double x16 = bc(bx, 16);
+    b -= a[22] * x22;
     if ((um >> 16) & 1u) b -= a[16] * x16;
   }
   {
-    if (lane == 17) b *= a[21];
-    if (lane == 10) b *= a[19];
-    if (lane == 11) b *= a[17];
-    if (lane == 20) b *= a[15];
-    if (lane == 18) b *= a[14];
-    if (lane == 3) b *= a[13];
-    const double x21 = bc(b, 17);
-    const double x19 = bc(b, 10);
-    const double x17 = bc(b, 11);
-    const double x14 = bc(b, 18);
-    const double x13 = bc(b, 3);
-    if ((um >> 21) & 1u) b -= a[21] * x21;
-    if ((um >> 19) & 1u) b -= a[19] * x19;
-    if ((um >> 17) & 1u) b -= a[17] * x17;
-    if ((um >> 14) & 1u) b -= a[14] * x14;
-    if ((um >> 13) & 1u) b -= a[13] * x13;
+    const double bx = b * dinv;
+    if (lvb == 10) xs[pos] = bx;
+    const double x21 = bc(bx, 17);
+    const double x19 = bc(bx, 10);
+    const double x17 = bc(bx, 11);
+    const double x14 = bc(bx, 18);
+    const double x13 = bc(bx, 3);
+    b -= a[21] * x21;
+    b -= a[19] * x19;
+    b -= a[17] * x17;
+    b -= a[14] * x14;
+    b -= a[13] * x13;
   }
   {
-    if (lane == 22) b *= a[18];
-    if (lane == 19) b *= a[12];
-    if (lane == 12) b *= a[11];
-    if (lane == 14) b *= a[10];
+    const double bx = b * dinv;
+    if (lvb == 11) xs[pos] = bx;
   }
-  return b;
+  return xs[pos];
 }
 
 }  // namespace wave

@@ -523,32 +523,63 @@ def generate_wave(net=None):
         w("  }")
     w("}")
     w("")
+    # per lane (species): elimination position and its forward / backward
+    # solve levels (inactive lanes: a spare slot and level 0, never written)
+    lpos = [pos[s_] if s_ < ns else 31 for s_ in range(64)]
+    llev = [(lev_r[pos[s_]] | (lev_b[pos[s_]] << 8)) if s_ < ns else 0 for s_ in range(64)]
+    # a component whose broadcast reaches a lane that is not final yet has to
+    # be masked: forward, lane i with an upper entry at k > pos(i) in a later
+    # level than k; backward, lane i with a multiplier at k < pos(i) in a
+    # later backward level than k
+    fw_mask = [any(F[i, k] and k > i and lev_r[i] > lev_r[k] for i in range(ns))
+               for k in range(ns)]
+    bw_mask = [any(F[i, k] and k < i and lev_b[i] > lev_b[k] for i in range(ns))
+               for k in range(ns)]
+    w("// per lane: elimination position (31: no row) and the forward | backward << 8")
+    w("// dependency levels of the solves")
+    w("#define P3D_WAVE_POS {" + ", ".join(str(x) for x in lpos) + "}")
+    w("#define P3D_WAVE_LEVELS {" + ", ".join(str(x) for x in llev) + "}")
+    w("")
     w("// Solve M x = b with the factors; b and x: the lane's species component.")
-    w("// Components of one dependency level are broadcast together.")
+    w("// Level by level: the components of one level are broadcast, each lane of")
+    w("// the level records its component in xs[pos], then every lane subtracts")
+    w("// a[k] x_k with no mask.  A lane reached by a component it has no entry")
+    w("// for in this triangle (an entry of the other triangle, or its own")
+    w("// diagonal) is always final by then, so only its register is spoilt and")
+    w("// the recorded xs stays exact; the few components for which that is not")
+    w("// so keep their mask.  dinv: the lane's 1 / u_ii.")
     w("template <class BC>")
     w("__device__ __forceinline__ double lu_solve(const double (&a)[32], unsigned lm, unsigned um, int lane,")
-    w("                                          double b, const BC &bc) {")
+    w("                                          double b, const BC &bc, double *xs, int pos,")
+    w("                                          int lev, double dinv) {")
+    w("  const int lvf = lev & 0xff, lvb = lev >> 8;")
     for g in groups(lev_r):
-        g = [k for k in g if has_l[k]]
-        if not g:
-            continue
         w("  {")
-        for k in g:
+        gl = [k for k in g if has_l[k]]
+        for k in gl:
             w(f"    const double x{k} = bc(b, {perm[k]});")
-        for k in g:
-            w(f"    if ((lm >> {k}) & 1u) b -= a[{k}] * x{k};")
+        w(f"    if (lvf == {lev_r[g[0]]}) xs[pos] = b;")
+        for k in gl:
+            if fw_mask[k]:
+                w(f"    if ((lm >> {k}) & 1u) b -= a[{k}] * x{k};")
+            else:
+                w(f"    b -= a[{k}] * x{k};")
         w("  }")
+    w("  b = xs[pos];")
     for g in groups(lev_b):
         w("  {")
-        for k in sorted(g, reverse=True):
-            w(f"    if (lane == {perm[k]}) b *= a[{k}];")
+        w("    const double bx = b * dinv;")
+        w(f"    if (lvb == {lev_b[g[0]]}) xs[pos] = bx;")
         gu = [k for k in sorted(g, reverse=True) if has_u[k]]
         for k in gu:
-            w(f"    const double x{k} = bc(b, {perm[k]});")
+            w(f"    const double x{k} = bc(bx, {perm[k]});")
         for k in gu:
-            w(f"    if ((um >> {k}) & 1u) b -= a[{k}] * x{k};")
+            if bw_mask[k]:
+                w(f"    if ((um >> {k}) & 1u) b -= a[{k}] * x{k};")
+            else:
+                w(f"    b -= a[{k}] * x{k};")
         w("  }")
-    w("  return b;")
+    w("  return xs[pos];")
     w("}")
     w("")
     w("}  // namespace wave")

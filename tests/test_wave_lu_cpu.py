@@ -21,6 +21,8 @@ HDR = os.path.join(ROOT, "prism3d_amd", "csrc", "wave_net.h")
 def _parse():
     txt = open(HDR).read()
     perm = [int(x) for x in re.search(r"kPerm\[kNS\] = \{([^}]*)\}", txt).group(1).split(",")]
+    lpos = [int(x) for x in re.search(r"P3D_WAVE_POS \{([^}]*)\}", txt).group(1).split(",")]
+    llev = [int(x) for x in re.search(r"P3D_WAVE_LEVELS \{([^}]*)\}", txt).group(1).split(",")]
     lmask = [int(x.strip().rstrip("u"), 16) for x in
              re.search(r"P3D_WAVE_LMASK \{([^}]*)\}", txt).group(1).split(",")]
     umask = [int(x.strip().rstrip("u"), 16) for x in
@@ -29,7 +31,8 @@ def _parse():
         i = txt.index(start)
         i = txt.index("{\n", i) + 2
         return [l.strip() for l in txt[i:txt.index("\n}\n", i)].split("\n") if l.strip()]
-    return perm, lmask, umask, body("void lu_factor"), body("double lu_solve")
+    return (perm, lmask, umask, np.array(lpos), np.array(llev), body("void lu_factor"),
+            body("double lu_solve"))
 
 
 def _run(lines, env):
@@ -38,13 +41,28 @@ def _run(lines, env):
     stack = []  # active-lane masks of the open if-blocks
     act = lambda: np.logical_and.reduce(stack) if stack else np.ones(64, bool)
     for l in lines:
-        if l.startswith("//") or l in ("{", "}") or l.startswith("{  //") or l.startswith("return"):
+        if l.startswith("//") or l in ("{", "}") or l.startswith("{  //") or l == "return b;":
             if l == "}" and stack:
                 stack.pop()
             continue
         m = re.match(r"if \(\(lm >> (\d+)\) & 1u\) \{$", l)
         if m:
             stack.append(((env["lm"] >> int(m.group(1))) & 1).astype(bool))
+            continue
+        if l == "const int lvf = lev & 0xff, lvb = lev >> 8;":
+            env["lvf"], env["lvb"] = env["lev"] & 0xFF, env["lev"] >> 8
+            continue
+        m = re.match(r"if \((lvf|lvb) == (\d+)\) xs\[pos\] = (b|bx);$", l)
+        if m:
+            # the lanes of this level record their component (LDS: one slot per position)
+            sel = (env[m.group(1)] == int(m.group(2))) & act()
+            env["xs"][env["pos"][sel]] = env[m.group(3)][sel]
+            continue
+        if l == "b = xs[pos];":
+            env["b"] = env["xs"][env["pos"]].copy()
+            continue
+        if l == "return xs[pos];":
+            env["ret"] = env["xs"][env["pos"]].copy()
             continue
         m = re.match(r"(?:\{ )?const double (\w+) = (.*);(?: \})?$", l)
         if m:
@@ -69,6 +87,7 @@ def _run(lines, env):
 def _eval(e, env):
     e = re.sub(r"bc\(a\[(\d+)\], (\d+)\)", r"A[\1][\2]", e)
     e = re.sub(r"bc\(b, (\d+)\)", r"B[\1]", e)
+    e = re.sub(r"bc\(bx, (\d+)\)", r"bx[\1]", e)
     e = re.sub(r"a\[(\d+)\]", r"A[\1]", e)
     e = re.sub(r"\bb\b", "B", e)
     v = eval(e, {}, {**env, "A": env["a"], "B": env["b"]})
@@ -102,7 +121,7 @@ def _pattern():
 
 @pytest.mark.parametrize("seed", range(6))
 def test_generated_register_lu_solves_like_lapack(seed):
-    perm, lmask, umask, fac, sol = _parse()
+    perm, lmask, umask, lpos, llev, fac, sol = _parse()
     pat = _pattern()
     ns = pat.shape[0]
     rng = np.random.RandomState(seed)
@@ -122,8 +141,14 @@ def test_generated_register_lu_solves_like_lapack(seed):
     env = {"a": a, "lm": lm, "um": um, "lane": lanes, "b": np.zeros(64)}
     _run(fac, env)
     rhs = rng.normal(size=ns)
-    env["b"] = np.concatenate([rhs, np.zeros(64 - ns)])
+    # the solve's inputs as LinStatic::solve passes them: the lane's 1 / u_ii
+    # (a[pos]), its position and levels, and a 32-slot record in LDS whose
+    # spare slot 31 holds garbage
+    dinv = np.array([env["a"][lpos[s], s] if s < ns else 0.0 for s in lanes])
+    xs = np.full(32, np.nan)
+    env.update({"b": np.concatenate([rhs, np.zeros(64 - ns)]), "pos": lpos, "lev": llev,
+                "dinv": dinv, "xs": xs})
     _run(sol, env)
-    x = env["b"][:ns]
+    x = env["ret"][:ns]
     ref = np.linalg.solve(M, rhs)
     assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref)) * np.linalg.cond(M)
