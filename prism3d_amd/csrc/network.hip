@@ -93,6 +93,7 @@ static Network *find_net(int64_t h) {
 
 struct CellParams {
   double nH, T, G0, AV, zeta, fH2, fCO;
+  double g0_sqrt_t;  // G0 sqrt(T) of the PAH charge balance, formed once per cell
 };
 
 // LDS layout of one cell (one workgroup), all doubles unless noted.
@@ -184,6 +185,11 @@ struct Wave1 {
   static constexpr int W = 64;
   double *red;
   __device__ __forceinline__ double sum(double v) const { return wave_sum(v); }
+  // every lane's v finite (the per-element test of the reference's Newton
+  // iteration), as one ballot
+  __device__ __forceinline__ bool all_finite(double v) const {
+    return __builtin_amdgcn_ballot_w64(!isfinite(v)) == 0;
+  }
   __device__ __forceinline__ double bc(double v, int src) const { return bcast(v, src); }
   __device__ __forceinline__ int argmax(double v, int lane) const {
     return wave_argmax(v, lane);
@@ -199,6 +205,9 @@ struct Wave2 {
     const double r = red[0] + red[1];
     __syncthreads();
     return r;
+  }
+  __device__ __forceinline__ bool all_finite(double v) const {
+    return sum(isfinite(v) ? 0.0 : 1.0) == 0.0;
   }
   __device__ __forceinline__ double bc(double v, int src) const {
     if ((int)threadIdx.x == src) red[0] = v;
@@ -304,7 +313,7 @@ __device__ __forceinline__ void pah_minus(double ye_floored, const CellParams &p
   double ye = fmax(ye_floored, 1e-10);
   double ne = ye * p.nH;
   if (p.G0 > 0.0 && ne > 0.0) {
-    double psi = p.G0 * sqrt(p.T) / ne;
+    double psi = p.g0_sqrt_t / ne;
     double fm = fmax(500.0 / (psi + 500.0) * 0.5, 0.0);
     pm = xpah * fm;
     dpm = ye_floored > 1e-10 ? xpah * 250.0 * psi / ((psi + 500.0) * (psi + 500.0) * ye) : 0.0;
@@ -1205,7 +1214,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
           publish_y(N, L, ynew, lane);
           double f = rhs();
           st.rhs++;
-          if (!isfinite(g.sum(act ? f : 0.0))) break;
+          if (!g.all_finite(act ? f : 0.0)) break;
           double dy = act ? c * f - psi - d : 0.0;
           dy = lin.solve(N, L, dy, lane);
           if (!act) dy = 0.0;
@@ -1374,6 +1383,7 @@ __device__ __forceinline__ CellParams load_params(int64_t c, const double *nH, c
   CellParams p;
   p.nH = nH[c]; p.T = T[c]; p.G0 = G0[c]; p.AV = AV[c];
   p.zeta = zeta[c]; p.fH2 = fH2[c]; p.fCO = fCO[c];
+  p.g0_sqrt_t = p.G0 * sqrt(p.T);
   return p;
 }
 
@@ -1568,7 +1578,7 @@ chem_post_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__re
   const int lane = threadIdx.x;
   CellParams p;
   p.nH = nH[cell]; p.T = T[cell]; p.zeta = zeta[cell];
-  p.G0 = 0.0; p.AV = 0.0; p.fH2 = 1.0; p.fCO = 1.0;
+  p.G0 = 0.0; p.AV = 0.0; p.fH2 = 1.0; p.fCO = 1.0; p.g0_sqrt_t = 0.0;
   double yc = lane < N.ns ? y[(int64_t)lane * n + cell] : 0.0;
   post_solve(N, p, lane, yc, mode, x_c_total, x_o_total);
   if (lane < N.ns) y[(int64_t)lane * n + cell] = yc;
