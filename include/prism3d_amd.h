@@ -255,13 +255,59 @@ int p3d_chem_polish(int64_t handle, int64_t n, double *y, const double *n_h,
  * = {Lambda_CII, n_H x_CO}; grad (n x (n_reactions + 3)) = dJ_c/dlog10 k_r for
  * every reaction, then dJ_c/dln G0, dJ_c/dln zeta, dJ_c/dln n_H.  One
  * transposed solve per cell (implicit function theorem).  Differentiation of
- * the chemistry-thermal step (north-star objective; no upstream counterpart). */
+ * the chemistry step; no upstream counterpart. */
 int p3d_chem_adjoint(int64_t handle, int64_t n, const double *y,
                      const double *n_h, const double *t_gas, const double *g0,
                      const double *a_v, const double *zeta, const double *fsh_h2,
                      const double *fsh_co, const p3d_steady_cons *cons,
                      const double *cii5, double w_cii, double w_co, double *obj,
                      double *grad, void *stream);
+
+/* Discrete adjoint of the implicit steps.  The differentiated map is the BDF
+ * integration of chemistry/solver.py:185-217 over one span (opts:
+ * n_checkpoints 2, early_stop 0, post_mode -1, FP64 rates) with its step
+ * sequence held fixed; bundled network only (the register-LU form).  No
+ * upstream counterpart (the reference integrates with scipy and has no
+ * gradient).
+ *
+ * p3d_chem_tape_bytes: bytes of the tape of n cells at cap accepted steps per
+ * cell (cap a multiple of 16): 24 bytes per step and the differences (2 KiB)
+ * every 16 steps, per cell. */
+int p3d_chem_tape_bytes(int64_t n, int32_t cap, int64_t *bytes);
+
+/* The integration of p3d_chem_solve (same kernel, same results: y updated
+ * in place to max(y(t_max), 1e-30), stats as there, may be NULL) that also
+ * records into the device buffer tape (tape_bytes >= p3d_chem_tape_bytes(n,
+ * cap)) per accepted step its c = h / alpha_k, order k and step-size change
+ * factor, and the differences every 16 steps.  A cell that takes more than
+ * cap steps keeps its result but gets no gradient. */
+int p3d_chem_solve_tape(int64_t handle, int64_t n, double *y, const double *n_h,
+                        const double *t_gas, const double *g0, const double *a_v,
+                        const double *zeta, const double *fsh_h2,
+                        const double *fsh_co, const p3d_solve_opts *opts, void *tape,
+                        int64_t tape_bytes, int32_t cap, int32_t *stats,
+                        void *stream);
+
+/* Gradient of J_c = w_cii * Lambda_CII(T, n_H, y_end) + w_co * n_H * x_CO(y_end)
+ * per cell (as p3d_chem_adjoint; cii5 = host {freq, A, T_ul, g_u, g_l}) with
+ * respect to log10 k_r of every reaction, ln G0, ln zeta and ln n_H, through
+ * the taped integration from y0 (the initial abundances it started from) to
+ * y_end (its output), with the same inputs, opts and tape.  Walks the steps
+ * backwards in 16-step segments: each segment is recomputed from its
+ * checkpoint, the factors of I - c J(y_n+1) of its steps are stored and
+ * reused by the transposed solves.  obj (n x 2) = {Lambda_CII, n_H x_CO};
+ * grad (n x (n_reactions + 3)); status[c] (may be NULL) = 0, or -1 failed
+ * integration, -2 tape too short, -3 a recomputed segment missed the
+ * forward's state at its end by more than 1e-6 relative (species above
+ * 1e-14) (those cells' gradients are not valid). */
+int p3d_chem_tape_adjoint(int64_t handle, int64_t n, const double *y0,
+                          const double *y_end, const double *n_h,
+                          const double *t_gas, const double *g0, const double *a_v,
+                          const double *zeta, const double *fsh_h2,
+                          const double *fsh_co, const p3d_solve_opts *opts,
+                          const void *tape, int64_t tape_bytes, int32_t cap,
+                          const double *cii5, double w_cii, double w_co, double *obj,
+                          double *grad, int32_t *status, void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Whole-grid kernels around the per-cell step                              */

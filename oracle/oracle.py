@@ -480,6 +480,109 @@ def polish_steady(y, nH, T, G0, AV, zeta, fH2=1.0, fCO=1.0, max_iter=600, y_tota
     return x, -1
 
 
+def param_derivatives(y, nH, T, G0, AV, zeta, fH2=1.0, fCO=1.0):
+    """df/dtheta (NSP, 78) of the rate equations at y with respect to log10
+    k_r (75), ln G0, ln zeta and ln n_H at fixed abundances: the rates'
+    explicit dependence (photo rates on G0, cosmic-ray rates on zeta,
+    two-body and grain rates on n_H) and the PAH- closed form's through
+    psi = G0 sqrt(T) / n_e (chemistry/solver.py:74-87)."""
+    rate, psi = reaction_rates(y, nH, T, G0, AV, zeta, fH2, fCO)
+    types, react, _ = reaction_table()
+    fpah = np.where(react[:, 1] == -2, psi / (psi + 500.0), 0.0) if psi > 0 else np.zeros(75)
+    dr = np.zeros((75, 78))
+    dr[np.arange(75), np.arange(75)] = np.log(10.0) * rate
+    photo = np.isin(types, [T_PH, T_PH_H2, T_PH_CO])
+    dr[:, 75] = rate * photo - rate * fpah
+    dr[:, 76] = rate * (types == T_CR)
+    dr[:, 77] = rate * np.where(types == T_2B, 1.0 + fpah, np.where(types == T_GS, 1.0, 0.0))
+    return stoichiometry() @ dr
+
+
+# NDF coefficients of the variable-order BDF (Shampine & Reichelt 1997, as
+# scipy's BDF uses them): gamma_k = sum_{j<=k} 1/j, kappa_k, alpha_k
+BDF_GAMMA = np.concatenate([[0.0], np.cumsum(1.0 / np.arange(1, 7))])
+BDF_KAPPA = np.array([0.0, -0.1850, -1.0 / 9.0, -0.0823, -0.0415, 0.0])
+BDF_ALPHA = (1.0 - BDF_KAPPA) * BDF_GAMMA[:6]
+
+
+def change_matrix(factor, order):
+    """RU of a step-size change by `factor` at `order`: the differences
+    D[0..order] become RU^T D (R_im = prod_{q<=i} (q - 1 - factor m) / q,
+    U = R at factor 1; Shampine & Reichelt 1997, scipy's change_D)."""
+    i = np.arange(1, order + 1)[:, None]
+    j = np.arange(1, order + 1)[None, :]
+    def r_of(f):
+        m = np.zeros((order + 1, order + 1))
+        m[1:, 1:] = (i - 1 - f * j) / i
+        m[0] = 1.0
+        return np.cumprod(m, axis=0)
+    return r_of(factor) @ r_of(1.0)
+
+
+def taped_sensitivities(y0, steps, h0, nH, T, G0, AV, zeta, fH2=1.0, fCO=1.0, w_cii=1.0,
+                        w_co=1.0, line=None):
+    """Forward (tangent-linear) sensitivities of the BDF integration through a
+    recorded step sequence, the reference for the discrete adjoint of the
+    implicit steps.  steps: (n_steps, 3) rows {c = h / alpha_k, F, k}: before
+    step n the differences are rescaled by F at order k, then the implicit
+    step d - c f(ypred + d) + psi = 0 is solved (Newton to convergence with the
+    exact Jacobian) and the differences updated as scipy's BDF does; h0 gives
+    D[1] = h0 f(y0).  The sensitivities dD/dtheta (78 parameters: log10 k_r,
+    ln G0, ln zeta, ln n_H) follow the same linear steps with
+    (I - c J) d' = c J ypred' + c df/dtheta - psi'.  Returns (y_end,
+    dy_end/dtheta (NSP, 78), objective (2,), dJ/dtheta (78,)) for
+    J = w_cii Lambda_CII + w_co n_H x_CO at y_end = max(D[0], 1e-30)."""
+    par = (nH, T, G0, AV, zeta, fH2, fCO)
+    D = np.zeros((8, NSP))
+    S = np.zeros((8, NSP, 78))
+    D[0] = y0
+    D[1] = h0 * rhs(y0, *par)
+    S[1] = h0 * param_derivatives(y0, *par)
+    eye = np.eye(NSP)
+    for c, F, k in steps:
+        k = int(k)
+        if F != 1.0:
+            RU = change_matrix(F, k)
+            D[:k + 1] = RU.T @ D[:k + 1]
+            S[:k + 1] = np.einsum('ij,ism->jsm', RU, S[:k + 1])
+        ypred = D[:k + 1].sum(axis=0)
+        psi = BDF_GAMMA[1:k + 1] @ D[1:k + 1] / BDF_ALPHA[k]
+        d = np.zeros(NSP)
+        for _ in range(30):
+            y = ypred + d
+            G = d - c * rhs(y, *par) + psi
+            dd = np.linalg.solve(eye - c * jacobian(y, *par), -G)
+            d = d + dd
+            if np.max(np.abs(dd) / (np.abs(y) + 1e-300)) < 1e-15:
+                break
+        yn = ypred + d
+        Jn = jacobian(yn, *par)
+        Sp = S[:k + 1].sum(axis=0)
+        Spsi = np.einsum('j,jsm->sm', BDF_GAMMA[1:k + 1], S[1:k + 1]) / BDF_ALPHA[k]
+        Sd = np.linalg.solve(eye - c * Jn, c * Jn @ Sp + c * param_derivatives(yn, *par) - Spsi)
+        # difference update (D[k+2] = d - D[k+1], D[k+1] = d, D[j] += D[j+1])
+        D[k + 2] = d - D[k + 1]
+        S[k + 2] = Sd - S[k + 1]
+        D[k + 1] = d
+        S[k + 1] = Sd
+        for j in range(k, -1, -1):
+            D[j] += D[j + 1]
+            S[j] += S[j + 1]
+    y = np.maximum(D[0], 1e-30)
+    dy = np.where((D[0] > 1e-30)[:, None], S[0], 0.0)
+    lam, dl = cii_cooling(T, nH, y[IDX['C+']], y[IDX['e-']], y[IDX['H']], y[IDX['H2']],
+                          *(() if line is None else (line,)))
+    g = np.zeros(NSP)
+    g[IDX['C+']] += w_cii * dl['x_cp']
+    g[IDX['e-']] += w_cii * dl['x_e']
+    g[IDX['H']] += w_cii * dl['x_h']
+    g[IDX['H2']] += w_cii * dl['x_h2']
+    g[IDX['CO']] += w_co * nH
+    grad = g @ dy
+    grad[77] += w_cii * dl['ln_nh'] + w_co * nH * y[IDX['CO']]
+    return y, dy, np.array([lam, nH * y[IDX['CO']]]), grad
+
+
 def steady_sensitivities(x, nH, T, G0, AV, zeta, fH2, fCO, w_cii=1.0, w_co=1.0, forward=True):
     """Gradient of J = w_cii Lambda_CII(T, n_H, x*) + w_co n_H x*_CO at the
     charge-neutral steady state x* of one cell (polish_steady; radiation,
@@ -493,9 +596,6 @@ def steady_sensitivities(x, nH, T, G0, AV, zeta, fH2, fCO, w_cii=1.0, w_co=1.0, 
     x = np.asarray(x, dtype=np.float64)
     C, frozen, piv = _steady_roles(x)
     _, M = _steady_system(x, C @ x, C, frozen, piv, par)
-    rate, psi = reaction_rates(x, *par)
-    types, react, _ = reaction_table()
-    nu = stoichiometry()
     moving = ~frozen
     moving[I['e-']] = False
     for s in piv:
@@ -508,14 +608,7 @@ def steady_sensitivities(x, nH, T, G0, AV, zeta, fH2, fCO, w_cii=1.0, w_co=1.0, 
     g[I['H']] += w_cii * dl['x_h']
     g[I['H2']] += w_cii * dl['x_h2']
     g[I['CO']] += w_co * nH
-    fpah = np.where(react[:, 1] == -2, psi / (psi + 500.0), 0.0) if psi > 0 else np.zeros(75)
-    dr = np.zeros((75, 78))
-    dr[np.arange(75), np.arange(75)] = np.log(10.0) * rate
-    photo = np.isin(types, [T_PH, T_PH_H2, T_PH_CO])
-    dr[:, 75] = rate * photo - rate * fpah
-    dr[:, 76] = rate * (types == T_CR)
-    dr[:, 77] = rate * np.where(types == T_2B, 1.0 + fpah, np.where(types == T_GS, 1.0, 0.0))
-    Fp = nu @ dr
+    Fp = param_derivatives(x, *par)
     Fp[~moving] = 0.0
     explicit = np.zeros(78)
     explicit[77] = w_cii * dl['ln_nh'] + w_co * nH * x[I['CO']]

@@ -61,6 +61,12 @@ SIGNATURES = {
                                                         _VP, _VP]),
     "p3d_chem_adjoint": (_i32, [_i64, _i64] + [_VP] * 8 + [ctypes.POINTER(SteadyCons), _D, _dbl,
                                                          _dbl, _VP, _VP, _VP]),
+    "p3d_chem_tape_bytes": (_i32, [_i64, _i32, _I64P]),
+    "p3d_chem_solve_tape": (_i32, [_i64, _i64] + [_VP] * 8 + [ctypes.POINTER(SolveOpts), _VP,
+                                                            _i64, _i32, _VP, _VP]),
+    "p3d_chem_tape_adjoint": (_i32, [_i64, _i64] + [_VP] * 9 + [ctypes.POINTER(SolveOpts), _VP,
+                                                               _i64, _i32, _D, _dbl, _dbl, _VP,
+                                                               _VP, _VP, _VP]),
     "p3d_fuv_columns": (_i32, [_i64, _i64, _i64, _VP, _VP, _VP, _dbl, _i32, _D, _D, _dbl]
                         + [_VP] * 5 + [_VP, _VP]),
     "p3d_fuv_point_source": (_i32, [_i64, _i64, _i64, _VP, _VP, _VP, _D, _dbl, _dbl, _dbl, _dbl]

@@ -110,3 +110,103 @@ class SteadyObjective(torch.autograd.Function):
 
 def steady_objective(theta, problem):
     return SteadyObjective.apply(theta, problem)
+
+
+class TransientProblem(SteadyStateProblem):
+    """The same objective at the end of the BDF integration itself: every
+    cell integrated from y0 over one span to t_end (the reference's
+    integration, chemistry/solver.py:185-217; t_end None = its steady-state
+    horizon min(3e17, max(1e16, 3e17 / n_H)) s) with frozen fields.  The
+    gradient is the discrete adjoint of the implicit steps with the step
+    sequence held fixed (p3d_chem_tape_adjoint): the taped forward records
+    each step's c, order and step-size change and the differences every 16
+    steps; the adjoint recomputes each 16-step segment from its checkpoint,
+    stores its steps' factors of I - c J and reuses them in the transposed
+    solves.  cap bounds the accepted steps per cell of the tape (a cell past
+    it gets no gradient: status -2)."""
+
+    def __init__(self, n_H, T, G0, A_V, zeta, fsh_H2, fsh_CO, y0, w_cii=1.0, w_co=1.0,
+                 network=None, t_end=None, rtol=1e-9, atol=1e-20, cap=4096, max_steps=1000000):
+        super().__init__(n_H, T, G0, A_V, zeta, fsh_H2, fsh_CO, y0, w_cii=w_cii, w_co=w_co,
+                         network=network, rtol=rtol, atol=atol)
+        self.opts = dict(t_max=t_end, t_start=1e4, n_checkpoints=2, early_stop=False,
+                         rtol=rtol, atol=atol, post_mode=-1, max_steps=max_steps)
+        self.cap = int(cap)
+        self.last = None
+
+    def state(self, theta=None):
+        """State at t_end (n_species, n) at parameters theta, with its tape."""
+        th, ins = self.inputs(theta)
+        cs = self._solver(th[:self.nr])
+        y = self.y0.clone()
+        stats, tape = cs.solve_tape(y, *ins, opts=solve_opts(**self.opts), cap=self.cap,
+                                    sync=False)
+        self.last = dict(theta=th, cs=cs, ins=ins, y=y, tape=tape, stats=stats)
+        return y, cs, ins
+
+    def objective(self, theta=None):
+        """(J, per-cell objective (n, 2)) at the end of the integration."""
+        y, cs, ins = self.state(theta)
+        obj = _objective_values(y, ins, self.network, CII_158)
+        J = self.w_cii * obj[:, 0].sum() + self.w_co * obj[:, 1].sum()
+        return J, obj
+
+    def gradient(self, last=None, w_cii=None, w_co=None):
+        """dJ/dtheta (n_params,), per-cell objective, status and per-cell
+        gradient of a forward state() (the last one by default): the adjoint
+        walk over its tape (weights: the problem's unless given)."""
+        L = last if last is not None else self.last
+        if L is None:
+            raise RuntimeError("gradient() needs a forward state() first")
+        wc = self.w_cii if w_cii is None else float(w_cii)
+        wo = self.w_co if w_co is None else float(w_co)
+        obj, grad, status = L["cs"].tape_adjoint(self.y0, L["y"], *L["ins"], tape=L["tape"],
+                                                 w_cii=wc, w_co=wo, line=CII_158, sync=False)
+        return grad.sum(dim=0), obj, status, grad
+
+    def evaluate(self, theta=None, want_grad=True):
+        """(J, per-cell objective (n, 2), dJ/dtheta or None, state)."""
+        y, cs, ins = self.state(theta)
+        g, obj, status, _ = self.gradient()
+        J = self.w_cii * obj[:, 0].sum() + self.w_co * obj[:, 1].sum()
+        return J, obj, (g if want_grad else None), y
+
+
+def _objective_values(y, ins, network, line):
+    """{Lambda_CII, n_H x_CO} per cell at abundances y (the objective of the
+    adjoints, solver_3d.py:956-965), on the device with torch arithmetic."""
+    n_H, T = ins[0], ins[1]
+    idx = network.idx
+    x = lambda s: y[idx[s]]
+    freq, A, Tul, gu, gl = [float(v) for v in line]
+    ge = 8.63e-6 / (gl * torch.sqrt(T)) * 2.15 * torch.exp(-Tul / T)
+    gh = 8.0e-10 * (T / 100.0) ** 0.07
+    gh2 = 4.9e-10 * (T / 100.0) ** 0.12
+    q = ge * (x('e-') * n_H) + gh * (x('H') * n_H) + gh2 * (x('H2') * n_H)
+    ar = (gu / gl) * torch.exp(-Tul / T)
+    frac = ar * q / (ar * q + A + q)
+    lam = x('C+') * n_H * frac * (A * 6.62607015e-27 * freq)
+    return torch.stack([lam, n_H * x('CO')], dim=1)
+
+
+class TransientObjective(torch.autograd.Function):
+    """J(theta) of a TransientProblem: ``J = TransientObjective.apply(theta,
+    problem)``; ``J.backward()`` runs the discrete adjoint of the implicit
+    steps over the tape of this forward and fills theta.grad."""
+
+    @staticmethod
+    def forward(ctx, theta, problem):
+        J, _ = problem.objective(theta)
+        ctx.problem, ctx.last = problem, problem.last
+        ctx.meta = (theta.device, theta.dtype)
+        return J.to(theta.device, theta.dtype)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        g, _, status, _ = ctx.problem.gradient(ctx.last)
+        dev, dt = ctx.meta
+        return gJ * g.to(dev, dt), None
+
+
+def transient_objective(theta, problem):
+    return TransientObjective.apply(theta, problem)

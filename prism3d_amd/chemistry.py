@@ -271,6 +271,83 @@ class ChemistrySolver:
         _lib.check(rc, "p3d_chem_adjoint")
         return obj, grad
 
+    # --- discrete adjoint of the implicit steps ------------------------------
+    def solve_tape(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0, f_shield_CO=1.0,
+                   opts=None, cap=4096, sync=True):
+        """The integration of solve_steady_batch (one span, no post-solve
+        corrections: opts must have n_checkpoints 2, early_stop False,
+        post_mode -1) with its step sequence recorded for tape_adjoint.
+        y (n_species, n) is updated in place.  Returns (stats int32[n, 4],
+        tape) where tape is a dict holding the device buffer and its cap."""
+        self._check_state(y)
+        n = y.shape[1]
+        p = _cell_arrays(n, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO)
+        nbytes = ctypes.c_int64(0)
+        _lib.check(_lib.lib().p3d_chem_tape_bytes(n, int(cap), ctypes.byref(nbytes)),
+                   "p3d_chem_tape_bytes")
+        buf = torch.empty(max(1, (nbytes.value + 7) // 8), dtype=torch.float64, device=y.device)
+        stats = torch.empty((n, 4), dtype=torch.int32, device=y.device)
+        o = opts if opts is not None else solve_opts(n_checkpoints=2, early_stop=False,
+                                                     post_mode=-1)
+        rc = _lib.lib().p3d_chem_solve_tape(self.dnet.handle, n, y.data_ptr(),
+                                            *[t.data_ptr() for t in p], o, buf.data_ptr(),
+                                            nbytes.value, int(cap), stats.data_ptr(),
+                                            stream_handle())
+        _lib.check(rc, "p3d_chem_solve_tape")
+        if sync:
+            torch.cuda.synchronize(y.device)
+        return stats, {"buffer": buf, "bytes": nbytes.value, "cap": int(cap), "opts": o, "n": n}
+
+    @staticmethod
+    def tape_steps(tape, cells):
+        """Host copy of the recorded step sequence of the given cells: a list
+        of (steps (n_steps, 3) rows {c, F, k}, h0, n_steps); the layout of
+        p3d_chem_tape_bytes: step records n x cap x 3, differences every 16
+        steps n x (cap / 16 + 1) x 8 x 32, initial step sizes n, step counts n
+        (int32)."""
+        n, cap = tape["n"], tape["cap"]
+        buf = tape["buffer"]
+        rec_d = n * cap * 3
+        ck_d = n * (cap // 16 + 1) * 8 * 32
+        h0 = buf[rec_d + ck_d:rec_d + ck_d + n].cpu().numpy()
+        ns = buf[rec_d + ck_d + n:rec_d + ck_d + n + (n + 1) // 2].cpu().numpy()
+        nsteps = ns.view(np.int32)[:n]
+        out = []
+        for c in cells:
+            k = int(nsteps[c])
+            rec = buf[c * cap * 3:c * cap * 3 + 3 * max(k, 0)].cpu().numpy().reshape(-1, 3)
+            out.append((rec, float(h0[c]), k))
+        return out
+
+    def tape_adjoint(self, y0, y_end, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0,
+                     f_shield_CO=1.0, tape=None, w_cii=1.0, w_co=1.0, line=CII_158,
+                     sync=True):
+        """Per-cell objective {Lambda_CII, n_H x_CO} (n, 2), gradient (n,
+        n_reactions + 3) of w_cii Lambda_CII + w_co n_H x_CO at y_end with
+        respect to log10 k_r, ln G0, ln zeta, ln n_H through the taped
+        integration from y0 (p3d_chem_tape_adjoint), and status int32[n]
+        (0 valid)."""
+        self._check_state(y_end)
+        self._check_state(y0)
+        n = y_end.shape[1]
+        if tape is None or tape["n"] != n:
+            raise ValueError("tape_adjoint needs the tape of solve_tape on the same cells")
+        p = _cell_arrays(n, n_H, T, G0, A_V, zeta_CR, f_shield_H2, f_shield_CO)
+        obj = torch.empty((n, 2), dtype=torch.float64, device=y_end.device)
+        grad = torch.empty((n, self.network.n_reactions + 3), dtype=torch.float64,
+                           device=y_end.device)
+        status = torch.empty(n, dtype=torch.int32, device=y_end.device)
+        lp = np.ascontiguousarray(line, dtype=np.float64)
+        rc = _lib.lib().p3d_chem_tape_adjoint(
+            self.dnet.handle, n, y0.contiguous().data_ptr(), y_end.contiguous().data_ptr(),
+            *[t.data_ptr() for t in p], tape["opts"], tape["buffer"].data_ptr(), tape["bytes"],
+            tape["cap"], _lib.dptr(lp), float(w_cii), float(w_co), obj.data_ptr(),
+            grad.data_ptr(), status.data_ptr(), stream_handle())
+        _lib.check(rc, "p3d_chem_tape_adjoint")
+        if sync:
+            torch.cuda.synchronize(y_end.device)
+        return obj, grad, status
+
     def rhs_batch(self, y, n_H, T, G0, A_V, zeta_CR, f_shield_H2=1.0, f_shield_CO=1.0):
         self._check_state(y)
         n = y.shape[1]

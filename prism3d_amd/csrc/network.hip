@@ -47,6 +47,7 @@ struct NetDev {
   const double *alpha, *beta, *gamma;
   const int *react;   // 2 per reaction
   const int *dep;     // 2 per reaction: species the rate depends on (or -1)
+  const int *st5;     // 5 per reaction: consumed species (2 slots) then produced (3), or -1
   const int *rhs_off; // ns+1
   const int *rhs_rxn;
   const double *rhs_sgn;
@@ -72,12 +73,16 @@ struct Network {
   // work counter; calls on one handle share it, so they run in stream order
   void *lane_ws = nullptr;
   size_t lane_ws_bytes = 0;
-  // work counter of the persistent one-cell-per-wavefront kernel (variant 6)
+  // work counter of the persistent one-cell-per-wavefront kernels
   unsigned long long *queue_ctr = nullptr;
+  // per resident wavefront of the tape adjoint: one segment's stored factors
+  void *segbuf = nullptr;
+  size_t seg_bytes = 0;
   ~Network() {
     if (blob) (void)hipFree(blob);
     if (lane_ws) (void)hipFree(lane_ws);
     if (queue_ctr) (void)hipFree(queue_ctr);
+    if (segbuf) (void)hipFree(segbuf);
   }
 };
 
@@ -1116,13 +1121,22 @@ struct BdfStats {
   int steps, rhs, jac, lu, rejects;
 };
 
+// Step recorder of bdf_integrate.  NoRec records nothing (every other form);
+// the taped forward of the discrete adjoint records the step sequence.
+struct NoRec {
+  __device__ __forceinline__ void init(const double (&)[kNumD], double, int) {}
+  __device__ __forceinline__ void change(double) {}
+  __device__ __forceinline__ void accept(int, double, const double (&)[kNumD], int) {}
+};
+
 // Variable-order BDF from t0 to t1 for the wave's cell.  y: lane's species.
 // Returns 0 on success, -1 step underflow, -2 step limit.
-template <class Lin, class G = Wave1>
-__device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
-                                             Lin &lin, const RxCache &rc, int lane, double &y,
-                                             double t0, double t1, double rtol, double atol,
-                                             double max_step, int max_steps, BdfStats &st) {
+template <class Lin, class G, class Rec>
+__device__ __forceinline__ int bdf_integrate_rec(const NetDev &N, const CellParams &p,
+                                                 const Lds &L, Lin &lin, const RxCache &rc,
+                                                 int lane, double &y, double t0, double t1,
+                                                 double rtol, double atol, double max_step,
+                                                 int max_steps, BdfStats &st, Rec &rec) {
   const G g{L.red};
   const bool cached = G::W == 64 && N.nr <= 64 * kRxPerLane;
   auto rhs = [&]() {
@@ -1166,6 +1180,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
   for (int j = 0; j < kNumD; ++j) D[j] = 0.0;
   D[0] = y;
   D[1] = act ? f0 * h : 0.0;
+  rec.init(D, h, lane);
   int order = 1, n_equal = 0;
   double t = t0;
   publish_y(N, L, y, lane);
@@ -1181,11 +1196,11 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
   while (t < t1) {
     if (steps++ > max_steps) return -2;
     const double hmin = 10.0 * fabs(nextafter(t, INFINITY) - t);
-    if (h > max_step) { change_D(L, lane, D, order, max_step / h); h = max_step; n_equal = 0; lu_valid = false; }
-    if (h < hmin) { change_D(L, lane, D, order, hmin / h); h = hmin; n_equal = 0; lu_valid = false; }
-    if (t + h > t1) { change_D(L, lane, D, order, (t1 - t) / h); h = t1 - t; n_equal = 0; lu_valid = false; }
+    if (h > max_step) { rec.change(max_step / h); change_D(L, lane, D, order, max_step / h); h = max_step; n_equal = 0; lu_valid = false; }
+    if (h < hmin) { rec.change(hmin / h); change_D(L, lane, D, order, hmin / h); h = hmin; n_equal = 0; lu_valid = false; }
+    if (t + h > t1) { rec.change((t1 - t) / h); change_D(L, lane, D, order, (t1 - t) / h); h = t1 - t; n_equal = 0; lu_valid = false; }
 
-    double ynew = 0.0;
+    double ynew = 0.0, c_acc = 0.0;
     for (;;) {  // attempt the step until accepted
       const double tn = fmin(t + h, t1);
       double ypred = 0.0, psi = 0.0;
@@ -1198,6 +1213,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
       psi /= alp(order);
       scale = atol + rtol * fabs(ypred);
       const double c = h / alp(order);
+      c_acc = c;
       bool converged = false;
       for (;;) {
         if (!lu_valid || c != c_lu) {
@@ -1248,6 +1264,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
       if (!converged) {
         h *= 0.5;
         if (!(h >= hmin)) return -1;  // also ends the span on a NaN step size
+        rec.change(0.5);
         change_D(L, lane, D, order, 0.5);
         n_equal = 0;
         lu_valid = false;
@@ -1261,6 +1278,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
         double factor = fmax(0.2, 0.9 * pow(en, -1.0 / (order + 1)));
         h *= factor;
         if (!(h > 0.0)) return -1;  // step size underflow or NaN
+        rec.change(factor);
         change_D(L, lane, D, order, factor);
         n_equal = 0;
         lu_valid = false;
@@ -1282,6 +1300,7 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
 #pragma unroll
     for (int j = kNumD - 2; j >= 0; --j)
       if (j <= order) D[j] += D[j + 1];
+    rec.accept(order, c_acc, D, lane);
     n_equal++;
     if (t >= t1) break;
     if (n_equal < order + 1) continue;
@@ -1317,12 +1336,23 @@ __device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &
     order += dord;
     const double factor = fmin(10.0, 0.9 * best);
     h *= factor;
+    rec.change(factor);
     change_D(L, lane, D, order, factor);
     n_equal = 0;
     lu_valid = false;
   }
   y = D[0];
   return 0;
+}
+
+template <class Lin, class G = Wave1>
+__device__ __forceinline__ int bdf_integrate(const NetDev &N, const CellParams &p, const Lds &L,
+                                             Lin &lin, const RxCache &rc, int lane, double &y,
+                                             double t0, double t1, double rtol, double atol,
+                                             double max_step, int max_steps, BdfStats &st) {
+  NoRec rec;
+  return bdf_integrate_rec<Lin, G, NoRec>(N, p, L, lin, rc, lane, y, t0, t1, rtol, atol,
+                                          max_step, max_steps, st, rec);
 }
 
 // H+/e- steady-state correction (solver.py:227-281) and conservation
@@ -1773,27 +1803,17 @@ struct CiiLine {
   double freq, A, Tul, gu, gl;
 };
 
-__global__ void __launch_bounds__(64)
-chem_adjoint_kernel(NetDev N, SteadyCons K, int64_t n, const double *__restrict__ y,
-                    const double *__restrict__ nH, const double *__restrict__ T,
-                    const double *__restrict__ G0, const double *__restrict__ AV,
-                    const double *__restrict__ zeta, const double *__restrict__ fH2,
-                    const double *__restrict__ fCO, CiiLine cl, double w_cii, double w_co,
-                    double *__restrict__ obj, double *__restrict__ grad) {
+// The differentiation objective of one cell at abundances x (lane's
+// species): the [CII] 158 um cooling (solver_3d.py:956-965) weighted by
+// w_cii plus the CO column density n_H x_CO weighted by w_co.  g: its
+// derivative with respect to the lane's abundance; explicit_n: its explicit
+// derivative with respect to ln n_H (abundances held).
+struct CellObjective {
+  double g, lam, co, explicit_n;
+};
+__device__ CellObjective cell_objective(const NetDev &N, const CellParams &p, const CiiLine &cl,
+                                        double w_cii, double w_co, double x, int lane) {
 #pragma clang fp contract(off)
-  extern __shared__ double smem[];
-  const int64_t cell = blockIdx.x;
-  const int lane = threadIdx.x;
-  const Lds L = carve(smem, N);
-  const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
-  rate_coeffs(N, p, L, lane);
-  __syncthreads();
-  const double x = lane < N.ns ? y[(int64_t)lane * n + cell] : 0.0;
-  publish_y(N, L, x, lane);
-  jacobian(N, p, L, lane);
-  const SteadyRows rows = steady_rows(N, K, x, lane);
-  steady_matrix(N, K, L, rows, true, lane);
-  // objective: [CII] 158 um cooling (solver_3d.py:956-965) + the CO column
   const int iH = N.roles[0], iH2 = N.roles[1], iCp = N.roles[3], iCO = N.roles[5];
   const double xcp = bcast(x, iCp), xe = bcast(x, N.e_idx), xh = bcast(x, iH),
                xh2 = bcast(x, iH2), xco = bcast(x, iCO);
@@ -1814,8 +1834,37 @@ chem_adjoint_kernel(NetDev N, SteadyCons K, int64_t n, const double *__restrict_
   if (lane == iH) g += w_cii * (base * gh * nh);
   if (lane == iH2) g += w_cii * (base * gh2 * nh);
   if (lane == iCO) g += w_co * nh;
+  CellObjective o;
+  o.g = g;
+  o.lam = lam;
+  o.co = nh * xco;
+  o.explicit_n = w_cii * (lam + base * q) + w_co * (nh * xco);
+  return o;
+}
+
+__global__ void __launch_bounds__(64)
+chem_adjoint_kernel(NetDev N, SteadyCons K, int64_t n, const double *__restrict__ y,
+                    const double *__restrict__ nH, const double *__restrict__ T,
+                    const double *__restrict__ G0, const double *__restrict__ AV,
+                    const double *__restrict__ zeta, const double *__restrict__ fH2,
+                    const double *__restrict__ fCO, CiiLine cl, double w_cii, double w_co,
+                    double *__restrict__ obj, double *__restrict__ grad) {
+#pragma clang fp contract(off)
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  const Lds L = carve(smem, N);
+  const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
+  rate_coeffs(N, p, L, lane);
+  __syncthreads();
+  const double x = lane < N.ns ? y[(int64_t)lane * n + cell] : 0.0;
+  publish_y(N, L, x, lane);
+  jacobian(N, p, L, lane);
+  const SteadyRows rows = steady_rows(N, K, x, lane);
+  steady_matrix(N, K, L, rows, true, lane);
+  const CellObjective ob = cell_objective(N, p, cl, w_cii, w_co, x, lane);
   lu_factor_inplace(N, L, lane);
-  const double lmb = lu_solve(N, L, lane < N.ns ? g : 0.0, lane);
+  const double lmb = lu_solve(N, L, lane < N.ns ? ob.g : 0.0, lane);
   // multipliers of the rows that move with the parameters (the species
   // balances); the constraint and frozen rows do not
   const bool dyn = lane < N.ns && !rows.frozen && rows.pivot_of < 0 && lane != N.e_idx;
@@ -1851,12 +1900,432 @@ chem_adjoint_kernel(NetDev N, SteadyCons K, int64_t n, const double *__restrict_
   sZ = wave_sum(sZ);
   sN = wave_sum(sN);
   if (lane == 0) {
-    const double explicit_n = w_cii * (lam + base * q) + w_co * (nh * xco);
     gout[N.nr] = -sG;
     gout[N.nr + 1] = -sZ;
-    gout[N.nr + 2] = explicit_n - sN;
-    obj[2 * cell] = lam;
-    obj[2 * cell + 1] = nh * xco;
+    gout[N.nr + 2] = ob.explicit_n - sN;
+    obj[2 * cell] = ob.lam;
+    obj[2 * cell + 1] = ob.co;
+  }
+}
+
+// ===========================================================================
+// Discrete adjoint of the implicit steps (bundled network, one cell per
+// wavefront).  The forward map is the BDF integration of a cell from its
+// initial abundances over one span (chemistry/solver.py:185-217) with the
+// step sequence held fixed: the step sizes, orders and step-size changes the
+// integration chose are data, the implicit steps are differentiated exactly.
+//
+// The taped forward (chem_tape_kernel) records per accepted step its
+// c = h / alpha_k, its order k and the product F of the step-size changes
+// applied to the differences before it (changes compose: rescaling by f1
+// then f2 is rescaling by f1 f2), plus the differences D every 16 steps.
+// The adjoint (chem_tape_adjoint_kernel) walks the 16-step segments from the
+// last: it recomputes the segment from its checkpoint (Newton on each
+// implicit step with the recorded c, k, F), stores per step the factors of
+// I - c J(y_n+1) in the column layout, then runs the segment backwards:
+//   d' = adjoint of the difference update, lambda = (I - c J)^-T d',
+//   differences' += lambda - d' (predictor) and -gamma_j / alpha_k lambda
+//   (psi), parameters' += c (df/dtheta)^T lambda, then the adjoint of the
+//   step-size change (D' <- RU D' on rows 0..k);
+// and ends with D[1] = h0 f(y0): parameters' += h0 (df/dtheta(y0))^T D'_1.
+// ===========================================================================
+constexpr int kTapeSeg = 16;     // steps per checkpoint segment
+constexpr int kTapeCol = 33;     // doubles per lane per stored step: 31 factors, 1/u, y
+constexpr double kLn10 = 2.302585092994046;
+
+__constant__ unsigned c_wave_tum[64] = P3D_WAVE_TUMASK;
+__constant__ unsigned c_wave_tlm[64] = P3D_WAVE_TLMASK;
+__constant__ int c_wave_tlev[64] = P3D_WAVE_TLEVELS;
+
+struct TapeDev {   // kernel argument: the tape of n cells
+  double *rec;     // n x cap x 3: per accepted step {c, F, k}
+  double *ck;      // n x (cap / 16 + 1) x kNumD x 32: differences every 16 steps
+  double *hinit;   // n: step size of D[1] = h f(y0)
+  int32_t *nsteps; // n: accepted steps; -1 failed integration, -2 tape too short
+  int cap;         // steps per cell (a multiple of 16)
+};
+
+struct TapeRec {
+  double *rec, *ck;
+  int cap, n;
+  double F, h0;
+  bool over;
+  __device__ __forceinline__ void ckpt(const double (&D)[kNumD], int lane) {
+    double *c = ck + (size_t)(n / kTapeSeg) * kNumD * 32;
+    if (lane < 32) {
+#pragma unroll
+      for (int j = 0; j < kNumD; ++j) c[j * 32 + lane] = D[j];
+    }
+  }
+  __device__ __forceinline__ void init(const double (&D)[kNumD], double h, int lane) {
+    F = 1.0;
+    n = 0;
+    over = false;
+    h0 = h;
+    ckpt(D, lane);
+  }
+  __device__ __forceinline__ void change(double f) { F *= f; }
+  __device__ __forceinline__ void accept(int order, double c, const double (&D)[kNumD],
+                                         int lane) {
+    if (n < cap) {
+      const double v = lane == 0 ? c : (lane == 1 ? F : (double)order);
+      if (lane < 3) rec[3 * n + lane] = v;
+    } else {
+      over = true;
+    }
+    F = 1.0;
+    ++n;
+    if (n % kTapeSeg == 0 && n < cap) ckpt(D, lane);
+  }
+};
+
+__device__ __forceinline__ int64_t next_cell(unsigned long long *ctr) {
+  unsigned long long c = 0;
+  if (threadIdx.x == 0) c = atomicAdd(ctr, 1ull);
+  c = __shfl(c, 0, 64);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)c);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(c >> 32));
+  return (int64_t)(((unsigned long long)hi << 32) | lo);
+}
+
+// Taped forward: the one-span integration of chem_queue_kernel (same
+// solve, same results) with the step sequence recorded.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+chem_tape_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
+                 const double *__restrict__ T, const double *__restrict__ G0,
+                 const double *__restrict__ AV, const double *__restrict__ zeta,
+                 const double *__restrict__ fH2, const double *__restrict__ fCO,
+                 p3d_solve_opts o, TapeDev TD, int32_t *__restrict__ stats_out,
+                 unsigned long long *__restrict__ ctr) {
+  extern __shared__ double smem[];
+  const Lds L = carve(smem, N, true, true);
+  stage_tables(N, L, threadIdx.x);
+  const int lane = threadIdx.x;
+  for (;;) {
+    const int64_t cell = next_cell(ctr);
+    if (cell >= n) break;  // scalar branch: every lane leaves together
+    const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
+    const bool act = lane < N.ns;
+    double yc = act ? y[(int64_t)lane * n + cell] : 0.0;
+    rate_coeffs<64>(N, p, L, lane, o.rate_fp32 != 0);
+    __syncthreads();
+    RxCache rc;
+    rc.load(N, L, lane);
+    LinStatic lin;
+    double tmax = o.t_max;
+    if (tmax <= 0.0) tmax = fmin(3e17, fmax(1e16, 3e17 / fmax(p.nH, 1.0)));
+    TapeRec rec;
+    rec.rec = TD.rec + (size_t)cell * TD.cap * 3;
+    rec.ck = TD.ck + (size_t)cell * (TD.cap / kTapeSeg + 1) * kNumD * 32;
+    rec.cap = TD.cap;
+    BdfStats st = {0, 0, 0, 0, 0};
+    double yn = yc;
+    const int status = bdf_integrate_rec<LinStatic, Wave1, TapeRec>(
+        N, p, L, lin, rc, lane, yn, o.t_start, tmax, o.rtol, o.atol, tmax - o.t_start,
+        o.max_steps, st, rec);
+    if (status == 0) yc = fmax(yn, 1e-30);
+    if (act) y[(int64_t)lane * n + cell] = yc;
+    if (lane == 0) {
+      TD.nsteps[cell] = status != 0 ? -1 : (rec.over ? -2 : rec.n);
+      TD.hinit[cell] = rec.h0;
+      if (stats_out) {
+        stats_out[4 * cell + 0] = st.steps;
+        stats_out[4 * cell + 1] = st.lu;
+        stats_out[4 * cell + 2] = st.rhs;
+        stats_out[4 * cell + 3] = st.jac;
+      }
+    }
+    __syncthreads();  // LDS of this cell is rewritten by the next
+  }
+}
+
+// Adjoint of change_D: D' <- RU D' on rows 0..order (change_D applies RU^T).
+__device__ void change_D_adj(const Lds &L, int lane, double (&Db)[kNumD], int order,
+                             double factor) {
+  if (lane < 36) {
+    const int i = lane / 6, j = lane % 6;
+    double acc = 0.0;
+    if (i <= order && j <= order) {
+      for (int m = 0; m <= order; ++m) {
+        double r = 1.0, u = 1.0;
+        for (int q = 1; q <= i; ++q) r *= ((double)(q - 1) - factor * (double)m) / (double)q;
+        for (int q = 1; q <= m; ++q) u *= ((double)(q - 1) - (double)j) / (double)q;
+        acc += r * u;
+      }
+    }
+    L.ru[lane] = acc;
+  }
+  __syncthreads();
+  double Dn[kMaxOrder + 1];
+#pragma unroll
+  for (int i = 0; i <= kMaxOrder; ++i) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j <= kMaxOrder; ++j) acc += L.ru[6 * i + j] * Db[j];
+    Dn[i] = acc;
+    __asm__ volatile("" ::: "memory");
+  }
+#pragma unroll
+  for (int i = 0; i <= kMaxOrder; ++i)
+    if (i <= order) Db[i] = Dn[i];
+  __syncthreads();
+}
+
+// w (df/dtheta(y))^T lambda accumulated per lane: reaction lane + 64 q into
+// gk[q] (d/dlog10 k), and the lane's shares of d/dln G0, d/dln zeta and
+// d/dln n_H (the rates' explicit dependence, the PAH charge balance's
+// through psi = G0 sqrt(T) / n_e included).  Requires ys published (y) and
+// psum[s] = lambda_s (0 past the species).
+__device__ void param_adjoint(const NetDev &N, const CellParams &p, const Lds &L, int lane,
+                              double w, double (&gk)[kRxPerLane], double &sG, double &sZ,
+                              double &sN) {
+  eval_reactions<64>(N, p, L, lane, false);  // rates into part[2r] (syncs)
+  const double yef = fmax(L.ys[N.e_idx], 1e-10);
+  const double ne = yef * p.nH;
+  const double psi = (p.G0 > 0.0 && ne > 0.0) ? p.g0_sqrt_t / ne : -1.0;
+  const double fp = psi > 0.0 ? psi / (psi + 500.0) : 0.0;
+#pragma unroll
+  for (int q = 0; q < kRxPerLane; ++q) {
+    const int r = lane + 64 * q;
+    if (r >= N.nr) continue;
+    const int *s5 = N.st5 + 5 * r;
+    double wr = 0.0;
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+      if (s5[m] >= 0) wr -= L.psum[s5[m]];
+#pragma unroll
+    for (int m = 2; m < 5; ++m)
+      if (s5[m] >= 0) wr += L.psum[s5[m]];
+    const double v = w * (wr * L.part[2 * r]);
+    gk[q] += kLn10 * v;
+    const int t = N.rtype[r];
+    const bool pah = N.react[2 * r + 1] == P3D_SP_PAH_MINUS;
+    const bool photo = t == P3D_RX_PHOTO || t == P3D_RX_PHOTO_FH2 || t == P3D_RX_PHOTO_FCO;
+    sG += v * ((photo ? 1.0 : 0.0) - (pah && t == P3D_RX_TWO_BODY ? fp : 0.0));
+    sZ += v * (t == P3D_RX_CR ? 1.0 : 0.0);
+    sN += v * (t == P3D_RX_TWO_BODY ? 1.0 + (pah ? fp : 0.0) : (t == P3D_RX_GRAIN ? 1.0 : 0.0));
+  }
+  __syncthreads();  // part[] and psum[] are rewritten next
+}
+
+__global__ void __launch_bounds__(64)
+chem_tape_adjoint_kernel(NetDev N, int64_t n, const double *__restrict__ y0,
+                         const double *__restrict__ yend, const double *__restrict__ nH,
+                         const double *__restrict__ T, const double *__restrict__ G0,
+                         const double *__restrict__ AV, const double *__restrict__ zeta,
+                         const double *__restrict__ fH2, const double *__restrict__ fCO,
+                         p3d_solve_opts o, TapeDev TD, CiiLine cl, double w_cii, double w_co,
+                         double *__restrict__ obj, double *__restrict__ grad,
+                         int32_t *__restrict__ status_out, double *__restrict__ segbuf,
+                         unsigned long long *__restrict__ ctr) {
+  extern __shared__ double smem[];
+  const Lds L = carve(smem, N, true, true);
+  stage_tables(N, L, threadIdx.x);
+  const int lane = threadIdx.x;
+  const bool act = lane < N.ns;
+  const bool row = lane < wave::kNS;
+  double *seg = segbuf + (size_t)blockIdx.x * kTapeSeg * kTapeCol * 64;
+  const unsigned tlm = row ? c_wave_tlm[lane] : 0u, tum = row ? c_wave_tum[lane] : 0u;
+  const int tlev = c_wave_tlev[lane];
+  auto gam = [](int k) { return c_gam[k]; };
+  auto alp = [](int k) { return (1.0 - c_kappa[k]) * c_gam[k]; };
+  const double newton_tol =
+      fmax(10.0 * 2.220446049250313e-16 / o.rtol, fmin(0.03, sqrt(o.rtol)));
+  for (;;) {
+    const int64_t cell = next_cell(ctr);
+    if (cell >= n) break;
+    const CellParams p = load_params(cell, nH, T, G0, AV, zeta, fH2, fCO);
+    rate_coeffs<64>(N, p, L, lane, o.rate_fp32 != 0);
+    __syncthreads();
+    RxCache rc;
+    rc.load(N, L, lane);
+    LinStatic lin;
+    const double xe = act ? yend[(int64_t)lane * n + cell] : 0.0;
+    const CellObjective ob = cell_objective(N, p, cl, w_cii, w_co, xe, lane);
+    double Db[kNumD];
+#pragma unroll
+    for (int j = 0; j < kNumD; ++j) Db[j] = 0.0;
+    Db[0] = (act && xe > 1e-30) ? ob.g : 0.0;  // y_end = max(D0, 1e-30)
+    double gk[kRxPerLane] = {0.0, 0.0};
+    double sG = 0.0, sZ = 0.0, sN = 0.0;
+    const int nst = TD.nsteps[cell];
+    int failed = nst < 0 ? nst : 0;
+    if (nst >= 0) {
+      const double *rec = TD.rec + (size_t)cell * TD.cap * 3;
+      const double *ck = TD.ck + (size_t)cell * (TD.cap / kTapeSeg + 1) * kNumD * 32;
+      const int nseg = (nst + kTapeSeg - 1) / kTapeSeg;
+      for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int i0 = sg * kTapeSeg, i1 = min(i0 + kTapeSeg, nst);
+        double D[kNumD];
+#pragma unroll
+        for (int j = 0; j < kNumD; ++j) D[j] = act ? ck[(sg * kNumD + j) * 32 + lane] : 0.0;
+        // recompute the segment; L.J holds J at the latest state, L.M the
+        // factors of I - c_lu J
+        double c_lu = -1.0;
+        bool jac_ok = false;
+        for (int i = i0; i < i1; ++i) {
+          const double c = rec[3 * i], F = rec[3 * i + 1];
+          const int k = (int)rec[3 * i + 2];
+          if (F != 1.0) change_D(L, lane, D, k, F);
+          double ypred = 0.0, psi = 0.0;
+#pragma unroll
+          for (int j = 0; j < kNumD; ++j)
+            if (j <= k) ypred += D[j];
+#pragma unroll
+          for (int j = 1; j < kNumD; ++j)
+            if (j <= k) psi += gam(j) * D[j];
+          psi /= alp(k);
+          const double scale = o.atol + o.rtol * fabs(ypred);
+          if (!jac_ok) {
+            publish_y(N, L, ypred, lane);
+            jac_eval<Wave1>(N, p, L, lane);
+            jac_ok = true;
+            c_lu = -1.0;
+          }
+          // Newton on the recorded implicit step.  Species at the abundance
+          // floor make the equations non-smooth at the atol level, so the
+          // iteration may stall there; the recomputed segment is checked
+          // against the next checkpoint instead (below).
+          double d = 0.0, ynew = ypred;
+          bool conv = false;
+          for (int round = 0; round < 2 && !conv; ++round) {
+            if (c != c_lu) {
+              lin.factor(N, L, c, lane);
+              c_lu = c;
+            }
+            double dy_old = -1.0, rate = -1.0;
+            for (int it = 0; it < 4; ++it) {
+              publish_y(N, L, ynew, lane);
+              const double f = rhs_pieces(N, p, L, rc, lin.pw0, lin.pw1, lane);
+              double dy = act ? c * f - psi - d : 0.0;
+              dy = lin.solve(N, L, dy, lane);
+              if (!act) dy = 0.0;
+              const double dyn = sqrt(wave_sum(act ? (dy / scale) * (dy / scale) : 0.0) / N.ns);
+              if (dy_old >= 0.0) rate = dyn / dy_old;
+              // the step was accepted by the forward: a correction far below
+              // the Newton tolerance ends the iteration even when rounding
+              // makes the rate estimate meaningless
+              const bool tiny = dyn < 1e-3 * newton_tol;
+              if (rate >= 1.0 && !tiny) break;
+              ynew += dy;
+              d += dy;
+              if (dyn == 0.0 || tiny ||
+                  (rate >= 0.0 && rate / (1.0 - rate) * dyn < newton_tol)) {
+                conv = true;
+                break;
+              }
+              dy_old = dyn;
+            }
+            if (!conv) {  // refresh the Jacobian at the current iterate
+              publish_y(N, L, ynew, lane);
+              jac_eval<Wave1>(N, p, L, lane);
+              c_lu = -1.0;
+            }
+          }
+          // the difference update of the step (as bdf_integrate)
+#pragma unroll
+          for (int j = kNumD - 1; j >= 2; --j)
+            if (j == k + 2) D[j] = d - D[j - 1];
+#pragma unroll
+          for (int j = kNumD - 1; j >= 1; --j)
+            if (j == k + 1) D[j] = d;
+#pragma unroll
+          for (int j = kNumD - 2; j >= 0; --j)
+            if (j <= k) D[j] += D[j + 1];
+          // factors of I - c J(y_n+1): kept in L.M for the next step's
+          // Newton, stored in the column layout for the backward pass
+          publish_y(N, L, D[0], lane);
+          jac_eval<Wave1>(N, p, L, lane);
+          lin.factor(N, L, c, lane);
+          c_lu = c;
+          double *sp = seg + (size_t)(i - i0) * kTapeCol * 64;
+#pragma unroll
+          for (int q = 0; q < wave::kNS; ++q)
+            sp[q * 64 + lane] = row ? L.M[wave::kPerm[q] * L.ld + lin.pos] : 0.0;
+          sp[31 * 64 + lane] = row ? L.M[lane * L.ld + lin.pos] : 0.0;
+          sp[32 * 64 + lane] = D[0];
+        }
+        // the recomputed segment must land on the forward's state: the next
+        // checkpoint, or the integration's output after the last segment
+        {
+          const bool last = i1 >= nst;
+          const double yref = last ? xe : (act ? ck[((sg + 1) * kNumD) * 32 + lane] : 0.0);
+          const double ycur = last ? fmax(D[0], 1e-30) : D[0];
+          const double dev =
+              (act && fabs(yref) > 1e-14) ? fabs(ycur - yref) / fabs(yref) : 0.0;
+          if (wave_max(dev) > 1e-6 && failed == 0) failed = -3;
+        }
+        // backwards through the segment
+        for (int i = i1 - 1; i >= i0; --i) {
+          const double c = rec[3 * i], F = rec[3 * i + 1];
+          const int k = (int)rec[3 * i + 2];
+          // adjoint of the difference update: D'_k+1 = d, D'_k+2 = d - D_k+1,
+          // D'_j = D_j + ... + D_k + d (j <= k)
+          double db = 0.0, cum = 0.0, Dn[kNumD];
+#pragma unroll
+          for (int j = 0; j < kNumD; ++j) {
+            if (j <= k) {
+              cum += Db[j];
+              Dn[j] = cum;
+            } else if (j == k + 1) {
+              Dn[j] = j + 1 < kNumD ? -Db[j + 1 < kNumD ? j + 1 : j] : 0.0;
+            } else if (j == k + 2) {
+              Dn[j] = 0.0;
+            } else {
+              Dn[j] = Db[j];
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < kNumD; ++j)
+            if (j <= k + 2) db += Db[j];
+          // lambda = (I - c J)^-T d'
+          const double *sp = seg + (size_t)(i - i0) * kTapeCol * 64;
+          double at[32];
+#pragma unroll
+          for (int q = 0; q < 32; ++q) at[q] = (q < wave::kNS) ? sp[q * 64 + lane] : 0.0;
+          const double dinv = sp[31 * 64 + lane];
+          const double ynew = sp[32 * 64 + lane];
+          double lam = wave::lu_solve_t(at, tlm, tum, lane, act ? db : 0.0, WaveBcast{}, L.xs,
+                                        lin.pos, tlev, dinv);
+          if (!act) lam = 0.0;
+#pragma unroll
+          for (int j = 0; j < kNumD; ++j) {
+            if (j <= k) Dn[j] += lam - db;
+            if (j >= 1 && j <= k) Dn[j] -= gam(j) / alp(k) * lam;
+          }
+          // parameters: c (df/dtheta)^T lambda at y_n+1
+          publish_y(N, L, ynew, lane);
+          L.psum[lane] = lam;
+          __syncthreads();
+          param_adjoint(N, p, L, lane, c, gk, sG, sZ, sN);
+#pragma unroll
+          for (int j = 0; j < kNumD; ++j) Db[j] = Dn[j];
+          if (F != 1.0) change_D_adj(L, lane, Db, k, F);
+        }
+      }
+      // D[1] = h0 f(y0)
+      const double yi = act ? y0[(int64_t)lane * n + cell] : 0.0;
+      publish_y(N, L, yi, lane);
+      L.psum[lane] = act ? Db[1] : 0.0;
+      __syncthreads();
+      param_adjoint(N, p, L, lane, TD.hinit[cell], gk, sG, sZ, sN);
+    }
+    sG = wave_sum(sG);
+    sZ = wave_sum(sZ);
+    sN = wave_sum(sN);
+    double *gout = grad + cell * (int64_t)(N.nr + 3);
+#pragma unroll
+    for (int q = 0; q < kRxPerLane; ++q)
+      if (lane + 64 * q < N.nr) gout[lane + 64 * q] = gk[q];
+    if (lane == 0) {
+      gout[N.nr] = sG;
+      gout[N.nr + 1] = sZ;
+      gout[N.nr + 2] = ob.explicit_n + sN;
+      obj[2 * cell] = ob.lam;
+      obj[2 * cell + 1] = ob.co;
+      if (status_out) status_out[cell] = failed;
+    }
+    __syncthreads();
   }
 }
 
@@ -1956,6 +2425,71 @@ static int queue_solve(Network *net, int64_t n, double *y, const double *n_h,
   return launch_status("chem_queue_kernel");
 }
 
+// Tape workspace of n cells at cap steps per cell: step records, checkpoints,
+// initial step sizes, step counts (8-byte aligned sections).
+static size_t tape_section(int64_t n, int cap, int k) {
+  const size_t rec = (size_t)n * cap * 3 * 8;
+  const size_t ck = (size_t)n * (cap / kTapeSeg + 1) * kNumD * 32 * 8;
+  const size_t h = (size_t)n * 8;
+  const size_t ns = ((size_t)n * 4 + 7) / 8 * 8;
+  const size_t sz[4] = {rec, ck, h, ns};
+  size_t off = 0;
+  for (int i = 0; i < k; ++i) off += sz[i];
+  return off;
+}
+static size_t tape_bytes_of(int64_t n, int cap) { return tape_section(n, cap, 4); }
+static TapeDev tape_view(void *ws, int64_t n, int cap) {
+  char *b = static_cast<char *>(ws);
+  TapeDev t;
+  t.rec = reinterpret_cast<double *>(b + tape_section(n, cap, 0));
+  t.ck = reinterpret_cast<double *>(b + tape_section(n, cap, 1));
+  t.hinit = reinterpret_cast<double *>(b + tape_section(n, cap, 2));
+  t.nsteps = reinterpret_cast<int32_t *>(b + tape_section(n, cap, 3));
+  t.cap = cap;
+  return t;
+}
+
+static int tape_checks(Network *net, int64_t n, const p3d_solve_opts *o, const void *tape,
+                       int64_t tape_bytes, int32_t cap, const char *what) {
+  P3D_REQUIRE(net->lane_ok,
+              "%s: the discrete adjoint runs on the bundled network's register LU only", what);
+  P3D_REQUIRE(o, "%s: solve options are required", what);
+  P3D_REQUIRE(o->n_checkpoints == 2 && o->early_stop == 0 && o->post_mode == -1,
+              "%s: the taped map is one span without post-solve corrections "
+              "(n_checkpoints 2, early_stop 0, post_mode -1)", what);
+  P3D_REQUIRE(o->rate_fp32 == 0, "%s: the taped map evaluates the rates in FP64", what);
+  P3D_REQUIRE(cap >= kTapeSeg && cap % kTapeSeg == 0 && cap <= (1 << 24),
+              "%s: steps per cell must be a multiple of %d in [%d, 2^24] (got %d)", what,
+              kTapeSeg, kTapeSeg, cap);
+  P3D_REQUIRE(tape && tape_bytes >= 0 && (size_t)tape_bytes >= tape_bytes_of(n, cap),
+              "%s: the tape needs %zu bytes for %lld cells at %d steps (got %lld)", what,
+              tape_bytes_of(n, cap), (long long)n, cap, (long long)tape_bytes);
+  return P3D_OK;
+}
+
+// Resident wavefronts of a persistent one-cell-per-wavefront kernel.
+static int resident_waves(const void *kernel, size_t sh, int64_t n, int64_t *waves) {
+  int dev = 0, cus = 0, per_cu = 0;
+  P3D_CHECK_HIP(hipGetDevice(&dev));
+  P3D_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  P3D_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, sh));
+  per_cu = std::max(per_cu, 1);
+  *waves = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)cus * per_cu));
+  return P3D_OK;
+}
+
+static int ensure_queue_ctr(Network *net) {
+  if (!net->queue_ctr) {
+    if (hipMalloc(&net->queue_ctr, 256) != hipSuccess) {
+      (void)hipGetLastError();
+      net->queue_ctr = nullptr;
+      set_error("hipMalloc of the work counter failed");
+      return P3D_ENOMEM;
+    }
+  }
+  return P3D_OK;
+}
+
 }  // namespace p3d
 
 using namespace p3d;
@@ -2010,6 +2544,15 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
       int s = products[3 * r + m];
       if (s >= 0) rows[s].push_back({r, +1.0});
     }
+  }
+  // the same occurrences per reaction: consumed (2 slots), produced (3)
+  std::vector<int> st5(5 * (size_t)nr, -1);
+  for (int r = 0; r < nr; ++r) {
+    const int slots = rtype[r] == P3D_RX_TWO_BODY || rtype[r] == P3D_RX_GRAIN ? 2 : 1;
+    for (int m = 0; m < slots; ++m)
+      if (reactants[2 * r + m] >= 0) st5[5 * r + m] = reactants[2 * r + m];
+    for (int m = 0; m < 3; ++m)
+      if (products[3 * r + m] >= 0) st5[5 * r + 2 + m] = products[3 * r + m];
   }
   std::vector<int> rhs_off(ns + 1, 0), rhs_rxn, rhs_pk;
   std::vector<double> rhs_sgn;
@@ -2098,7 +2641,7 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
          o_tr = put(jt_rxn.data(), 4 * jt_rxn.size()), o_ts = put(jt_slot.data(), 4 * jt_slot.size()),
          o_tc = put(jt_coef.data(), 8 * jt_coef.size()), o_pw = put(post.data(), 8 * post.size()),
          o_pb = put(pc_beg.data(), 4 * pc_beg.size()), o_pl = put(pc_lane.data(), 4 * pc_lane.size()),
-         o_pr = put(pc_row.data(), 4 * pc_row.size());
+         o_pr = put(pc_row.data(), 4 * pc_row.size()), o_s5 = put(st5.data(), 4 * st5.size());
   auto net = std::make_unique<Network>();
   if (hipMalloc(&net->blob, host.size()) != hipSuccess) {
     set_error("hipMalloc of %zu bytes for network tables failed", host.size());
@@ -2115,6 +2658,7 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   d.gamma = (const double *)(b + o_g);
   d.react = (const int *)(b + o_re);
   d.dep = (const int *)(b + o_dep);
+  d.st5 = (const int *)(b + o_s5);
   d.rhs_off = (const int *)(b + o_ro);
   d.rhs_rxn = (const int *)(b + o_rr);
   d.rhs_sgn = (const double *)(b + o_rs);
@@ -2249,6 +2793,89 @@ int p3d_chem_adjoint(int64_t handle, int64_t n, const double *y, const double *n
                      net->d, K, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, cl, w_cii, w_co,
                      obj, grad);
   return launch_status("chem_adjoint_kernel");
+}
+
+int p3d_chem_tape_bytes(int64_t n, int32_t cap, int64_t *bytes) {
+  P3D_REQUIRE(bytes, "bytes pointer is required");
+  P3D_REQUIRE(n >= 0, "n must be non-negative");
+  P3D_REQUIRE(cap >= kTapeSeg && cap % kTapeSeg == 0 && cap <= (1 << 24),
+              "steps per cell must be a multiple of %d in [%d, 2^24] (got %d)", kTapeSeg,
+              kTapeSeg, cap);
+  *bytes = (int64_t)tape_bytes_of(n, cap);
+  return P3D_OK;
+}
+
+int p3d_chem_solve_tape(int64_t handle, int64_t n, double *y, const double *n_h,
+                        const double *t_gas, const double *g0, const double *a_v,
+                        const double *zeta, const double *fsh_h2, const double *fsh_co,
+                        const p3d_solve_opts *opts, void *tape, int64_t tape_bytes, int32_t cap,
+                        int32_t *stats, void *stream) {
+  Network *net = find_net(handle);
+  int rc = check_cells(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
+  if (rc != P3D_OK || n == 0) return rc;
+  if ((rc = tape_checks(net, n, opts, tape, tape_bytes, cap, "p3d_chem_solve_tape")) != P3D_OK)
+    return rc;
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e, true, true);
+  if ((rc = ensure_queue_ctr(net)) != P3D_OK) return rc;
+  int64_t waves = 0;
+  if ((rc = resident_waves(reinterpret_cast<const void *>(chem_tape_kernel), sh, n, &waves)) !=
+      P3D_OK)
+    return rc;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, net->queue_ctr);
+  hipLaunchKernelGGL(chem_tape_kernel, dim3((unsigned)waves), dim3(64), sh, s, net->d, n, y, n_h,
+                     t_gas, g0, a_v, zeta, fsh_h2, fsh_co, *opts, tape_view(tape, n, cap), stats,
+                     net->queue_ctr);
+  return launch_status("chem_tape_kernel");
+}
+
+int p3d_chem_tape_adjoint(int64_t handle, int64_t n, const double *y0, const double *y_end,
+                          const double *n_h, const double *t_gas, const double *g0,
+                          const double *a_v, const double *zeta, const double *fsh_h2,
+                          const double *fsh_co, const p3d_solve_opts *opts, const void *tape,
+                          int64_t tape_bytes, int32_t cap, const double *cii5, double w_cii,
+                          double w_co, double *obj, double *grad, int32_t *status,
+                          void *stream) {
+  Network *net = find_net(handle);
+  int rc = check_cells(net, n, y_end, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co);
+  if (rc != P3D_OK || n == 0) return rc;
+  if ((rc = tape_checks(net, n, opts, tape, tape_bytes, cap, "p3d_chem_tape_adjoint")) != P3D_OK)
+    return rc;
+  P3D_REQUIRE(y0, "the initial abundances y0 are required");
+  P3D_REQUIRE(cii5, "[CII] line parameters (host, 5 doubles) are required");
+  P3D_REQUIRE(obj && grad, "null output");
+  for (int k = 0; k < 7; ++k)
+    P3D_REQUIRE(k == 2 || k == 4 || k == 6 || net->d.roles[k] >= 0,
+                "the network lacks a species the objective needs (H, H2, C+, CO)");
+  CiiLine cl;
+  cl.freq = cii5[0]; cl.A = cii5[1]; cl.Tul = cii5[2]; cl.gu = cii5[3]; cl.gl = cii5[4];
+  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e, true, true);
+  if ((rc = ensure_queue_ctr(net)) != P3D_OK) return rc;
+  int64_t waves = 0;
+  if ((rc = resident_waves(reinterpret_cast<const void *>(chem_tape_adjoint_kernel), sh, n,
+                           &waves)) != P3D_OK)
+    return rc;
+  // per resident wavefront: the factors, 1/u and state of one segment's steps
+  const size_t seg_bytes = (size_t)waves * kTapeSeg * kTapeCol * 64 * 8;
+  if (net->seg_bytes < seg_bytes) {
+    if (net->segbuf) (void)hipFree(net->segbuf);
+    net->segbuf = nullptr;
+    net->seg_bytes = 0;
+    if (hipMalloc(&net->segbuf, seg_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      net->segbuf = nullptr;
+      set_error("hipMalloc of %zu bytes for the adjoint's segment buffer failed", seg_bytes);
+      return P3D_ENOMEM;
+    }
+    net->seg_bytes = seg_bytes;
+  }
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, net->queue_ctr);
+  hipLaunchKernelGGL(chem_tape_adjoint_kernel, dim3((unsigned)waves), dim3(64), sh, s, net->d, n,
+                     y0, y_end, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, *opts,
+                     tape_view(const_cast<void *>(tape), n, cap), cl, w_cii, w_co, obj, grad,
+                     status, static_cast<double *>(net->segbuf), net->queue_ctr);
+  return launch_status("chem_tape_adjoint_kernel");
 }
 
 int p3d_chem_post(int64_t handle, int64_t n, double *y, const double *n_h,

@@ -662,5 +662,202 @@ __device__ __forceinline__ double lu_solve(const double (&a)[32], unsigned lm, u
   return xs[pos];
 }
 
+// Transposed solve: per species (lane), bit i where its column of the
+// factors holds a U^T (upper of the lane's row of U^T) or L^T entry, and
+// the forward | backward << 8 levels of the transposed solves
+#define P3D_WAVE_TUMASK {0x00ff7bfeu, 0x01882480u, 0x03ff5801u, 0x00000400u, 0x00000000u, 0x00000000u, 0x000f0000u, 0x002ea000u, 0x07fa8000u, 0x0ff40300u, 0x00040000u, 0x00000800u, 0x00000000u, 0x1ffda400u, 0x00000000u, 0x0060a000u, 0x00004000u, 0x00040000u, 0x00001000u, 0x00000000u, 0x00000000u, 0x3f042000u, 0x00000000u, 0x00000000u, 0x00000002u, 0x00000000u, 0x00000008u, 0x00000000u, 0x00000020u, 0x00000000u, 0x00000100u}
+#define P3D_WAVE_TLMASK {0x7e000000u, 0x7c000000u, 0x78000000u, 0x23000000u, 0x07000000u, 0x06100000u, 0x7fa00000u, 0x7f800000u, 0x70000000u, 0x60000000u, 0x7f000000u, 0x0f080000u, 0x0b080000u, 0x40000000u, 0x22002000u, 0x7f000000u, 0x27a00000u, 0x5bc00000u, 0x27a00000u, 0x03a00000u, 0x78400000u, 0x00000000u, 0x53000000u, 0x01000004u, 0x01000000u, 0x01000010u, 0x01000000u, 0x01000040u, 0x01000000u, 0x01000200u, 0x31400000u}
+#define P3D_WAVE_TLEVELS {1797, 1542, 1287, 2050, 2049, 3073, 2820, 2307, 1032, 777, 2050, 2306, 2305, 522, 2305, 2052, 2819, 2562, 2818, 2817, 2561, 267, 2049, 2305, 2050, 2305, 2050, 2305, 2050, 2817, 2562, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}
+
+// Solve M^T x = b with the factors of lu_factor in the column layout: a[i]
+// is entry (i, pos) of the factored matrix in elimination positions, i.e.
+// u_(i,pos) above the diagonal, l_(i,pos) below it and 1 / u_(pos,pos) on
+// it (dinv).  U^T z = b runs level by level with the division, then
+// L^T x = z; components are recorded in xs as in lu_solve, and the
+// components that could reach a lane that is not final keep a mask (um:
+// the lane's U^T entries, lm: its L^T entries).
+template <class BC>
+__device__ __forceinline__ double lu_solve_t(const double (&a)[32], unsigned lm, unsigned um,
+                                            int lane, double b, const BC &bc, double *xs,
+                                            int pos, int lev, double dinv) {
+  const int lvf = lev & 0xff, lvb = lev >> 8;
+  {
+    const double bx = b * dinv;
+    if (lvf == 1) xs[pos] = bx;
+    const double x0 = bc(bx, 5);
+    const double x1 = bc(bx, 23);
+    const double x3 = bc(bx, 25);
+    const double x5 = bc(bx, 27);
+    const double x7 = bc(bx, 4);
+    const double x8 = bc(bx, 29);
+    const double x10 = bc(bx, 14);
+    const double x11 = bc(bx, 12);
+    const double x12 = bc(bx, 19);
+    const double x15 = bc(bx, 20);
+    const double x18 = bc(bx, 22);
+    b -= a[0] * x0;
+    b -= a[1] * x1;
+    b -= a[3] * x3;
+    b -= a[5] * x5;
+    b -= a[7] * x7;
+    b -= a[8] * x8;
+    b -= a[10] * x10;
+    b -= a[11] * x11;
+    b -= a[12] * x12;
+    b -= a[15] * x15;
+    b -= a[18] * x18;
+  }
+  {
+    const double bx = b * dinv;
+    if (lvf == 2) xs[pos] = bx;
+    const double x2 = bc(bx, 24);
+    const double x4 = bc(bx, 26);
+    const double x6 = bc(bx, 28);
+    const double x9 = bc(bx, 30);
+    const double x13 = bc(bx, 3);
+    const double x14 = bc(bx, 18);
+    const double x17 = bc(bx, 11);
+    const double x19 = bc(bx, 10);
+    const double x21 = bc(bx, 17);
+    b -= a[2] * x2;
+    b -= a[4] * x4;
+    b -= a[6] * x6;
+    b -= a[9] * x9;
+    b -= a[13] * x13;
+    b -= a[14] * x14;
+    b -= a[17] * x17;
+    b -= a[19] * x19;
+    if ((um >> 21) & 1u) b -= a[21] * x21;
+  }
+  {
+    const double bx = b * dinv;
+    if (lvf == 3) xs[pos] = bx;
+    const double x16 = bc(bx, 16);
+    const double x22 = bc(bx, 7);
+    b -= a[16] * x16;
+    b -= a[22] * x22;
+  }
+  {
+    const double bx = b * dinv;
+    if (lvf == 4) xs[pos] = bx;
+    const double x20 = bc(bx, 6);
+    const double x23 = bc(bx, 15);
+    b -= a[20] * x20;
+    b -= a[23] * x23;
+  }
+  {
+    const double bx = b * dinv;
+    if (lvf == 5) xs[pos] = bx;
+    const double x24 = bc(bx, 0);
+    b -= a[24] * x24;
+  }
+  {
+    const double bx = b * dinv;
+    if (lvf == 6) xs[pos] = bx;
+    const double x25 = bc(bx, 1);
+    b -= a[25] * x25;
+  }
+  {
+    const double bx = b * dinv;
+    if (lvf == 7) xs[pos] = bx;
+    const double x26 = bc(bx, 2);
+    b -= a[26] * x26;
+  }
+  {
+    const double bx = b * dinv;
+    if (lvf == 8) xs[pos] = bx;
+    const double x27 = bc(bx, 8);
+    b -= a[27] * x27;
+  }
+  {
+    const double bx = b * dinv;
+    if (lvf == 9) xs[pos] = bx;
+    const double x28 = bc(bx, 9);
+    b -= a[28] * x28;
+  }
+  {
+    const double bx = b * dinv;
+    if (lvf == 10) xs[pos] = bx;
+    const double x29 = bc(bx, 13);
+    b -= a[29] * x29;
+  }
+  {
+    const double bx = b * dinv;
+    if (lvf == 11) xs[pos] = bx;
+  }
+  b = xs[pos];
+  {
+    if (lvb == 1) xs[pos] = b;
+    const double x30 = bc(b, 21);
+    b -= a[30] * x30;
+  }
+  {
+    if (lvb == 2) xs[pos] = b;
+    const double x29 = bc(b, 13);
+    b -= a[29] * x29;
+  }
+  {
+    if (lvb == 3) xs[pos] = b;
+    const double x28 = bc(b, 9);
+    b -= a[28] * x28;
+  }
+  {
+    if (lvb == 4) xs[pos] = b;
+    const double x27 = bc(b, 8);
+    b -= a[27] * x27;
+  }
+  {
+    if (lvb == 5) xs[pos] = b;
+    const double x26 = bc(b, 2);
+    b -= a[26] * x26;
+  }
+  {
+    if (lvb == 6) xs[pos] = b;
+    const double x25 = bc(b, 1);
+    b -= a[25] * x25;
+  }
+  {
+    if (lvb == 7) xs[pos] = b;
+    const double x24 = bc(b, 0);
+    b -= a[24] * x24;
+  }
+  {
+    if (lvb == 8) xs[pos] = b;
+    const double x23 = bc(b, 15);
+    const double x19 = bc(b, 10);
+    const double x13 = bc(b, 3);
+    const double x6 = bc(b, 28);
+    const double x4 = bc(b, 26);
+    const double x2 = bc(b, 24);
+    b -= a[23] * x23;
+    if ((lm >> 19) & 1u) b -= a[19] * x19;
+    if ((lm >> 13) & 1u) b -= a[13] * x13;
+    b -= a[6] * x6;
+    b -= a[4] * x4;
+    b -= a[2] * x2;
+  }
+  {
+    if (lvb == 9) xs[pos] = b;
+    const double x22 = bc(b, 7);
+    b -= a[22] * x22;
+  }
+  {
+    if (lvb == 10) xs[pos] = b;
+    const double x21 = bc(b, 17);
+    const double x9 = bc(b, 30);
+    b -= a[21] * x21;
+    b -= a[9] * x9;
+  }
+  {
+    if (lvb == 11) xs[pos] = b;
+    const double x20 = bc(b, 6);
+    b -= a[20] * x20;
+  }
+  {
+    if (lvb == 12) xs[pos] = b;
+  }
+  return xs[pos];
+}
+
 }  // namespace wave
 }  // namespace p3d

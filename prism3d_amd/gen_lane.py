@@ -582,6 +582,78 @@ def generate_wave(net=None):
     w("  return xs[pos];")
     w("}")
     w("")
+    # transposed solve M^T x = b in the column layout (lane s holds column
+    # pos(s) of the factors): U^T z = b forward, then L^T x = z backward
+    lev_ft = [0] * ns
+    for q in range(ns):
+        lev_ft[q] = 1 + max([lev_ft[i] for i in range(q) if F[i, q]], default=0)
+    lev_bt = [0] * ns
+    for q in range(ns - 1, -1, -1):
+        lev_bt[q] = 1 + max([lev_bt[i] for i in range(q + 1, ns) if F[i, q]], default=0)
+    ft_mask = [any(F[k, q] and k > q and lev_ft[q] > lev_ft[k] for q in range(ns))
+               for k in range(ns)]
+    bt_mask = [any(F[k, q] and k < q and lev_bt[q] > lev_bt[k] for q in range(ns))
+               for k in range(ns)]
+    tum = [0] * ns  # per species: bit i where its column has a U^T entry (F[i, pos], i < pos)
+    tlm = [0] * ns  # bit i where its column has an L^T entry (F[i, pos], i > pos)
+    for s_ in range(ns):
+        q = pos[s_]
+        for i in range(ns):
+            if F[i, q] and i < q:
+                tum[s_] |= 1 << i
+            if F[i, q] and i > q:
+                tlm[s_] |= 1 << i
+    tlev = [(lev_ft[pos[s_]] | (lev_bt[pos[s_]] << 8)) if s_ < ns else 0 for s_ in range(64)]
+    w("// Transposed solve: per species (lane), bit i where its column of the")
+    w("// factors holds a U^T (upper of the lane's row of U^T) or L^T entry, and")
+    w("// the forward | backward << 8 levels of the transposed solves")
+    w("#define P3D_WAVE_TUMASK {" + ", ".join(f"0x{m:08x}u" for m in tum) + "}")
+    w("#define P3D_WAVE_TLMASK {" + ", ".join(f"0x{m:08x}u" for m in tlm) + "}")
+    w("#define P3D_WAVE_TLEVELS {" + ", ".join(str(x) for x in tlev) + "}")
+    w("")
+    w("// Solve M^T x = b with the factors of lu_factor in the column layout: a[i]")
+    w("// is entry (i, pos) of the factored matrix in elimination positions, i.e.")
+    w("// u_(i,pos) above the diagonal, l_(i,pos) below it and 1 / u_(pos,pos) on")
+    w("// it (dinv).  U^T z = b runs level by level with the division, then")
+    w("// L^T x = z; components are recorded in xs as in lu_solve, and the")
+    w("// components that could reach a lane that is not final keep a mask (um:")
+    w("// the lane's U^T entries, lm: its L^T entries).")
+    w("template <class BC>")
+    w("__device__ __forceinline__ double lu_solve_t(const double (&a)[32], unsigned lm, unsigned um,")
+    w("                                            int lane, double b, const BC &bc, double *xs,")
+    w("                                            int pos, int lev, double dinv) {")
+    w("  const int lvf = lev & 0xff, lvb = lev >> 8;")
+    has_ut = [any(F[k, q] for q in range(k + 1, ns)) for k in range(ns)]  # row k of U has entries
+    has_lt = [any(F[k, q] for q in range(k)) for k in range(ns)]          # row k of L has entries
+    for g in groups(lev_ft):
+        w("  {")
+        w("    const double bx = b * dinv;")
+        w(f"    if (lvf == {lev_ft[g[0]]}) xs[pos] = bx;")
+        gu = [k for k in g if has_ut[k]]
+        for k in gu:
+            w(f"    const double x{k} = bc(bx, {perm[k]});")
+        for k in gu:
+            if ft_mask[k]:
+                w(f"    if ((um >> {k}) & 1u) b -= a[{k}] * x{k};")
+            else:
+                w(f"    b -= a[{k}] * x{k};")
+        w("  }")
+    w("  b = xs[pos];")
+    for g in groups(lev_bt):
+        w("  {")
+        w(f"    if (lvb == {lev_bt[g[0]]}) xs[pos] = b;")
+        gl = [k for k in sorted(g, reverse=True) if has_lt[k]]
+        for k in gl:
+            w(f"    const double x{k} = bc(b, {perm[k]});")
+        for k in gl:
+            if bt_mask[k]:
+                w(f"    if ((lm >> {k}) & 1u) b -= a[{k}] * x{k};")
+            else:
+                w(f"    b -= a[{k}] * x{k};")
+        w("  }")
+    w("  return xs[pos];")
+    w("}")
+    w("")
     w("}  // namespace wave")
     w("}  // namespace p3d")
     return "\n".join(L) + "\n"

@@ -31,8 +31,14 @@ def _parse():
         i = txt.index(start)
         i = txt.index("{\n", i) + 2
         return [l.strip() for l in txt[i:txt.index("\n}\n", i)].split("\n") if l.strip()]
+    tum = [int(x.strip().rstrip("u"), 16) for x in
+           re.search(r"P3D_WAVE_TUMASK \{([^}]*)\}", txt).group(1).split(",")]
+    tlm = [int(x.strip().rstrip("u"), 16) for x in
+           re.search(r"P3D_WAVE_TLMASK \{([^}]*)\}", txt).group(1).split(",")]
+    tlev = [int(x) for x in re.search(r"P3D_WAVE_TLEVELS \{([^}]*)\}", txt).group(1).split(",")]
+    _parse.transposed = (tum, tlm, np.array(tlev), body("double lu_solve_t"))
     return (perm, lmask, umask, np.array(lpos), np.array(llev), body("void lu_factor"),
-            body("double lu_solve"))
+            body("double lu_solve("))
 
 
 def _run(lines, env):
@@ -152,3 +158,20 @@ def test_generated_register_lu_solves_like_lapack(seed):
     x = env["ret"][:ns]
     ref = np.linalg.solve(M, rhs)
     assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref)) * np.linalg.cond(M)
+    # transposed solve M^T x = g in the column layout: lane s holds entry
+    # (i, pos(s)) of the factors at a[i], gathered from the row layout
+    tum, tlm, tlev, solt = _parse.transposed
+    fac_rows = env["a"].copy()
+    at = np.zeros((32, 64))
+    for s in range(ns):
+        for i in range(ns):
+            at[i, s] = fac_rows[lpos[s], perm[i]]
+    g = rng.normal(size=ns)
+    envt = {"a": at, "lane": lanes, "pos": lpos, "lev": tlev, "dinv": dinv,
+            "xs": np.full(32, np.nan), "b": np.concatenate([g, np.zeros(64 - ns)]),
+            "lm": np.array([tlm[s] if s < ns else 0 for s in lanes], dtype=np.int64),
+            "um": np.array([tum[s] if s < ns else 0 for s in lanes], dtype=np.int64)}
+    _run(solt, envt)
+    xt = envt["ret"][:ns]
+    reft = np.linalg.solve(M.T, g)
+    assert np.max(np.abs(xt - reft)) <= 1e-12 * np.max(np.abs(reft)) * np.linalg.cond(M)
