@@ -291,22 +291,30 @@ def forms_same_inputs(solver):
 
 
 # --- differentiation: adjoint / forward time ----------------------------------
-def adjoint_measure(n=32, steps=2):
+def adjoint_measure(n=32, steps=2, steady=True):
     """The differentiation workload: an n^3 log-normal cube (median n_H 1e4,
     sigma_ln 1.0, 0.5 pc), G0 1e3 from -x over 48 rays, zeta 2e-16.  Its
     fields (T, G0, A_V, zeta, shielding) come from the coupled fixed point
     (explicit chemistry, converged); the objective is the summed [CII] 158 um
-    cooling plus the CO column along +x summed over the exit face, at the
-    charge-neutral steady state of every cell.  Forward = tight BDF
-    integration of every cell from the initial abundances + polish; adjoint
-    = the gradient with respect to log10 of every rate coefficient, ln G0,
-    ln zeta and ln density scale (p3d_chem_adjoint + the sum over cells).
-    Both timed with HIP events on the launch stream."""
+    cooling plus the CO column along +x summed over the exit face.
+
+    Implicit-step adjoint (the line's ratio): the forward is the BDF
+    integration of every cell from the initial abundances to its
+    steady-state horizon at rtol 1e-9 (one launch of the one-cell-per-
+    wavefront solve); the gradient with respect to log10 of every rate
+    coefficient, ln G0, ln zeta and ln density scale is the discrete adjoint
+    of the implicit steps over the tape of a taped forward (16-step
+    checkpoints, recomputation, stored factors; chunked when the tapes
+    exceed the memory budget).  Reported: forward, taped forward and adjoint
+    times (HIP events on the launch stream) and adjoint / forward.
+    With steady: the steady-state (implicit-function) gradient at the
+    charge-neutral steady state beside it, a different quantity."""
     import torch
-    from prism3d_amd.adjoint import SteadyStateProblem
-    from prism3d_amd.chemistry import CII_158
+    from prism3d_amd.adjoint import SteadyStateProblem, TransientProblem
+    from prism3d_amd.chemistry import CII_158, solve_opts
     from prism3d_amd.density import PC, lognormal_median
     from prism3d_amd.grid import directional_G0
+    from prism3d_amd.network import TRACKED
     from prism3d_amd.solver3d import PDRSolverGPU
     box = 0.5 * PC
     dens, cell = lognormal_median(n, box, 1e4, 1.0, seed=42)
@@ -318,34 +326,63 @@ def adjoint_measure(n=32, steps=2):
     f = lambda t: t.reshape(-1)
     y0 = {'H': 1.0, 'C+': 1.4e-4, 'O': 3.0e-4, 'e-': 1.4e-4, 'He': 0.0851}
     y0v = np.full(31, 1e-30)
-    from prism3d_amd.network import TRACKED
     for s_, v in y0.items():
         y0v[TRACKED.index(s_)] = v
-    prob = SteadyStateProblem(f(sv.density), f(sv.T_gas), f(sv.G0), f(sv.A_V), f(sv.zeta_CR),
-                              f(sv.fsh_H2), f(sv.fsh_CO), y0v, w_cii=1.0, w_co=cell)
-    t_f, t_a = [], []
+    fields = [f(sv.density), f(sv.T_gas), f(sv.G0), f(sv.A_V), f(sv.zeta_CR), f(sv.fsh_H2),
+              f(sv.fsh_CO)]
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    tp = TransientProblem(*fields, y0v, w_cii=1.0, w_co=cell)
+    t_f, t_t, t_a = [], [], []
     for _ in range(steps):
-        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        y = tp.y0.clone()
+        e0, e1 = ev(), ev()
+        e0.record()
+        tp.base.solve_steady_batch(y, *tp.inputs(None)[1], opts=solve_opts(**tp.opts),
+                                   sync=False)
+        e1.record()
+        e1.synchronize()
+        t_f.append(e0.elapsed_time(e1) * 1e-3)
+        e0, e1, e2 = ev(), ev(), ev()
+        e0.record()
+        J, _ = tp.objective(None)
+        e1.record()
+        g, obj, status, _ = tp.gradient()
+        e2.record()
+        e2.synchronize()
+        t_t.append(e0.elapsed_time(e1) * 1e-3)
+        t_a.append(e1.elapsed_time(e2) * 1e-3)
+    st = status.cpu().numpy()
+    out = {"workload": f"{n}^3 log-normal cube (median n_H 1e4, sigma_ln 1.0, 0.5 pc), G0 1e3 "
+                       "from -x over 48 rays, zeta 2e-16; objective sum Lambda_CII + CO column "
+                       "along +x at the end of every cell's BDF integration to its steady-state "
+                       "horizon (rtol 1e-9)",
+           "mode": "discrete adjoint of the implicit steps (16-step checkpoints, recomputation, "
+                   "stored factors)",
+           "cells": tp.n, "parameters": tp.n_params, "cells_per_tape_chunk": tp.chunk,
+           "forward_s": float(np.median(t_f)), "taped_forward_s": float(np.median(t_t)),
+           "adjoint_s": float(np.median(t_a)),
+           "ratio": float(np.median(t_a) / np.median(t_f)),
+           "ratio_with_taped_forward": float((np.median(t_t) + np.median(t_a)) / np.median(t_f)),
+           "cells_without_gradient": {str(k): int(v) for k, v in
+                                      zip(*np.unique(st[st != 0], return_counts=True))},
+           "objective": float(J), "grad_max_abs": float(g.abs().max())}
+    if steady:
+        prob = SteadyStateProblem(*fields, y0v, w_cii=1.0, w_co=cell)
+        e0, e1, e2 = ev(), ev(), ev()
         e0.record()
         y, cs, ins = prob.state()
         e1.record()
-        obj, grad = cs.steady_adjoint_batch(y, *ins, w_cii=prob.w_cii, w_co=prob.w_co,
-                                            line=CII_158)
-        g = grad.sum(dim=0)
+        cs.steady_adjoint_batch(y, *ins, w_cii=prob.w_cii, w_co=prob.w_co, line=CII_158)
         e2.record()
         e2.synchronize()
-        t_f.append(e0.elapsed_time(e1) * 1e-3)
-        t_a.append(e1.elapsed_time(e2) * 1e-3)
-    iters = prob.last_iters.cpu().numpy()
-    J = float(obj[:, 0].sum() + cell * obj[:, 1].sum())
-    return {"workload": f"{n}^3 log-normal cube (median n_H 1e4, sigma_ln 1.0, 0.5 pc), G0 1e3 "
-                        "from -x over 48 rays, zeta 2e-16; objective sum Lambda_CII + CO column "
-                        "along +x at the charge-neutral steady state",
-            "cells": prob.n, "parameters": prob.n_params,
-            "forward_s": float(np.median(t_f)), "adjoint_s": float(np.median(t_a)),
-            "ratio": float(np.median(t_a) / np.median(t_f)),
-            "polish_iterations_max": int(iters.max()), "polish_unconverged": int((iters < 0).sum()),
-            "objective": J, "grad_norm": float(g.abs().max())}
+        iters = prob.last_iters.cpu().numpy()
+        out["steady_state_adjoint"] = {
+            "what": "implicit-function gradient at the charge-neutral steady state (integration "
+                    "+ polish forward, one transposed solve per cell)",
+            "forward_s": e0.elapsed_time(e1) * 1e-3, "adjoint_s": e1.elapsed_time(e2) * 1e-3,
+            "ratio": e1.elapsed_time(e2) / e0.elapsed_time(e1),
+            "polish_unconverged": int((iters < 0).sum())}
+    return out
 
 
 # --- per-kernel lines (--kernel) ---------------------------------------------
@@ -826,8 +863,9 @@ def main():
         forms_line(args)
         return
     if args.adjoint_only:
-        adj = adjoint_measure(args.adjoint_n)
-        print(json.dumps({"metric": "adjoint / forward time, steady-state gradient",
+        adj = adjoint_measure(args.adjoint_n, steady=args.adjoint_n <= 32)
+        print(json.dumps({"metric": "adjoint / forward time, discrete adjoint of the implicit "
+                                    "steps (full rate-coefficient gradient)",
                           "value": adj["ratio"], "unit": "ratio", "n_gpus": 1,
                           "higher_is_better": False, "dtype": "f64", "data": "synthetic",
                           "adjoint": adj}), flush=True)
@@ -850,7 +888,7 @@ def headline(args):
     from prism3d_amd.gen_lane import op_counts
     deadline = T_START + args.budget_s
     reserve = 0.0 if args.no_cpu else 35.0          # CPU replay + oracle check
-    reserve += 0.0 if args.no_adjoint else 20.0     # differentiation workload
+    reserve += 0.0 if args.no_adjoint else 30.0     # differentiation workload
     reserve += 0.0 if args.no_forms else 25.0       # the forms on the same inputs
     solver = make_solver(args.n, args.variant, args.waves_per_cu, args.autotune)
     print(f"[bench] {args.n}^3 = {solver.n_cells} cells; budget {args.budget_s:.0f} s",
@@ -947,7 +985,7 @@ def headline(args):
            "roofline": roof}
     if not args.no_forms and time.time() + 25 < deadline:
         out["forms_same_inputs"] = forms_same_inputs(solver)
-    if not args.no_adjoint and time.time() + 20 < deadline:
+    if not args.no_adjoint and time.time() + 30 < deadline:
         adj = adjoint_measure(args.adjoint_n)
         out["adjoint_over_forward"] = adj["ratio"]
         out["adjoint"] = adj

@@ -298,8 +298,8 @@ int p3d_chem_solve_tape(int64_t handle, int64_t n, double *y, const double *n_h,
  * reused by the transposed solves.  obj (n x 2) = {Lambda_CII, n_H x_CO};
  * grad (n x (n_reactions + 3)); status[c] (may be NULL) = 0, or -1 failed
  * integration, -2 tape too short, -3 a recomputed segment missed the
- * forward's state at its end by more than 1e-6 relative (species above
- * 1e-14) (those cells' gradients are not valid). */
+ * forward's state at its end by more than 1e-6 |y| + 1e-14 (those cells'
+ * gradients are not valid). */
 int p3d_chem_tape_adjoint(int64_t handle, int64_t n, const double *y0,
                           const double *y_end, const double *n_h,
                           const double *t_gas, const double *g0, const double *a_v,

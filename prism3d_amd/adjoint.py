@@ -56,6 +56,7 @@ class SteadyStateProblem:
                          atol=atol, post_mode=-1, variant=variant)
         self.polish_iter = int(polish_iter)
         self.last_iters = None
+        self.last = None
 
     def _solver(self, dlog10k):
         if not np.any(dlog10k):
@@ -90,26 +91,52 @@ class SteadyStateProblem:
         g = grad.sum(dim=0) if want_grad else None
         return J, obj, g, y
 
+    def objective(self, theta=None):
+        """(J, per-cell objective (n, 2)); the gradient is formed with the
+        forward (one transposed solve per cell) and kept for gradient()."""
+        J, obj, g, _ = self.evaluate(theta, want_grad=True)
+        self.last = dict(g=g, obj=obj)
+        return J, obj
 
-class SteadyObjective(torch.autograd.Function):
-    """J(theta) of a SteadyStateProblem with its adjoint gradient:
-    ``J = SteadyObjective.apply(theta, problem)``; ``J.backward()`` fills
-    theta.grad with dJ/dtheta."""
+    def gradient(self, last=None):
+        """(dJ/dtheta, per-cell objective, None, None) of a forward objective()."""
+        L = last if last is not None else self.last
+        if L is None:
+            raise RuntimeError("gradient() needs a forward objective() first")
+        return L["g"], L["obj"], None, None
+
+
+class ChemistryObjective(torch.autograd.Function):
+    """J(theta) of a chemistry problem with its adjoint gradient:
+    ``J = ChemistryObjective.apply(theta, problem)``; ``J.backward()`` fills
+    theta.grad with dJ/dtheta.  The problem chooses the mode:
+    SteadyStateProblem (the implicit-function adjoint at the charge-neutral
+    steady state, computed with the forward) or TransientProblem (the
+    discrete adjoint of the implicit steps, run in the backward pass over
+    the tape of this forward)."""
 
     @staticmethod
     def forward(ctx, theta, problem):
-        J, _, g, _ = problem.evaluate(theta, want_grad=True)
-        ctx.save_for_backward(g.to(theta.device, theta.dtype))
+        J, _ = problem.objective(theta)
+        ctx.problem, ctx.last = problem, problem.last
+        ctx.meta = (theta.device, theta.dtype)
         return J.to(theta.device, theta.dtype)
 
     @staticmethod
     def backward(ctx, gJ):
-        (g,) = ctx.saved_tensors
-        return gJ * g, None
+        g = ctx.problem.gradient(ctx.last)[0]
+        dev, dt = ctx.meta
+        return gJ * g.to(dev, dt), None
 
 
-def steady_objective(theta, problem):
-    return SteadyObjective.apply(theta, problem)
+SteadyObjective = TransientObjective = ChemistryObjective
+
+
+def chemistry_objective(theta, problem):
+    return ChemistryObjective.apply(theta, problem)
+
+
+steady_objective = transient_objective = chemistry_objective
 
 
 class TransientProblem(SteadyStateProblem):
@@ -123,25 +150,45 @@ class TransientProblem(SteadyStateProblem):
     steps; the adjoint recomputes each 16-step segment from its checkpoint,
     stores its steps' factors of I - c J and reuses them in the transposed
     solves.  cap bounds the accepted steps per cell of the tape (a cell past
-    it gets no gradient: status -2)."""
+    it gets no gradient: status -2).
+
+    Memory: the tape takes ~(24 cap + 2048 (cap / 16 + 1)) bytes per cell.
+    When the cells' tapes exceed tape_budget bytes the forward keeps none:
+    it runs the taped integration in chunks of cells that fit, and the
+    gradient recomputes each chunk's taped forward before its adjoint walk
+    (one more forward of work, the same results)."""
 
     def __init__(self, n_H, T, G0, A_V, zeta, fsh_H2, fsh_CO, y0, w_cii=1.0, w_co=1.0,
-                 network=None, t_end=None, rtol=1e-9, atol=1e-20, cap=4096, max_steps=1000000):
+                 network=None, t_end=None, rtol=1e-9, atol=1e-20, cap=4096, max_steps=1000000,
+                 tape_budget=48 * 2 ** 30):
         super().__init__(n_H, T, G0, A_V, zeta, fsh_H2, fsh_CO, y0, w_cii=w_cii, w_co=w_co,
                          network=network, rtol=rtol, atol=atol)
         self.opts = dict(t_max=t_end, t_start=1e4, n_checkpoints=2, early_stop=False,
                          rtol=rtol, atol=atol, post_mode=-1, max_steps=max_steps)
         self.cap = int(cap)
+        per_cell = 24 * self.cap + 2048 * (self.cap // 16 + 1) + 16
+        self.chunk = max(1, min(self.n, int(tape_budget // per_cell)))
         self.last = None
 
+    def _chunks(self):
+        return [(a, min(a + self.chunk, self.n)) for a in range(0, self.n, self.chunk)]
+
     def state(self, theta=None):
-        """State at t_end (n_species, n) at parameters theta, with its tape."""
+        """State at t_end (n_species, n) at parameters theta (its tape kept
+        when all cells' tapes fit the budget)."""
         th, ins = self.inputs(theta)
         cs = self._solver(th[:self.nr])
         y = self.y0.clone()
-        stats, tape = cs.solve_tape(y, *ins, opts=solve_opts(**self.opts), cap=self.cap,
-                                    sync=False)
-        self.last = dict(theta=th, cs=cs, ins=ins, y=y, tape=tape, stats=stats)
+        o = solve_opts(**self.opts)
+        tape = None
+        if self.chunk >= self.n:
+            _, tape = cs.solve_tape(y, *ins, opts=o, cap=self.cap, sync=False)
+        else:
+            for a, b in self._chunks():
+                yc = y[:, a:b].contiguous()
+                cs.solve_tape(yc, *[t[a:b] for t in ins], opts=o, cap=self.cap, sync=False)
+                y[:, a:b] = yc
+        self.last = dict(theta=th, cs=cs, ins=ins, y=y, tape=tape)
         return y, cs, ins
 
     def objective(self, theta=None):
@@ -152,16 +199,31 @@ class TransientProblem(SteadyStateProblem):
         return J, obj
 
     def gradient(self, last=None, w_cii=None, w_co=None):
-        """dJ/dtheta (n_params,), per-cell objective, status and per-cell
-        gradient of a forward state() (the last one by default): the adjoint
-        walk over its tape (weights: the problem's unless given)."""
+        """dJ/dtheta (n_params,), per-cell objective (n, 2), status (n,) and
+        per-cell gradient (n, n_params) of a forward state() (the last one by
+        default): the adjoint walk over its tape (weights: the problem's
+        unless given)."""
         L = last if last is not None else self.last
         if L is None:
             raise RuntimeError("gradient() needs a forward state() first")
         wc = self.w_cii if w_cii is None else float(w_cii)
         wo = self.w_co if w_co is None else float(w_co)
-        obj, grad, status = L["cs"].tape_adjoint(self.y0, L["y"], *L["ins"], tape=L["tape"],
-                                                 w_cii=wc, w_co=wo, line=CII_158, sync=False)
+        cs, ins = L["cs"], L["ins"]
+        if L["tape"] is not None:
+            obj, grad, status = cs.tape_adjoint(self.y0, L["y"], *ins, tape=L["tape"],
+                                                w_cii=wc, w_co=wo, line=CII_158, sync=False)
+        else:
+            o = solve_opts(**self.opts)
+            parts = []
+            for a, b in self._chunks():
+                sub = [t[a:b] for t in ins]
+                y0c = self.y0[:, a:b].contiguous()
+                yc = y0c.clone()
+                _, tape = cs.solve_tape(yc, *sub, opts=o, cap=self.cap, sync=False)
+                parts.append(cs.tape_adjoint(y0c, yc, *sub, tape=tape, w_cii=wc, w_co=wo,
+                                             line=CII_158, sync=False))
+                del tape
+            obj, grad, status = (torch.cat([p_[k] for p_ in parts]) for k in range(3))
         return grad.sum(dim=0), obj, status, grad
 
     def evaluate(self, theta=None, want_grad=True):
@@ -187,26 +249,3 @@ def _objective_values(y, ins, network, line):
     frac = ar * q / (ar * q + A + q)
     lam = x('C+') * n_H * frac * (A * 6.62607015e-27 * freq)
     return torch.stack([lam, n_H * x('CO')], dim=1)
-
-
-class TransientObjective(torch.autograd.Function):
-    """J(theta) of a TransientProblem: ``J = TransientObjective.apply(theta,
-    problem)``; ``J.backward()`` runs the discrete adjoint of the implicit
-    steps over the tape of this forward and fills theta.grad."""
-
-    @staticmethod
-    def forward(ctx, theta, problem):
-        J, _ = problem.objective(theta)
-        ctx.problem, ctx.last = problem, problem.last
-        ctx.meta = (theta.device, theta.dtype)
-        return J.to(theta.device, theta.dtype)
-
-    @staticmethod
-    def backward(ctx, gJ):
-        g, _, status, _ = ctx.problem.gradient(ctx.last)
-        dev, dt = ctx.meta
-        return gJ * g.to(dev, dt), None
-
-
-def transient_objective(theta, problem):
-    return TransientObjective.apply(theta, problem)

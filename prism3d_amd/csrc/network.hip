@@ -2245,15 +2245,17 @@ chem_tape_adjoint_kernel(NetDev N, int64_t n, const double *__restrict__ y0,
           sp[31 * 64 + lane] = row ? L.M[lane * L.ld + lin.pos] : 0.0;
           sp[32 * 64 + lane] = D[0];
         }
-        // the recomputed segment must land on the forward's state: the next
-        // checkpoint, or the integration's output after the last segment
+        // the recomputed segment must land on the forward's state (the next
+        // checkpoint, or the integration's output after the last segment)
+        // within 1e-6 relative + 1e-14 absolute (species far below the
+        // abundances that matter differ at the atol level, where the floor
+        // of the rate equations leaves the Newton iteration no grip)
         {
           const bool last = i1 >= nst;
           const double yref = last ? xe : (act ? ck[((sg + 1) * kNumD) * 32 + lane] : 0.0);
           const double ycur = last ? fmax(D[0], 1e-30) : D[0];
-          const double dev =
-              (act && fabs(yref) > 1e-14) ? fabs(ycur - yref) / fabs(yref) : 0.0;
-          if (wave_max(dev) > 1e-6 && failed == 0) failed = -3;
+          const double dev = act ? fabs(ycur - yref) / (1e-6 * fabs(yref) + 1e-14) : 0.0;
+          if (wave_max(dev) > 1.0 && failed == 0) failed = -3;
         }
         // backwards through the segment
         for (int i = i1 - 1; i >= i0; --i) {

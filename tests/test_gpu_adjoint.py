@@ -264,3 +264,30 @@ def test_tape_adjoint_against_central_differences_on_cube(gpu):
               f"1% of it agree to {rel_big:.2e} relative", flush=True)
         worst.append(err.max())
     assert max(worst) < 1e-4
+
+
+def test_transient_autograd_and_chunked_tapes(gpu):
+    """The autograd front end of the implicit-step adjoint: J.backward() fills
+    theta.grad with the adjoint's gradient, and the memory-short mode (tapes
+    of chunks of cells, recomputed in the backward pass) gives the same
+    objective and gradient bit for bit."""
+    import torch
+    from prism3d_amd.adjoint import TransientProblem, chemistry_objective
+    n = 300
+    par, y0 = _cells(n, 5)
+    grads, Js = [], []
+    for budget in (48 * 2 ** 30, 128 * (24 * 4096 + 2048 * 257 + 16)):
+        prob = TransientProblem(*par, y0, w_cii=1e24, w_co=1e-6, t_end=1e6 * 3.15576e7,
+                                cap=4096, tape_budget=budget)
+        theta = torch.zeros(prob.n_params, dtype=torch.float64, requires_grad=True)
+        J = chemistry_objective(theta, prob)
+        J.backward()
+        g, _, status, _ = prob.gradient()
+        assert torch.equal(theta.grad, g.cpu())
+        st = status.cpu().numpy()
+        assert (st == 0).all(), dict(zip(*np.unique(st, return_counts=True)))
+        grads.append(theta.grad.clone())
+        Js.append(float(J))
+        print(f"tape budget {budget}: {prob.chunk} cells per chunk", flush=True)
+    assert Js[0] == Js[1]
+    assert torch.equal(grads[0], grads[1])
