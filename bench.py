@@ -365,7 +365,8 @@ def adjoint_measure(n=32, steps=2, steady=True):
            "ratio_with_taped_forward": float((np.median(t_t) + np.median(t_a)) / np.median(t_f)),
            "cells_without_gradient": {str(k): int(v) for k, v in
                                       zip(*np.unique(st[st != 0], return_counts=True))},
-           "objective": float(J), "grad_max_abs": float(g.abs().max())}
+           "objective": float(J), "grad_max_abs": float(g.abs().max()),
+           "note": "cells without a valid gradient are left out of the summed gradient"}
     if steady:
         prob = SteadyStateProblem(*fields, y0v, w_cii=1.0, w_co=cell)
         e0, e1, e2 = ev(), ev(), ev()

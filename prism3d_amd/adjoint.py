@@ -150,7 +150,8 @@ class TransientProblem(SteadyStateProblem):
     steps; the adjoint recomputes each 16-step segment from its checkpoint,
     stores its steps' factors of I - c J and reuses them in the transposed
     solves.  cap bounds the accepted steps per cell of the tape (a cell past
-    it gets no gradient: status -2).
+    it gets no gradient: status -2).  Cells without a valid gradient (status
+    != 0) are left out of the summed gradient and reported by gradient().
 
     Memory: the tape takes ~(24 cap + 2048 (cap / 16 + 1)) bytes per cell.
     When the cells' tapes exceed tape_budget bytes the forward keeps none:
@@ -224,6 +225,10 @@ class TransientProblem(SteadyStateProblem):
                                              line=CII_158, sync=False))
                 del tape
             obj, grad, status = (torch.cat([p_[k] for p_ in parts]) for k in range(3))
+        # a cell without a valid gradient (status != 0: failed integration, tape
+        # too short, or a recomputed segment that missed the forward's state)
+        # contributes nothing to the sum; status reports it
+        grad = torch.where((status == 0)[:, None], grad, torch.zeros_like(grad))
         return grad.sum(dim=0), obj, status, grad
 
     def evaluate(self, theta=None, want_grad=True):

@@ -56,9 +56,11 @@ struct NetDev {
   const int *jt_rxn, *jt_slot;
   const double *jt_coef;
   const double *post_w;  // 4 * ns
-  // balanced RHS gather of the two-wavefront form: the packed entries
-  // (rhs_pk, species by species) cut into pieces at every species boundary
-  // and at 128 equal shares, one share per lane
+  // balanced RHS gather (the two-wavefront form, and the one-wavefront BDF
+  // solve of networks past the register cache): the packed entries (rhs_pk,
+  // species by species) cut into pieces at every species boundary and at 128
+  // equal shares.  The exported RHS, polish and steady adjoint sum each row
+  // serially (rhs_lane); a row split over shares differs from it by rounding.
   int n_pc;
   const int *pc_beg;   // n_pc + 1 entry offsets
   const int *pc_lane;  // 129: pieces of lane l are pc_lane[l] .. pc_lane[l+1]-1
