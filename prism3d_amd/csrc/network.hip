@@ -48,6 +48,11 @@ struct NetDev {
   const int *react;   // 2 per reaction
   const int *dep;     // 2 per reaction: species the rate depends on (or -1)
   const int *st5;     // 5 per reaction: consumed species (2 slots) then produced (3), or -1
+  // per species row of the Jacobian (networks of <= 32 species): its first
+  // structural entry in the jrow/jcol order and the mask of its structural
+  // columns (the register-LU form keeps J sparse)
+  const int *jr_start;
+  const unsigned *jr_mask;
   const int *rhs_off; // ns+1
   const int *rhs_rxn;
   const double *rhs_sgn;
@@ -165,6 +170,36 @@ __device__ __forceinline__ Lds carve_wide(double *base, const NetDev &N) {
   L.csr_off = nullptr;
   L.csr = nullptr;
   return L;
+}
+
+// LDS of one cell in the register-LU form (bundled network): J kept sparse
+// (its nnz structural entries in the jrow/jcol order), M dense for the
+// factors, the step-size change matrix, the 64 RHS piece sums and the 32
+// components of the solves.  No CSR lists (each lane's RHS piece is in
+// registers) and no pivots.
+// The sizes are the generated network's (wave_net.h), so every LDS address
+// is the array base plus a constant: no pointer is held in registers.
+__device__ __forceinline__ Lds carve_wave(double *base, const NetDev &) {
+  constexpr int ld = wave::kNS | 1;
+  Lds L;
+  L.ld = ld;
+  L.kco = base;
+  L.part = base + wave::kNR;
+  L.ys = base + 3 * wave::kNR;
+  L.J = base + 3 * wave::kNR + kMaxSpecies;
+  L.M = base + 3 * wave::kNR + kMaxSpecies + wave::kNNZ;
+  L.ru = base + 3 * wave::kNR + kMaxSpecies + wave::kNNZ + wave::kNS * ld;
+  L.psum = L.ru + 36;
+  L.xs = L.psum + 64;
+  L.piv = nullptr;
+  L.csr_off = nullptr;
+  L.csr = nullptr;
+  L.red = nullptr;
+  return L;
+}
+static size_t lds_bytes_wave() {
+  return sizeof(double) * (size_t)(3 * wave::kNR + kMaxSpecies + wave::kNNZ +
+                                   wave::kNS * (wave::kNS | 1) + 36 + 64 + 32);
 }
 
 static size_t lds_bytes(int ns, int nr, int n_rhs_e, bool with_m = true,
@@ -1038,16 +1073,22 @@ struct LinStatic {
 
   __device__ void load_jac(const NetDev &, const Lds &, int) {}
 
+  // J is sparse in LDS (carve_wave): the lane's row entries start at
+  // N.jr_start[lane], one per set bit of N.jr_mask[lane] in column order.
   __device__ __forceinline__ void factor(const NetDev &N, const Lds &L, double c, int lane) {
     const bool row = lane < wave::kNS;
     const unsigned lm = row ? c_wave_lmask[lane] : 0u;
-    const double *Jrow = L.J + (row ? lane : 0) * L.ld;
+    const unsigned jm = row ? N.jr_mask[lane] : 0u;
+    const int js = row ? N.jr_start[lane] : 0;
     double a[32];
 #pragma unroll
     for (int q = 0; q < 32; ++q) {
       if (q < wave::kNS) {
-        const double jv = row ? Jrow[wave::kPerm[q]] : 0.0;
-        a[q] = (wave::kPerm[q] == lane ? 1.0 : 0.0) - c * jv;
+        const int col = wave::kPerm[q];
+        const bool has = (jm >> col) & 1u;
+        const int z = has ? js + __builtin_popcount(jm & ((1u << col) - 1u)) : js;
+        const double jv = has ? L.J[z] : 0.0;
+        a[q] = (col == lane ? 1.0 : 0.0) - c * jv;
       } else {
         a[q] = 0.0;
       }
@@ -1076,6 +1117,15 @@ struct LinStatic {
 };
 
 __device__ __forceinline__ double rms(double v, int ns) { return sqrt(wave_sum(v * v) / ns); }
+
+// The Jacobian in the layout the linear-algebra form reads: sparse for the
+// register LU (LinStatic, carve_wave), otherwise jac_eval's.
+template <class Lin, class G>
+__device__ __forceinline__ void jac_eval_for(const NetDev &N, const CellParams &p, const Lds &L,
+                                             int lane) {
+  if constexpr (std::is_same<Lin, LinStatic>::value) jacobian_sparse<64>(N, p, L, lane);
+  else jac_eval<G>(N, p, L, lane);
+}
 
 // Step-size change of the backward differences D[0..order] by `factor`:
 // D <- (R(factor) U)^T D with R_im = prod_{q=1..i} (q-1 - factor m)/q and
@@ -1186,7 +1236,7 @@ __device__ __forceinline__ int bdf_integrate_rec(const NetDev &N, const CellPara
   int order = 1, n_equal = 0;
   double t = t0;
   publish_y(N, L, y, lane);
-  jac_eval<G>(N, p, L, lane);
+  jac_eval_for<Lin, G>(N, p, L, lane);
   lin.load_jac(N, L, lane);
   st.jac++;
   bool jac_current = true, lu_valid = false;
@@ -1254,7 +1304,7 @@ __device__ __forceinline__ int bdf_integrate_rec(const NetDev &N, const CellPara
         if (converged) break;
         if (!jac_current) {
           publish_y(N, L, ypred, lane);
-          jac_eval<G>(N, p, L, lane);
+          jac_eval_for<Lin, G>(N, p, L, lane);
           lin.load_jac(N, L, lane);
           st.jac++;
           jac_current = true;
@@ -1526,8 +1576,11 @@ __global__ void __launch_bounds__(64) queue_reset(unsigned long long *ctr) {
 // and readlane latency; the second one fills them.  On the 64^3 workload
 // this solves the same cells bit for bit in 2.94 s against 4.12 s with one
 // wavefront per SIMD (410 registers, no scratch).
+#ifndef P3D_QUEUE_WPE
+#define P3D_QUEUE_WPE 2
+#endif
 template <class Lin>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(P3D_QUEUE_WPE)))
 chem_queue_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__restrict__ nH,
                   const double *__restrict__ T, const double *__restrict__ G0,
                   const double *__restrict__ AV, const double *__restrict__ zeta,
@@ -1535,8 +1588,7 @@ chem_queue_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
                   p3d_solve_opts o, int32_t *__restrict__ conv_out,
                   int32_t *__restrict__ stats_out, unsigned long long *__restrict__ ctr) {
   extern __shared__ double smem[];
-  const Lds L = carve(smem, N, Lin::kNeedsM, true);
-  stage_tables(N, L, threadIdx.x);
+  const Lds L = carve_wave(smem, N);
   for (;;) {
     unsigned long long c = 0;
     if (threadIdx.x == 0) c = atomicAdd(ctr, 1ull);
@@ -2000,8 +2052,7 @@ chem_tape_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__re
                  p3d_solve_opts o, TapeDev TD, int32_t *__restrict__ stats_out,
                  unsigned long long *__restrict__ ctr) {
   extern __shared__ double smem[];
-  const Lds L = carve(smem, N, true, true);
-  stage_tables(N, L, threadIdx.x);
+  const Lds L = carve_wave(smem, N);
   const int lane = threadIdx.x;
   for (;;) {
     const int64_t cell = next_cell(ctr);
@@ -2121,8 +2172,7 @@ chem_tape_adjoint_kernel(NetDev N, int64_t n, const double *__restrict__ y0,
                          int32_t *__restrict__ status_out, double *__restrict__ segbuf,
                          unsigned long long *__restrict__ ctr) {
   extern __shared__ double smem[];
-  const Lds L = carve(smem, N, true, true);
-  stage_tables(N, L, threadIdx.x);
+  const Lds L = carve_wave(smem, N);
   const int lane = threadIdx.x;
   const bool act = lane < N.ns;
   const bool row = lane < wave::kNS;
@@ -2180,7 +2230,7 @@ chem_tape_adjoint_kernel(NetDev N, int64_t n, const double *__restrict__ y0,
           const double scale = o.atol + o.rtol * fabs(ypred);
           if (!jac_ok) {
             publish_y(N, L, ypred, lane);
-            jac_eval<Wave1>(N, p, L, lane);
+            jacobian_sparse<64>(N, p, L, lane);
             jac_ok = true;
             c_lu = -1.0;
           }
@@ -2220,7 +2270,7 @@ chem_tape_adjoint_kernel(NetDev N, int64_t n, const double *__restrict__ y0,
             }
             if (!conv) {  // refresh the Jacobian at the current iterate
               publish_y(N, L, ynew, lane);
-              jac_eval<Wave1>(N, p, L, lane);
+              jacobian_sparse<64>(N, p, L, lane);
               c_lu = -1.0;
             }
           }
@@ -2237,7 +2287,7 @@ chem_tape_adjoint_kernel(NetDev N, int64_t n, const double *__restrict__ y0,
           // factors of I - c J(y_n+1): kept in L.M for the next step's
           // Newton, stored in the column layout for the backward pass
           publish_y(N, L, D[0], lane);
-          jac_eval<Wave1>(N, p, L, lane);
+          jacobian_sparse<64>(N, p, L, lane);
           lin.factor(N, L, c, lane);
           c_lu = c;
           double *sp = seg + (size_t)(i - i0) * kTapeCol * 64;
@@ -2403,7 +2453,7 @@ static int queue_solve(Network *net, int64_t n, double *y, const double *n_h,
                        const double *zeta, const double *fsh_h2, const double *fsh_co,
                        const p3d_solve_opts &o, int32_t *converged, int32_t *stats,
                        void *stream) {
-  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e, LinStatic::kNeedsM, true);
+  const size_t sh = lds_bytes_wave();
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
   if (!net->queue_ctr) {
     if (hipMalloc(&net->queue_ctr, 256) != hipSuccess) {
@@ -2597,6 +2647,13 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
     joff.push_back((int)jt_rxn.size());
   }
   const int nnz = (int)jrow.size();
+  std::vector<int> jr_start(ns + 1, nnz);
+  std::vector<unsigned> jr_mask(ns, 0u);
+  for (int z = nnz - 1; z >= 0; --z) jr_start[jrow[z]] = z;
+  for (int r = ns - 1; r >= 0; --r)
+    if (jr_start[r] > jr_start[r + 1]) jr_start[r] = jr_start[r + 1];
+  for (int z = 0; z < nnz; ++z)
+    if (jcol[z] < 32) jr_mask[jrow[z]] |= 1u << jcol[z];
   // pieces of the balanced RHS gather: cuts at every species row start and
   // at 128 equal shares of the entries
   const int n_e = (int)rhs_pk.size();
@@ -2645,7 +2702,9 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
          o_tr = put(jt_rxn.data(), 4 * jt_rxn.size()), o_ts = put(jt_slot.data(), 4 * jt_slot.size()),
          o_tc = put(jt_coef.data(), 8 * jt_coef.size()), o_pw = put(post.data(), 8 * post.size()),
          o_pb = put(pc_beg.data(), 4 * pc_beg.size()), o_pl = put(pc_lane.data(), 4 * pc_lane.size()),
-         o_pr = put(pc_row.data(), 4 * pc_row.size()), o_s5 = put(st5.data(), 4 * st5.size());
+         o_pr = put(pc_row.data(), 4 * pc_row.size()), o_s5 = put(st5.data(), 4 * st5.size()),
+         o_js = put(jr_start.data(), 4 * jr_start.size()),
+         o_jm = put(jr_mask.data(), 4 * jr_mask.size());
   auto net = std::make_unique<Network>();
   if (hipMalloc(&net->blob, host.size()) != hipSuccess) {
     set_error("hipMalloc of %zu bytes for network tables failed", host.size());
@@ -2663,6 +2722,8 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   d.react = (const int *)(b + o_re);
   d.dep = (const int *)(b + o_dep);
   d.st5 = (const int *)(b + o_s5);
+  d.jr_start = (const int *)(b + o_js);
+  d.jr_mask = (const unsigned *)(b + o_jm);
   d.rhs_off = (const int *)(b + o_ro);
   d.rhs_rxn = (const int *)(b + o_rr);
   d.rhs_sgn = (const double *)(b + o_rs);
@@ -2678,8 +2739,11 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   d.pc_beg = (const int *)(b + o_pb);
   d.pc_lane = (const int *)(b + o_pl);
   d.pc_row = (const int *)(b + o_pr);
+  // the generated register-LU code and its LDS layout fix the species,
+  // reactions and Jacobian structure
   net->lane_ok = roles7 != nullptr &&
-                 lane_network_matches(ns, nr, rtype, reactants, products, e_index, roles);
+                 lane_network_matches(ns, nr, rtype, reactants, products, e_index, roles) &&
+                 ns == wave::kNS && nr == wave::kNR && nnz == wave::kNNZ;
   std::lock_guard<std::mutex> lk(g_net_mu);
   int64_t h = g_next_handle++;
   g_nets[h] = std::move(net);
@@ -2819,7 +2883,7 @@ int p3d_chem_solve_tape(int64_t handle, int64_t n, double *y, const double *n_h,
   if (rc != P3D_OK || n == 0) return rc;
   if ((rc = tape_checks(net, n, opts, tape, tape_bytes, cap, "p3d_chem_solve_tape")) != P3D_OK)
     return rc;
-  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e, true, true);
+  const size_t sh = lds_bytes_wave();
   if ((rc = ensure_queue_ctr(net)) != P3D_OK) return rc;
   int64_t waves = 0;
   if ((rc = resident_waves(reinterpret_cast<const void *>(chem_tape_kernel), sh, n, &waves)) !=
@@ -2853,7 +2917,7 @@ int p3d_chem_tape_adjoint(int64_t handle, int64_t n, const double *y0, const dou
                 "the network lacks a species the objective needs (H, H2, C+, CO)");
   CiiLine cl;
   cl.freq = cii5[0]; cl.A = cii5[1]; cl.Tul = cii5[2]; cl.gu = cii5[3]; cl.gl = cii5[4];
-  const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e, true, true);
+  const size_t sh = lds_bytes_wave();
   if ((rc = ensure_queue_ctr(net)) != P3D_OK) return rc;
   int64_t waves = 0;
   if ((rc = resident_waves(reinterpret_cast<const void *>(chem_tape_adjoint_kernel), sh, n,

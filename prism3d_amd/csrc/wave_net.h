@@ -10,6 +10,8 @@ namespace p3d {
 namespace wave {
 
 constexpr int kNS = 31;
+constexpr int kNR = 75;  // reactions
+constexpr int kNNZ = 232;  // structural entries of the Jacobian
 constexpr int kPerm[kNS] = {5, 23, 24, 25, 26, 27, 28, 4, 29, 30, 14, 12, 19, 3, 18, 20, 16, 11, 22, 10, 6, 17, 7, 15, 0, 1, 2, 8, 9, 13, 21};
 // per species (lane): bit k set where its row has a structural multiplier
 // (kLMask) or upper entry (kUMask) at elimination position k

@@ -437,6 +437,8 @@ def generate_wave(net=None):
     w("namespace wave {")
     w("")
     w(f"constexpr int kNS = {ns};")
+    w(f"constexpr int kNR = {net.n_reactions};  // reactions")
+    w(f"constexpr int kNNZ = {int(pat.sum())};  // structural entries of the Jacobian")
     w("constexpr int kPerm[kNS] = {" + ", ".join(str(int(x)) for x in perm) + "};")
     w("// per species (lane): bit k set where its row has a structural multiplier")
     w("// (kLMask) or upper entry (kUMask) at elimination position k")
