@@ -30,8 +30,10 @@ constexpr int kPieceLen = 6;
 // Afterwards a[k] holds the multiplier l_ik below the diagonal, 1 / u_ii
 // on it and u_ij above it (structural entries).  The pivot rows of one
 // dependency level are broadcast together, then applied in position order.
-template <class BC>
-__device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane, const BC &bc) {
+// A: the lane's row, a double[32] in registers or an accessor of its row in LDS
+// (bc then reads the pivot row there); the same statements serve both.
+template <class A, class BC>
+__device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC &bc) {
   {  // level 1: H2+, S, Si, Fe, H-, He, O+, CH2+, H3O+, O2, HCO+
     const double inv0 = 1.0 / bc(a[0], 5);
     const double u0_26 = bc(a[26], 5);
@@ -80,90 +82,90 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u18_30 = bc(a[30], 22);
     if ((lm >> 0) & 1u) {
       const double m = a[0] * inv0;
-      a[26] -= m * u0_26;
+      a[26] = a[26] - m * u0_26;
       a[0] = m;
     }
     if (lane == 5) a[0] = inv0;
     if ((lm >> 1) & 1u) {
       const double m = a[1] * inv1;
-      a[2] -= m * u1_2;
-      a[24] -= m * u1_24;
+      a[2] = a[2] - m * u1_2;
+      a[24] = a[24] - m * u1_24;
       a[1] = m;
     }
     if (lane == 23) a[1] = inv1;
     if ((lm >> 3) & 1u) {
       const double m = a[3] * inv3;
-      a[4] -= m * u3_4;
-      a[24] -= m * u3_24;
+      a[4] = a[4] - m * u3_4;
+      a[24] = a[24] - m * u3_24;
       a[3] = m;
     }
     if (lane == 25) a[3] = inv3;
     if ((lm >> 5) & 1u) {
       const double m = a[5] * inv5;
-      a[6] -= m * u5_6;
-      a[24] -= m * u5_24;
+      a[6] = a[6] - m * u5_6;
+      a[24] = a[24] - m * u5_24;
       a[5] = m;
     }
     if (lane == 27) a[5] = inv5;
     if ((lm >> 7) & 1u) {
       const double m = a[7] * inv7;
-      a[24] -= m * u7_24;
-      a[25] -= m * u7_25;
+      a[24] = a[24] - m * u7_24;
+      a[25] = a[25] - m * u7_25;
       a[7] = m;
     }
     if (lane == 4) a[7] = inv7;
     if ((lm >> 8) & 1u) {
       const double m = a[8] * inv8;
-      a[9] -= m * u8_9;
-      a[24] -= m * u8_24;
-      a[28] -= m * u8_28;
+      a[9] = a[9] - m * u8_9;
+      a[24] = a[24] - m * u8_24;
+      a[28] = a[28] - m * u8_28;
       a[8] = m;
     }
     if (lane == 29) a[8] = inv8;
     if ((lm >> 10) & 1u) {
       const double m = a[10] * inv10;
-      a[13] -= m * u10_13;
-      a[25] -= m * u10_25;
-      a[29] -= m * u10_29;
+      a[13] = a[13] - m * u10_13;
+      a[25] = a[25] - m * u10_25;
+      a[29] = a[29] - m * u10_29;
       a[10] = m;
     }
     if (lane == 14) a[10] = inv10;
     if ((lm >> 11) & 1u) {
       const double m = a[11] * inv11;
-      a[17] -= m * u11_17;
-      a[24] -= m * u11_24;
-      a[26] -= m * u11_26;
+      a[17] = a[17] - m * u11_17;
+      a[24] = a[24] - m * u11_24;
+      a[26] = a[26] - m * u11_26;
       a[11] = m;
     }
     if (lane == 12) a[11] = inv11;
     if ((lm >> 12) & 1u) {
       const double m = a[12] * inv12;
-      a[14] -= m * u12_14;
-      a[24] -= m * u12_24;
-      a[26] -= m * u12_26;
+      a[14] = a[14] - m * u12_14;
+      a[24] = a[24] - m * u12_24;
+      a[26] = a[26] - m * u12_26;
       a[12] = m;
     }
     if (lane == 19) a[12] = inv12;
     if ((lm >> 15) & 1u) {
       const double m = a[15] * inv15;
-      a[22] -= m * u15_22;
-      a[23] -= m * u15_23;
-      a[27] -= m * u15_27;
-      a[29] -= m * u15_29;
+      a[22] = a[22] - m * u15_22;
+      a[23] = a[23] - m * u15_23;
+      a[27] = a[27] - m * u15_27;
+      a[29] = a[29] - m * u15_29;
       a[15] = m;
     }
     if (lane == 20) a[15] = inv15;
     if ((lm >> 18) & 1u) {
       const double m = a[18] * inv18;
-      a[19] -= m * u18_19;
-      a[20] -= m * u18_20;
-      a[21] -= m * u18_21;
-      a[22] -= m * u18_22;
-      a[24] -= m * u18_24;
-      a[26] -= m * u18_26;
-      a[28] -= m * u18_28;
-      a[29] -= m * u18_29;
-      a[30] -= m * u18_30;
+      a[19] = a[19] - m * u18_19;
+      a[20] = a[20] - m * u18_20;
+      a[21] = a[21] - m * u18_21;
+      a[22] = a[22] - m * u18_22;
+      a[24] = a[24] - m * u18_24;
+      a[26] = a[26] - m * u18_26;
+      a[28] = a[28] - m * u18_28;
+      a[29] = a[29] - m * u18_29;
+      a[30] = a[30] - m * u18_30;
       a[18] = m;
     }
     if (lane == 22) a[18] = inv18;
@@ -197,55 +199,55 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u17_27 = bc(a[27], 11);
     if ((lm >> 2) & 1u) {
       const double m = a[2] * inv2;
-      a[24] -= m * u2_24;
+      a[24] = a[24] - m * u2_24;
       a[2] = m;
     }
     if (lane == 24) a[2] = inv2;
     if ((lm >> 4) & 1u) {
       const double m = a[4] * inv4;
-      a[24] -= m * u4_24;
+      a[24] = a[24] - m * u4_24;
       a[4] = m;
     }
     if (lane == 26) a[4] = inv4;
     if ((lm >> 6) & 1u) {
       const double m = a[6] * inv6;
-      a[24] -= m * u6_24;
+      a[24] = a[24] - m * u6_24;
       a[6] = m;
     }
     if (lane == 28) a[6] = inv6;
     if ((lm >> 9) & 1u) {
       const double m = a[9] * inv9;
-      a[24] -= m * u9_24;
-      a[28] -= m * u9_28;
+      a[24] = a[24] - m * u9_24;
+      a[28] = a[28] - m * u9_28;
       a[9] = m;
     }
     if (lane == 30) a[9] = inv9;
     if ((lm >> 13) & 1u) {
       const double m = a[13] * inv13;
-      a[22] -= m * u13_22;
-      a[23] -= m * u13_23;
-      a[24] -= m * u13_24;
-      a[25] -= m * u13_25;
-      a[29] -= m * u13_29;
-      a[30] -= m * u13_30;
+      a[22] = a[22] - m * u13_22;
+      a[23] = a[23] - m * u13_23;
+      a[24] = a[24] - m * u13_24;
+      a[25] = a[25] - m * u13_25;
+      a[29] = a[29] - m * u13_29;
+      a[30] = a[30] - m * u13_30;
       a[13] = m;
     }
     if (lane == 3) a[13] = inv13;
     if ((lm >> 14) & 1u) {
       const double m = a[14] * inv14;
-      a[16] -= m * u14_16;
-      a[24] -= m * u14_24;
-      a[26] -= m * u14_26;
+      a[16] = a[16] - m * u14_16;
+      a[24] = a[24] - m * u14_24;
+      a[26] = a[26] - m * u14_26;
       a[14] = m;
     }
     if (lane == 18) a[14] = inv14;
     if ((lm >> 17) & 1u) {
       const double m = a[17] * inv17;
-      a[20] -= m * u17_20;
-      a[22] -= m * u17_22;
-      a[24] -= m * u17_24;
-      a[26] -= m * u17_26;
-      a[27] -= m * u17_27;
+      a[20] = a[20] - m * u17_20;
+      a[22] = a[22] - m * u17_22;
+      a[24] = a[24] - m * u17_24;
+      a[26] = a[26] - m * u17_26;
+      a[27] = a[27] - m * u17_27;
       a[17] = m;
     }
     if (lane == 11) a[17] = inv17;
@@ -266,22 +268,22 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u19_29 = bc(a[29], 10);
     if ((lm >> 16) & 1u) {
       const double m = a[16] * inv16;
-      a[20] -= m * u16_20;
-      a[24] -= m * u16_24;
-      a[26] -= m * u16_26;
-      a[29] -= m * u16_29;
+      a[20] = a[20] - m * u16_20;
+      a[24] = a[24] - m * u16_24;
+      a[26] = a[26] - m * u16_26;
+      a[29] = a[29] - m * u16_29;
       a[16] = m;
     }
     if (lane == 16) a[16] = inv16;
     if ((lm >> 19) & 1u) {
       const double m = a[19] * inv19;
-      a[20] -= m * u19_20;
-      a[22] -= m * u19_22;
-      a[24] -= m * u19_24;
-      a[25] -= m * u19_25;
-      a[26] -= m * u19_26;
-      a[27] -= m * u19_27;
-      a[29] -= m * u19_29;
+      a[20] = a[20] - m * u19_20;
+      a[22] = a[22] - m * u19_22;
+      a[24] = a[24] - m * u19_24;
+      a[25] = a[25] - m * u19_25;
+      a[26] = a[26] - m * u19_26;
+      a[27] = a[27] - m * u19_27;
+      a[29] = a[29] - m * u19_29;
       a[19] = m;
     }
     if (lane == 10) a[19] = inv19;
@@ -295,11 +297,11 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u20_29 = bc(a[29], 6);
     if ((lm >> 20) & 1u) {
       const double m = a[20] * inv20;
-      a[24] -= m * u20_24;
-      a[26] -= m * u20_26;
-      a[27] -= m * u20_27;
-      a[28] -= m * u20_28;
-      a[29] -= m * u20_29;
+      a[24] = a[24] - m * u20_24;
+      a[26] = a[26] - m * u20_26;
+      a[27] = a[27] - m * u20_27;
+      a[28] = a[28] - m * u20_28;
+      a[29] = a[29] - m * u20_29;
       a[20] = m;
     }
     if (lane == 6) a[20] = inv20;
@@ -315,13 +317,13 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u21_29 = bc(a[29], 17);
     if ((lm >> 21) & 1u) {
       const double m = a[21] * inv21;
-      a[22] -= m * u21_22;
-      a[23] -= m * u21_23;
-      a[24] -= m * u21_24;
-      a[26] -= m * u21_26;
-      a[27] -= m * u21_27;
-      a[28] -= m * u21_28;
-      a[29] -= m * u21_29;
+      a[22] = a[22] - m * u21_22;
+      a[23] = a[23] - m * u21_23;
+      a[24] = a[24] - m * u21_24;
+      a[26] = a[26] - m * u21_26;
+      a[27] = a[27] - m * u21_27;
+      a[28] = a[28] - m * u21_28;
+      a[29] = a[29] - m * u21_29;
       a[21] = m;
     }
     if (lane == 17) a[21] = inv21;
@@ -336,12 +338,12 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u22_29 = bc(a[29], 7);
     if ((lm >> 22) & 1u) {
       const double m = a[22] * inv22;
-      a[23] -= m * u22_23;
-      a[24] -= m * u22_24;
-      a[26] -= m * u22_26;
-      a[27] -= m * u22_27;
-      a[28] -= m * u22_28;
-      a[29] -= m * u22_29;
+      a[23] = a[23] - m * u22_23;
+      a[24] = a[24] - m * u22_24;
+      a[26] = a[26] - m * u22_26;
+      a[27] = a[27] - m * u22_27;
+      a[28] = a[28] - m * u22_28;
+      a[29] = a[29] - m * u22_29;
       a[22] = m;
     }
     if (lane == 7) a[22] = inv22;
@@ -356,12 +358,12 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u23_29 = bc(a[29], 15);
     if ((lm >> 23) & 1u) {
       const double m = a[23] * inv23;
-      a[24] -= m * u23_24;
-      a[25] -= m * u23_25;
-      a[26] -= m * u23_26;
-      a[27] -= m * u23_27;
-      a[28] -= m * u23_28;
-      a[29] -= m * u23_29;
+      a[24] = a[24] - m * u23_24;
+      a[25] = a[25] - m * u23_25;
+      a[26] = a[26] - m * u23_26;
+      a[27] = a[27] - m * u23_27;
+      a[28] = a[28] - m * u23_28;
+      a[29] = a[29] - m * u23_29;
       a[23] = m;
     }
     if (lane == 15) a[23] = inv23;
@@ -376,12 +378,12 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u24_30 = bc(a[30], 0);
     if ((lm >> 24) & 1u) {
       const double m = a[24] * inv24;
-      a[25] -= m * u24_25;
-      a[26] -= m * u24_26;
-      a[27] -= m * u24_27;
-      a[28] -= m * u24_28;
-      a[29] -= m * u24_29;
-      a[30] -= m * u24_30;
+      a[25] = a[25] - m * u24_25;
+      a[26] = a[26] - m * u24_26;
+      a[27] = a[27] - m * u24_27;
+      a[28] = a[28] - m * u24_28;
+      a[29] = a[29] - m * u24_29;
+      a[30] = a[30] - m * u24_30;
       a[24] = m;
     }
     if (lane == 0) a[24] = inv24;
@@ -395,11 +397,11 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u25_30 = bc(a[30], 1);
     if ((lm >> 25) & 1u) {
       const double m = a[25] * inv25;
-      a[26] -= m * u25_26;
-      a[27] -= m * u25_27;
-      a[28] -= m * u25_28;
-      a[29] -= m * u25_29;
-      a[30] -= m * u25_30;
+      a[26] = a[26] - m * u25_26;
+      a[27] = a[27] - m * u25_27;
+      a[28] = a[28] - m * u25_28;
+      a[29] = a[29] - m * u25_29;
+      a[30] = a[30] - m * u25_30;
       a[25] = m;
     }
     if (lane == 1) a[25] = inv25;
@@ -412,10 +414,10 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u26_30 = bc(a[30], 2);
     if ((lm >> 26) & 1u) {
       const double m = a[26] * inv26;
-      a[27] -= m * u26_27;
-      a[28] -= m * u26_28;
-      a[29] -= m * u26_29;
-      a[30] -= m * u26_30;
+      a[27] = a[27] - m * u26_27;
+      a[28] = a[28] - m * u26_28;
+      a[29] = a[29] - m * u26_29;
+      a[30] = a[30] - m * u26_30;
       a[26] = m;
     }
     if (lane == 2) a[26] = inv26;
@@ -427,9 +429,9 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u27_30 = bc(a[30], 8);
     if ((lm >> 27) & 1u) {
       const double m = a[27] * inv27;
-      a[28] -= m * u27_28;
-      a[29] -= m * u27_29;
-      a[30] -= m * u27_30;
+      a[28] = a[28] - m * u27_28;
+      a[29] = a[29] - m * u27_29;
+      a[30] = a[30] - m * u27_30;
       a[27] = m;
     }
     if (lane == 8) a[27] = inv27;
@@ -440,8 +442,8 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u28_30 = bc(a[30], 9);
     if ((lm >> 28) & 1u) {
       const double m = a[28] * inv28;
-      a[29] -= m * u28_29;
-      a[30] -= m * u28_30;
+      a[29] = a[29] - m * u28_29;
+      a[30] = a[30] - m * u28_30;
       a[28] = m;
     }
     if (lane == 9) a[28] = inv28;
@@ -451,7 +453,7 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
     const double u29_30 = bc(a[30], 13);
     if ((lm >> 29) & 1u) {
       const double m = a[29] * inv29;
-      a[30] -= m * u29_30;
+      a[30] = a[30] - m * u29_30;
       a[29] = m;
     }
     if (lane == 13) a[29] = inv29;
@@ -475,8 +477,8 @@ __device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane
 // diagonal) is always final by then, so only its register is spoilt and
 // the recorded xs stays exact; the few components for which that is not
 // so keep their mask.  dinv: the lane's 1 / u_ii.
-template <class BC>
-__device__ __forceinline__ double lu_solve(const double (&a)[32], unsigned lm, unsigned um, int lane,
+template <class A, class BC>
+__device__ __forceinline__ double lu_solve(const A &a, unsigned lm, unsigned um, int lane,
                                           double b, const BC &bc, double *xs, int pos,
                                           int lev, double dinv) {
   const int lvf = lev & 0xff, lvb = lev >> 8;

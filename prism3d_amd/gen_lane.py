@@ -500,8 +500,10 @@ def generate_wave(net=None):
     w("// Afterwards a[k] holds the multiplier l_ik below the diagonal, 1 / u_ii")
     w("// on it and u_ij above it (structural entries).  The pivot rows of one")
     w("// dependency level are broadcast together, then applied in position order.")
-    w("template <class BC>")
-    w("__device__ __forceinline__ void lu_factor(double (&a)[32], unsigned lm, int lane, const BC &bc) {")
+    w("// A: the lane's row, a double[32] in registers or an accessor of its row in LDS")
+    w("// (bc then reads the pivot row there); the same statements serve both.")
+    w("template <class A, class BC>")
+    w("__device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC &bc) {")
     for g in groups(lev_f):
         w(f"  {{  // level {lev_f[g[0]]}: {', '.join(net.species[perm[k]] for k in g)}")
         for k in g:
@@ -518,7 +520,7 @@ def generate_wave(net=None):
                 w(f"      const double m = a[{k}] * inv{k};")
                 for j in range(k + 1, ns):
                     if F[k, j]:
-                        w(f"      a[{j}] -= m * u{k}_{j};")
+                        w(f"      a[{j}] = a[{j}] - m * u{k}_{j};")
                 w(f"      a[{k}] = m;")
                 w("    }")
             w(f"    if (lane == {P}) a[{k}] = inv{k};")
@@ -550,8 +552,8 @@ def generate_wave(net=None):
     w("// diagonal) is always final by then, so only its register is spoilt and")
     w("// the recorded xs stays exact; the few components for which that is not")
     w("// so keep their mask.  dinv: the lane's 1 / u_ii.")
-    w("template <class BC>")
-    w("__device__ __forceinline__ double lu_solve(const double (&a)[32], unsigned lm, unsigned um, int lane,")
+    w("template <class A, class BC>")
+    w("__device__ __forceinline__ double lu_solve(const A &a, unsigned lm, unsigned um, int lane,")
     w("                                          double b, const BC &bc, double *xs, int pos,")
     w("                                          int lev, double dinv) {")
     w("  const int lvf = lev & 0xff, lvb = lev >> 8;")
