@@ -291,6 +291,12 @@ constexpr int kRxPerLane = kRxPerLaneCap;
 struct RxCache {
   int t[kRxPerLane], a[kRxPerLane], b[kRxPerLane];
   double k[kRxPerLane];
+  // the rate's last factor by type (n_H two-body, n_H / 2 grain, 1 else), so
+  // rate = ((k y_a) y_b) fac with y_b = 1 when there is none: the same
+  // rounding as the per-type products (halving is exact)
+  __device__ __forceinline__ double fac(int q, const CellParams &p) const {
+    return t[q] == P3D_RX_TWO_BODY ? p.nH : (t[q] == P3D_RX_GRAIN ? p.nH * 0.5 : 1.0);
+  }
   __device__ __forceinline__ void load(const NetDev &N, const Lds &L, int lane) {
 #pragma unroll
     for (int q = 0; q < kRxPerLane; ++q) {
@@ -481,21 +487,16 @@ __device__ __forceinline__ double rhs_pieces(const NetDev &N, const CellParams &
                                              int lane) {
   double pm, dpm;
   pah_minus(L.ys[N.e_idx], p, pm, dpm);
+  // branch-free rates: every lane evaluates the same products
 #pragma unroll
   for (int q = 0; q < kRxPerLane; ++q) {
     const int t = rc.t[q];
-    if (t < 0) continue;
     const double ya = L.ys[rc.a[q]];
-    double rate;
-    if (t == P3D_RX_TWO_BODY) {
-      const double yb = (rc.b[q] == P3D_SP_PAH_MINUS) ? pm : L.ys[rc.b[q]];
-      rate = rc.k[q] * ya * yb * p.nH;
-    } else if (t == P3D_RX_GRAIN) {
-      rate = rc.k[q] * ya * p.nH * 0.5;
-    } else {
-      rate = rc.k[q] * ya;
-    }
-    L.part[2 * (lane + 64 * q)] = rate;
+    const int b = rc.b[q];
+    const bool two = t == P3D_RX_TWO_BODY;
+    const double yb = !two ? 1.0 : (b == P3D_SP_PAH_MINUS ? pm : L.ys[b < 0 ? 0 : b]);
+    const double rate = ((rc.k[q] * ya) * yb) * rc.fac(q, p);
+    if (t >= 0) L.part[2 * (lane + 64 * q)] = rate;
   }
   __syncthreads();
   double acc = 0.0;
