@@ -259,15 +259,17 @@ def forms_same_inputs(solver):
     """The in-loop chemistry of the converged state through the forms on the
     same inputs: the default batched persistent kernel (one cell per
     wavefront, variant 6), the persistent one-cell-per-lane kernel (variant
-    4) and the plain one-cell-per-workgroup drop-in (variant 2), each one
-    launch timed with HIP events on the launch stream, with their agreement
-    against the drop-in."""
+    4) and the plain one-cell-per-workgroup drop-in (variant 1, its default
+    form, the LU in LDS; variant 2, the same pivoted LU in registers with the
+    same results, is timed beside it), each one launch timed with HIP events
+    on the launch stream, with their agreement against the drop-in."""
     import torch
     ins = [t.reshape(-1) for t in (solver.density, solver.T_gas, solver.G0, solver.A_V,
                                    solver.zeta_CR, solver.fsh_H2, solver.fsh_CO)]
     out = {}
     ys = {}
-    for name, var in (("batched", 6), ("batched_lane", 4), ("plain_drop_in", 2)):
+    for name, var in (("batched", 6), ("batched_lane", 4), ("plain_drop_in", 1),
+                      ("plain_drop_in_registers", 2)):
         y = solver.y_init.clone()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -280,6 +282,7 @@ def forms_same_inputs(solver):
         ys[name] = y
     b = ys["plain_drop_in"]
     m = b > 1e-14
+    out["plain_drop_in_registers_bit_identical"] = bool(torch.equal(ys["plain_drop_in_registers"], b))
     for name in ("batched", "batched_lane"):
         rel = (torch.abs(ys[name] - b)[m] / b[m]).cpu().numpy()
         out[f"{name}_p999_rel_diff"] = float(np.percentile(rel, 99.9))
@@ -729,7 +732,8 @@ def forms_line(args):
     abundances to 1e7 yr at rtol 1e-9, atol 1e-20) through the default
     batched form (persistent one-cell-per-wavefront kernel, variant 6), the
     persistent one-cell-per-lane kernel (4) and the one-cell-per-workgroup
-    drop-in (2), on the same inputs: at the first fixed-point iteration and
+    drop-in (1, its default form; 2, the register LU with the same results,
+    beside it), on the same inputs: at the first fixed-point iteration and
     at the third.  The chemistry is >99.9% of each iteration's device time,
     so the ratio of these launches is the ratio of the coupled solves."""
     import torch
@@ -766,7 +770,8 @@ def forms_line(args):
             sv.rt_step(); sv.shielding_cr_step(); sv.dust_step()
         r = {}
         ys = {}
-        for name, var in (("batched", 6), ("batched_lane", 4), ("drop_in", 2)):
+        for name, var in (("batched", 6), ("batched_lane", 4), ("drop_in", 1),
+                          ("drop_in_registers", 2)):
             t, y, st = solve(var)
             r[f"s_{name}"] = t
             ys[name] = y
@@ -775,6 +780,8 @@ def forms_line(args):
                 r["bdf_steps_max"] = int(st[:, 0].max())
             print(f"[bench] {n}^3 {tag}: {name} {t:.2f} s", file=sys.stderr, flush=True)
         m = ys["drop_in"] > 1e-14
+        r["drop_in_registers_bit_identical"] = bool(torch.equal(ys["drop_in_registers"],
+                                                                ys["drop_in"]))
         for name in ("batched", "batched_lane"):
             rel = (torch.abs(ys[name] - ys["drop_in"])[m] / ys["drop_in"][m]).cpu().numpy()
             r[f"{name}_p999_rel_diff"] = float(np.percentile(rel, 99.9))
@@ -890,7 +897,7 @@ def headline(args):
     deadline = T_START + args.budget_s
     reserve = 0.0 if args.no_cpu else 35.0          # CPU replay + oracle check
     reserve += 0.0 if args.no_adjoint else 30.0     # differentiation workload
-    reserve += 0.0 if args.no_forms else 25.0       # the forms on the same inputs
+    reserve += 0.0 if args.no_forms else 35.0       # the forms on the same inputs
     solver = make_solver(args.n, args.variant, args.waves_per_cu, args.autotune)
     print(f"[bench] {args.n}^3 = {solver.n_cells} cells; budget {args.budget_s:.0f} s",
           file=sys.stderr, flush=True)
@@ -984,7 +991,7 @@ def headline(args):
                       "cells": solver.n_cells, "chem_form": form},
            "adjoint_over_forward": None,
            "roofline": roof}
-    if not args.no_forms and time.time() + 25 < deadline:
+    if not args.no_forms and time.time() + 35 < deadline:
         out["forms_same_inputs"] = forms_same_inputs(solver)
     if not args.no_adjoint and time.time() + 30 < deadline:
         adj = adjoint_measure(args.adjoint_n)

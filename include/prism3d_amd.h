@@ -168,8 +168,7 @@ typedef struct p3d_solve_opts {
   int32_t max_steps;   /* per span                                          */
   double x_c_total, x_o_total; /* element totals for the conservation step */
   int32_t variant;     /* kernel variant: 0 auto (6 when the network is the
-                          bundled one, else 2 for n_species <= 32, 5 for
-                          n_species >= 64, else 1),
+                          bundled one, else 5 for n_species >= 64, else 1),
                           1 LU in LDS, 2 LU in registers
                           (n_species <= 32), 3 LU in LDS with unrolled loops
                           (n_species <= 32), 5 LU in LDS with the panel

@@ -1089,7 +1089,7 @@ struct LinStatic {
   // solve loads the lane's row of the factors.
   static constexpr bool kNeedsM = true;
   unsigned pw0, pw1;  // this lane's RHS piece (rhs_pieces)
-  int pos, lev;       // this lane's elimination position and solve levels
+  int pos, lev;       // this lane's elimination position, solve and factorisation levels
   __device__ LinStatic() {
     pw0 = c_wave_pw0[threadIdx.x & 63];
     pw1 = c_wave_pw1[threadIdx.x & 63];
@@ -1116,7 +1116,7 @@ struct LinStatic {
       if (row) Mrow[q] = (col == lane ? 1.0 : 0.0) - c * jv;
     }
     LdsRow ar{Mrow};
-    wave::lu_factor(ar, lm, lane, LdsRowBcast{L.M, L.ld});
+    wave::lu_factor(ar, lm, lane, LdsRowBcast{L.M, L.ld}, pos, lev);
   }
 
   // The solves take the lane's row of the factors into registers first: the
@@ -3048,9 +3048,12 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
   const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
   int var = o.variant;
-  // auto: registers for small networks, the MFMA trailing update from 64
-  // species (where the panel products are 16 x 16 x 4 tiles), else LDS
-  if (var == 0) var = net->d.ns <= 32 ? 2 : (net->d.ns >= 64 ? 5 : 1);
+  // auto: the MFMA trailing update from 64 species (where the panel products
+  // are 16 x 16 x 4 tiles), else the LU in LDS.  The register LU (2) gives
+  // the same bits but needs 412 registers, one wavefront per SIMD: on the
+  // 64^3 workload's converged inputs it took 10.1 s against 7.1 s for the
+  // LDS LU (191 VGPRs, two per SIMD, no scratch).
+  if (var == 0) var = net->d.ns >= 64 ? 5 : 1;
   if (var == 5)
     hipLaunchKernelGGL(chem_solve_kernel<LinMfma>, dim3((unsigned)n), dim3(64), sh,
                        as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,

@@ -26,51 +26,55 @@ constexpr int kPieceLen = 6;
 #define P3D_WAVE_PIECE_W1 {0xffff0f0du, 0xffff423cu, 0xffff514eu, 0xffff807eu, 0xffff9290u, 0xffffffffu, 0xffff0906u, 0xffff1d1bu, 0xffff2b29u, 0xffff3533u, 0xffff4341u, 0xffff6357u, 0xffff7f7du, 0xffff8988u, 0xffff9391u, 0xffffffffu, 0xffff1e1cu, 0xffff3837u, 0xffff7b56u, 0xffffffffu, 0xffff5525u, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffff4644u, 0xffff5b1cu, 0xffffffffu, 0xffff4621u, 0xffff7d7bu, 0xffffffffu, 0xffff3d3au, 0xffffff87u, 0xffffff82u, 0xffffffffu, 0xffffffffu, 0xffff3126u, 0xffff706du, 0xffff7977u, 0xffffffffu, 0xffffffffu, 0xffff3533u, 0xffff9570u, 0xffffffffu, 0xffff866au, 0xffffffffu, 0xffffffffu, 0xffffff78u, 0xffffff84u, 0xffff8357u, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu}
 #define P3D_WAVE_ROW_PIECE {0, 6, 16, 20, 22, 23, 24, 25, 27, 30, 32, 33, 34, 35, 39, 40, 42, 43, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57}
 
-// a: the lane's row of M (positions); lm: the lane's kLMask entry.
-// Afterwards a[k] holds the multiplier l_ik below the diagonal, 1 / u_ii
-// on it and u_ij above it (structural entries).  The pivot rows of one
-// dependency level are broadcast together, then applied in position order.
-// A: the lane's row, a double[32] in registers or an accessor of its row in LDS
-// (bc then reads the pivot row there); the same statements serve both.
+// a: the lane's row of M (positions); lm: the lane's kLMask entry; pos, lev:
+// its elimination position and P3D_WAVE_LEVELS entry.  Afterwards a[k] holds
+// the multiplier l_ik below the diagonal, 1 / u_ii on it and u_ij above it
+// (structural entries).  At each dependency level the level's pivots invert
+// their diagonals in place at once, then the pivot rows are broadcast
+// together and applied in position order.  A: an accessor of the lane's row
+// in LDS (a[pos] indexes it at run time; bc reads the pivot row there).
 template <class A, class BC>
-__device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC &bc) {
+__device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC &bc,
+                                          int pos, int lev) {
+  const int lvk = lev >> 16;
   {  // level 1: H2+, S, Si, Fe, H-, He, O+, CH2+, H3O+, O2, HCO+
-    const double inv0 = 1.0 / bc(a[0], 5);
+    if (lvk == 1) a[pos] = 1.0 / a[pos];
+    const double inv0 = bc(a[0], 5);
     const double u0_26 = bc(a[26], 5);
-    const double inv1 = 1.0 / bc(a[1], 23);
+    const double inv1 = bc(a[1], 23);
     const double u1_2 = bc(a[2], 23);
     const double u1_24 = bc(a[24], 23);
-    const double inv3 = 1.0 / bc(a[3], 25);
+    const double inv3 = bc(a[3], 25);
     const double u3_4 = bc(a[4], 25);
     const double u3_24 = bc(a[24], 25);
-    const double inv5 = 1.0 / bc(a[5], 27);
+    const double inv5 = bc(a[5], 27);
     const double u5_6 = bc(a[6], 27);
     const double u5_24 = bc(a[24], 27);
-    const double inv7 = 1.0 / bc(a[7], 4);
+    const double inv7 = bc(a[7], 4);
     const double u7_24 = bc(a[24], 4);
     const double u7_25 = bc(a[25], 4);
-    const double inv8 = 1.0 / bc(a[8], 29);
+    const double inv8 = bc(a[8], 29);
     const double u8_9 = bc(a[9], 29);
     const double u8_24 = bc(a[24], 29);
     const double u8_28 = bc(a[28], 29);
-    const double inv10 = 1.0 / bc(a[10], 14);
+    const double inv10 = bc(a[10], 14);
     const double u10_13 = bc(a[13], 14);
     const double u10_25 = bc(a[25], 14);
     const double u10_29 = bc(a[29], 14);
-    const double inv11 = 1.0 / bc(a[11], 12);
+    const double inv11 = bc(a[11], 12);
     const double u11_17 = bc(a[17], 12);
     const double u11_24 = bc(a[24], 12);
     const double u11_26 = bc(a[26], 12);
-    const double inv12 = 1.0 / bc(a[12], 19);
+    const double inv12 = bc(a[12], 19);
     const double u12_14 = bc(a[14], 19);
     const double u12_24 = bc(a[24], 19);
     const double u12_26 = bc(a[26], 19);
-    const double inv15 = 1.0 / bc(a[15], 20);
+    const double inv15 = bc(a[15], 20);
     const double u15_22 = bc(a[22], 20);
     const double u15_23 = bc(a[23], 20);
     const double u15_27 = bc(a[27], 20);
     const double u15_29 = bc(a[29], 20);
-    const double inv18 = 1.0 / bc(a[18], 22);
+    const double inv18 = bc(a[18], 22);
     const double u18_19 = bc(a[19], 22);
     const double u18_20 = bc(a[20], 22);
     const double u18_21 = bc(a[21], 22);
@@ -85,35 +89,30 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[26] = a[26] - m * u0_26;
       a[0] = m;
     }
-    if (lane == 5) a[0] = inv0;
     if ((lm >> 1) & 1u) {
       const double m = a[1] * inv1;
       a[2] = a[2] - m * u1_2;
       a[24] = a[24] - m * u1_24;
       a[1] = m;
     }
-    if (lane == 23) a[1] = inv1;
     if ((lm >> 3) & 1u) {
       const double m = a[3] * inv3;
       a[4] = a[4] - m * u3_4;
       a[24] = a[24] - m * u3_24;
       a[3] = m;
     }
-    if (lane == 25) a[3] = inv3;
     if ((lm >> 5) & 1u) {
       const double m = a[5] * inv5;
       a[6] = a[6] - m * u5_6;
       a[24] = a[24] - m * u5_24;
       a[5] = m;
     }
-    if (lane == 27) a[5] = inv5;
     if ((lm >> 7) & 1u) {
       const double m = a[7] * inv7;
       a[24] = a[24] - m * u7_24;
       a[25] = a[25] - m * u7_25;
       a[7] = m;
     }
-    if (lane == 4) a[7] = inv7;
     if ((lm >> 8) & 1u) {
       const double m = a[8] * inv8;
       a[9] = a[9] - m * u8_9;
@@ -121,7 +120,6 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[28] = a[28] - m * u8_28;
       a[8] = m;
     }
-    if (lane == 29) a[8] = inv8;
     if ((lm >> 10) & 1u) {
       const double m = a[10] * inv10;
       a[13] = a[13] - m * u10_13;
@@ -129,7 +127,6 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[29] = a[29] - m * u10_29;
       a[10] = m;
     }
-    if (lane == 14) a[10] = inv10;
     if ((lm >> 11) & 1u) {
       const double m = a[11] * inv11;
       a[17] = a[17] - m * u11_17;
@@ -137,7 +134,6 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[26] = a[26] - m * u11_26;
       a[11] = m;
     }
-    if (lane == 12) a[11] = inv11;
     if ((lm >> 12) & 1u) {
       const double m = a[12] * inv12;
       a[14] = a[14] - m * u12_14;
@@ -145,7 +141,6 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[26] = a[26] - m * u12_26;
       a[12] = m;
     }
-    if (lane == 19) a[12] = inv12;
     if ((lm >> 15) & 1u) {
       const double m = a[15] * inv15;
       a[22] = a[22] - m * u15_22;
@@ -154,7 +149,6 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[29] = a[29] - m * u15_29;
       a[15] = m;
     }
-    if (lane == 20) a[15] = inv15;
     if ((lm >> 18) & 1u) {
       const double m = a[18] * inv18;
       a[19] = a[19] - m * u18_19;
@@ -168,30 +162,30 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[30] = a[30] - m * u18_30;
       a[18] = m;
     }
-    if (lane == 22) a[18] = inv18;
   }
   {  // level 2: S+, Si+, Fe+, He+, H+, H2O+, CH+
-    const double inv2 = 1.0 / bc(a[2], 24);
+    if (lvk == 2) a[pos] = 1.0 / a[pos];
+    const double inv2 = bc(a[2], 24);
     const double u2_24 = bc(a[24], 24);
-    const double inv4 = 1.0 / bc(a[4], 26);
+    const double inv4 = bc(a[4], 26);
     const double u4_24 = bc(a[24], 26);
-    const double inv6 = 1.0 / bc(a[6], 28);
+    const double inv6 = bc(a[6], 28);
     const double u6_24 = bc(a[24], 28);
-    const double inv9 = 1.0 / bc(a[9], 30);
+    const double inv9 = bc(a[9], 30);
     const double u9_24 = bc(a[24], 30);
     const double u9_28 = bc(a[28], 30);
-    const double inv13 = 1.0 / bc(a[13], 3);
+    const double inv13 = bc(a[13], 3);
     const double u13_22 = bc(a[22], 3);
     const double u13_23 = bc(a[23], 3);
     const double u13_24 = bc(a[24], 3);
     const double u13_25 = bc(a[25], 3);
     const double u13_29 = bc(a[29], 3);
     const double u13_30 = bc(a[30], 3);
-    const double inv14 = 1.0 / bc(a[14], 18);
+    const double inv14 = bc(a[14], 18);
     const double u14_16 = bc(a[16], 18);
     const double u14_24 = bc(a[24], 18);
     const double u14_26 = bc(a[26], 18);
-    const double inv17 = 1.0 / bc(a[17], 11);
+    const double inv17 = bc(a[17], 11);
     const double u17_20 = bc(a[20], 11);
     const double u17_22 = bc(a[22], 11);
     const double u17_24 = bc(a[24], 11);
@@ -202,26 +196,22 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[24] = a[24] - m * u2_24;
       a[2] = m;
     }
-    if (lane == 24) a[2] = inv2;
     if ((lm >> 4) & 1u) {
       const double m = a[4] * inv4;
       a[24] = a[24] - m * u4_24;
       a[4] = m;
     }
-    if (lane == 26) a[4] = inv4;
     if ((lm >> 6) & 1u) {
       const double m = a[6] * inv6;
       a[24] = a[24] - m * u6_24;
       a[6] = m;
     }
-    if (lane == 28) a[6] = inv6;
     if ((lm >> 9) & 1u) {
       const double m = a[9] * inv9;
       a[24] = a[24] - m * u9_24;
       a[28] = a[28] - m * u9_28;
       a[9] = m;
     }
-    if (lane == 30) a[9] = inv9;
     if ((lm >> 13) & 1u) {
       const double m = a[13] * inv13;
       a[22] = a[22] - m * u13_22;
@@ -232,7 +222,6 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[30] = a[30] - m * u13_30;
       a[13] = m;
     }
-    if (lane == 3) a[13] = inv13;
     if ((lm >> 14) & 1u) {
       const double m = a[14] * inv14;
       a[16] = a[16] - m * u14_16;
@@ -240,7 +229,6 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[26] = a[26] - m * u14_26;
       a[14] = m;
     }
-    if (lane == 18) a[14] = inv14;
     if ((lm >> 17) & 1u) {
       const double m = a[17] * inv17;
       a[20] = a[20] - m * u17_20;
@@ -250,15 +238,15 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[27] = a[27] - m * u17_27;
       a[17] = m;
     }
-    if (lane == 11) a[17] = inv17;
   }
   {  // level 3: OH+, CH
-    const double inv16 = 1.0 / bc(a[16], 16);
+    if (lvk == 3) a[pos] = 1.0 / a[pos];
+    const double inv16 = bc(a[16], 16);
     const double u16_20 = bc(a[20], 16);
     const double u16_24 = bc(a[24], 16);
     const double u16_26 = bc(a[26], 16);
     const double u16_29 = bc(a[29], 16);
-    const double inv19 = 1.0 / bc(a[19], 10);
+    const double inv19 = bc(a[19], 10);
     const double u19_20 = bc(a[20], 10);
     const double u19_22 = bc(a[22], 10);
     const double u19_24 = bc(a[24], 10);
@@ -274,7 +262,6 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[29] = a[29] - m * u16_29;
       a[16] = m;
     }
-    if (lane == 16) a[16] = inv16;
     if ((lm >> 19) & 1u) {
       const double m = a[19] * inv19;
       a[20] = a[20] - m * u19_20;
@@ -286,10 +273,10 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[29] = a[29] - m * u19_29;
       a[19] = m;
     }
-    if (lane == 10) a[19] = inv19;
   }
   {  // level 4: H3+
-    const double inv20 = 1.0 / bc(a[20], 6);
+    if (lvk == 4) a[pos] = 1.0 / a[pos];
+    const double inv20 = bc(a[20], 6);
     const double u20_24 = bc(a[24], 6);
     const double u20_26 = bc(a[26], 6);
     const double u20_27 = bc(a[27], 6);
@@ -304,10 +291,10 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[29] = a[29] - m * u20_29;
       a[20] = m;
     }
-    if (lane == 6) a[20] = inv20;
   }
   {  // level 5: H2O
-    const double inv21 = 1.0 / bc(a[21], 17);
+    if (lvk == 5) a[pos] = 1.0 / a[pos];
+    const double inv21 = bc(a[21], 17);
     const double u21_22 = bc(a[22], 17);
     const double u21_23 = bc(a[23], 17);
     const double u21_24 = bc(a[24], 17);
@@ -326,10 +313,10 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[29] = a[29] - m * u21_29;
       a[21] = m;
     }
-    if (lane == 17) a[21] = inv21;
   }
   {  // level 6: C+
-    const double inv22 = 1.0 / bc(a[22], 7);
+    if (lvk == 6) a[pos] = 1.0 / a[pos];
+    const double inv22 = bc(a[22], 7);
     const double u22_23 = bc(a[23], 7);
     const double u22_24 = bc(a[24], 7);
     const double u22_26 = bc(a[26], 7);
@@ -346,10 +333,10 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[29] = a[29] - m * u22_29;
       a[22] = m;
     }
-    if (lane == 7) a[22] = inv22;
   }
   {  // level 7: OH
-    const double inv23 = 1.0 / bc(a[23], 15);
+    if (lvk == 7) a[pos] = 1.0 / a[pos];
+    const double inv23 = bc(a[23], 15);
     const double u23_24 = bc(a[24], 15);
     const double u23_25 = bc(a[25], 15);
     const double u23_26 = bc(a[26], 15);
@@ -366,10 +353,10 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[29] = a[29] - m * u23_29;
       a[23] = m;
     }
-    if (lane == 15) a[23] = inv23;
   }
   {  // level 8: e-
-    const double inv24 = 1.0 / bc(a[24], 0);
+    if (lvk == 8) a[pos] = 1.0 / a[pos];
+    const double inv24 = bc(a[24], 0);
     const double u24_25 = bc(a[25], 0);
     const double u24_26 = bc(a[26], 0);
     const double u24_27 = bc(a[27], 0);
@@ -386,10 +373,10 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[30] = a[30] - m * u24_30;
       a[24] = m;
     }
-    if (lane == 0) a[24] = inv24;
   }
   {  // level 9: H
-    const double inv25 = 1.0 / bc(a[25], 1);
+    if (lvk == 9) a[pos] = 1.0 / a[pos];
+    const double inv25 = bc(a[25], 1);
     const double u25_26 = bc(a[26], 1);
     const double u25_27 = bc(a[27], 1);
     const double u25_28 = bc(a[28], 1);
@@ -404,10 +391,10 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[30] = a[30] - m * u25_30;
       a[25] = m;
     }
-    if (lane == 1) a[25] = inv25;
   }
   {  // level 10: H2
-    const double inv26 = 1.0 / bc(a[26], 2);
+    if (lvk == 10) a[pos] = 1.0 / a[pos];
+    const double inv26 = bc(a[26], 2);
     const double u26_27 = bc(a[27], 2);
     const double u26_28 = bc(a[28], 2);
     const double u26_29 = bc(a[29], 2);
@@ -420,10 +407,10 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[30] = a[30] - m * u26_30;
       a[26] = m;
     }
-    if (lane == 2) a[26] = inv26;
   }
   {  // level 11: C
-    const double inv27 = 1.0 / bc(a[27], 8);
+    if (lvk == 11) a[pos] = 1.0 / a[pos];
+    const double inv27 = bc(a[27], 8);
     const double u27_28 = bc(a[28], 8);
     const double u27_29 = bc(a[29], 8);
     const double u27_30 = bc(a[30], 8);
@@ -434,10 +421,10 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[30] = a[30] - m * u27_30;
       a[27] = m;
     }
-    if (lane == 8) a[27] = inv27;
   }
   {  // level 12: CO
-    const double inv28 = 1.0 / bc(a[28], 9);
+    if (lvk == 12) a[pos] = 1.0 / a[pos];
+    const double inv28 = bc(a[28], 9);
     const double u28_29 = bc(a[29], 9);
     const double u28_30 = bc(a[30], 9);
     if ((lm >> 28) & 1u) {
@@ -446,28 +433,27 @@ __device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC 
       a[30] = a[30] - m * u28_30;
       a[28] = m;
     }
-    if (lane == 9) a[28] = inv28;
   }
   {  // level 13: O
-    const double inv29 = 1.0 / bc(a[29], 13);
+    if (lvk == 13) a[pos] = 1.0 / a[pos];
+    const double inv29 = bc(a[29], 13);
     const double u29_30 = bc(a[30], 13);
     if ((lm >> 29) & 1u) {
       const double m = a[29] * inv29;
       a[30] = a[30] - m * u29_30;
       a[29] = m;
     }
-    if (lane == 13) a[29] = inv29;
   }
   {  // level 14: CO+
-    const double inv30 = 1.0 / bc(a[30], 21);
-    if (lane == 21) a[30] = inv30;
+    if (lvk == 14) a[pos] = 1.0 / a[pos];
+    const double inv30 = bc(a[30], 21);
   }
 }
 
 // per lane: elimination position (31: no row) and the forward | backward << 8
-// dependency levels of the solves
+// dependency levels of the solves | the factorisation's level << 16
 #define P3D_WAVE_POS {24, 25, 26, 13, 7, 0, 20, 22, 27, 28, 19, 17, 11, 29, 10, 23, 16, 21, 14, 12, 15, 30, 18, 1, 2, 3, 4, 5, 6, 8, 9, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31, 31}
-#define P3D_WAVE_LEVELS {1798, 1543, 1288, 2562, 2049, 1537, 2050, 2308, 1033, 778, 2562, 2561, 2817, 523, 2817, 2053, 2305, 2563, 2561, 2817, 2561, 268, 2817, 2305, 2050, 2305, 2050, 2305, 2050, 2305, 2050, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}
+#define P3D_WAVE_LEVELS {526086, 591367, 656648, 133634, 67585, 67073, 264194, 395524, 721929, 787210, 199170, 133633, 68353, 852491, 68353, 460805, 198913, 330243, 133633, 68353, 68097, 917772, 68353, 67841, 133122, 67841, 133122, 67841, 133122, 67841, 133122, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}
 
 // Solve M x = b with the factors; b and x: the lane's species component.
 // Level by level: the components of one level are broadcast, each lane of
@@ -481,7 +467,7 @@ template <class A, class BC>
 __device__ __forceinline__ double lu_solve(const A &a, unsigned lm, unsigned um, int lane,
                                           double b, const BC &bc, double *xs, int pos,
                                           int lev, double dinv) {
-  const int lvf = lev & 0xff, lvb = lev >> 8;
+  const int lvf = lev & 0xff, lvb = (lev >> 8) & 0xff;
   {
     const double x0 = bc(b, 5);
     const double x1 = bc(b, 23);

@@ -496,19 +496,25 @@ def generate_wave(net=None):
     w("#define P3D_WAVE_PIECE_W1 {" + ", ".join(f"0x{b_:08x}u" for _, b_ in words) + "}")
     w("#define P3D_WAVE_ROW_PIECE {" + ", ".join(str(x) for x in row_pc) + "}")
     w("")
-    w("// a: the lane's row of M (positions); lm: the lane's kLMask entry.")
-    w("// Afterwards a[k] holds the multiplier l_ik below the diagonal, 1 / u_ii")
-    w("// on it and u_ij above it (structural entries).  The pivot rows of one")
-    w("// dependency level are broadcast together, then applied in position order.")
-    w("// A: the lane's row, a double[32] in registers or an accessor of its row in LDS")
-    w("// (bc then reads the pivot row there); the same statements serve both.")
+    w("// a: the lane's row of M (positions); lm: the lane's kLMask entry; pos, lev:")
+    w("// its elimination position and P3D_WAVE_LEVELS entry.  Afterwards a[k] holds")
+    w("// the multiplier l_ik below the diagonal, 1 / u_ii on it and u_ij above it")
+    w("// (structural entries).  At each dependency level the level's pivots invert")
+    w("// their diagonals in place at once, then the pivot rows are broadcast")
+    w("// together and applied in position order.  A: an accessor of the lane's row")
+    w("// in LDS (a[pos] indexes it at run time; bc reads the pivot row there).")
     w("template <class A, class BC>")
-    w("__device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC &bc) {")
+    w("__device__ __forceinline__ void lu_factor(A &a, unsigned lm, int lane, const BC &bc,")
+    w("                                          int pos, int lev) {")
+    w("  const int lvk = lev >> 16;")
     for g in groups(lev_f):
         w(f"  {{  // level {lev_f[g[0]]}: {', '.join(net.species[perm[k]] for k in g)}")
+        # the pivots of this level invert their own diagonals in one vector
+        # division; the other lanes read the stored reciprocals
+        w(f"    if (lvk == {lev_f[g[0]]}) a[pos] = 1.0 / a[pos];")
         for k in g:
             P = perm[k]
-            w(f"    const double inv{k} = 1.0 / bc(a[{k}], {P});")
+            w(f"    const double inv{k} = bc(a[{k}], {P});")
             if has_l[k]:
                 for j in range(k + 1, ns):
                     if F[k, j]:
@@ -523,14 +529,14 @@ def generate_wave(net=None):
                         w(f"      a[{j}] = a[{j}] - m * u{k}_{j};")
                 w(f"      a[{k}] = m;")
                 w("    }")
-            w(f"    if (lane == {P}) a[{k}] = inv{k};")
         w("  }")
     w("}")
     w("")
     # per lane (species): elimination position and its forward / backward
     # solve levels (inactive lanes: a spare slot and level 0, never written)
     lpos = [pos[s_] if s_ < ns else 31 for s_ in range(64)]
-    llev = [(lev_r[pos[s_]] | (lev_b[pos[s_]] << 8)) if s_ < ns else 0 for s_ in range(64)]
+    llev = [(lev_r[pos[s_]] | (lev_b[pos[s_]] << 8) | (lev_f[pos[s_]] << 16))
+            if s_ < ns else 0 for s_ in range(64)]
     # a component whose broadcast reaches a lane that is not final yet has to
     # be masked: forward, lane i with an upper entry at k > pos(i) in a later
     # level than k; backward, lane i with a multiplier at k < pos(i) in a
@@ -540,7 +546,7 @@ def generate_wave(net=None):
     bw_mask = [any(F[i, k] and k < i and lev_b[i] > lev_b[k] for i in range(ns))
                for k in range(ns)]
     w("// per lane: elimination position (31: no row) and the forward | backward << 8")
-    w("// dependency levels of the solves")
+    w("// dependency levels of the solves | the factorisation's level << 16")
     w("#define P3D_WAVE_POS {" + ", ".join(str(x) for x in lpos) + "}")
     w("#define P3D_WAVE_LEVELS {" + ", ".join(str(x) for x in llev) + "}")
     w("")
@@ -556,7 +562,7 @@ def generate_wave(net=None):
     w("__device__ __forceinline__ double lu_solve(const A &a, unsigned lm, unsigned um, int lane,")
     w("                                          double b, const BC &bc, double *xs, int pos,")
     w("                                          int lev, double dinv) {")
-    w("  const int lvf = lev & 0xff, lvb = lev >> 8;")
+    w("  const int lvf = lev & 0xff, lvb = (lev >> 8) & 0xff;")
     for g in groups(lev_r):
         w("  {")
         gl = [k for k in g if has_l[k]]

@@ -55,8 +55,19 @@ def _run(lines, env):
         if m:
             stack.append(((env["lm"] >> int(m.group(1))) & 1).astype(bool))
             continue
-        if l == "const int lvf = lev & 0xff, lvb = lev >> 8;":
-            env["lvf"], env["lvb"] = env["lev"] & 0xFF, env["lev"] >> 8
+        if l in ("const int lvf = lev & 0xff, lvb = lev >> 8;",
+                 "const int lvf = lev & 0xff, lvb = (lev >> 8) & 0xff;"):
+            env["lvf"], env["lvb"] = env["lev"] & 0xFF, (env["lev"] >> 8) & 0xFF
+            continue
+        if l == "const int lvk = lev >> 16;":
+            env["lvk"] = env["lev"] >> 16
+            continue
+        m = re.match(r"if \(lvk == (\d+)\) a\[pos\] = 1\.0 / a\[pos\];$", l)
+        if m:
+            # the pivots of this factorisation level invert their own diagonals
+            sel = (env["lvk"] == int(m.group(1))) & act()
+            lanes = np.nonzero(sel)[0]
+            env["a"][env["pos"][lanes], lanes] = 1.0 / env["a"][env["pos"][lanes], lanes]
             continue
         m = re.match(r"if \((lvf|lvb) == (\d+)\) xs\[pos\] = (b|bx);$", l)
         if m:
@@ -144,7 +155,8 @@ def test_generated_register_lu_solves_like_lapack(seed):
             a[q, s] = (1.0 if perm[q] == s else 0.0) - c * J[s, perm[q]]
     lm = np.array([lmask[s] if s < ns else 0 for s in lanes], dtype=np.int64)
     um = np.array([umask[s] if s < ns else 0 for s in lanes], dtype=np.int64)
-    env = {"a": a, "lm": lm, "um": um, "lane": lanes, "b": np.zeros(64)}
+    env = {"a": a, "lm": lm, "um": um, "lane": lanes, "b": np.zeros(64), "pos": lpos,
+           "lev": llev}
     _run(fac, env)
     rhs = rng.normal(size=ns)
     # the solve's inputs as LinStatic::solve passes them: the lane's 1 / u_ii
