@@ -60,6 +60,13 @@ struct NetDev {
   const int *jrow, *jcol, *joff;  // nnz, nnz, nnz+1
   const int *jt_rxn, *jt_slot;
   const double *jt_coef;
+  // the same Jacobian sums on a balanced schedule (one-wavefront register-LU
+  // form): whole entries spread over the 64 lanes longest first, each summed
+  // in its own term order; lane l's slot s is jsched[64 s + l], packed as
+  // part index | int8 coefficient << 8 | entry << 16 | last term << 25 |
+  // used << 26 (0 when the network does not fit the packing)
+  int jslots;
+  const unsigned *jsched;
   const double *post_w;  // 4 * ns
   // balanced RHS gather (the two-wavefront form, and the one-wavefront BDF
   // solve of networks past the register cache): the packed entries (rhs_pk,
@@ -187,8 +194,8 @@ __device__ __forceinline__ Lds carve_wave(double *base, const NetDev &) {
   L.part = base + wave::kNR;
   L.ys = base + 3 * wave::kNR;
   L.J = base + 3 * wave::kNR + kMaxSpecies;
-  L.M = base + 3 * wave::kNR + kMaxSpecies + wave::kNNZ;
-  L.ru = base + 3 * wave::kNR + kMaxSpecies + wave::kNNZ + wave::kNS * ld;
+  L.M = base + 3 * wave::kNR + kMaxSpecies + wave::kNNZ + 1;  // J's spare slot
+  L.ru = L.M + (wave::kNS + 1) * ld;
   L.psum = L.ru + 36;
   L.xs = L.psum + 64;
   L.piv = nullptr;
@@ -198,8 +205,8 @@ __device__ __forceinline__ Lds carve_wave(double *base, const NetDev &) {
   return L;
 }
 static size_t lds_bytes_wave() {
-  return sizeof(double) * (size_t)(3 * wave::kNR + kMaxSpecies + wave::kNNZ +
-                                   wave::kNS * (wave::kNS | 1) + 36 + 64 + 32);
+  return sizeof(double) * (size_t)(3 * wave::kNR + kMaxSpecies + wave::kNNZ + 1 +
+                                   (wave::kNS + 1) * (wave::kNS | 1) + 36 + 64 + 32);
 }
 
 static size_t lds_bytes(int ns, int nr, int n_rhs_e, bool with_m = true,
@@ -496,17 +503,20 @@ __device__ __forceinline__ double rhs_pieces(const NetDev &N, const CellParams &
     const bool two = t == P3D_RX_TWO_BODY;
     const double yb = !two ? 1.0 : (b == P3D_SP_PAH_MINUS ? pm : L.ys[b < 0 ? 0 : b]);
     const double rate = ((rc.k[q] * ya) * yb) * rc.fac(q, p);
-    if (t >= 0) L.part[2 * (lane + 64 * q)] = rate;
+    // every lane holds a reaction in the first round (64 < kNR): no branch
+    if (64 * q + 63 < wave::kNR || t >= 0) L.part[2 * (lane + 64 * q)] = rate;
   }
   __syncthreads();
+  // piece sums without branches: an unused slot reads part[0] and keeps acc;
+  // acc + (-v) is acc - v bit for bit
   double acc = 0.0;
 #pragma unroll
   for (int q = 0; q < wave::kPieceLen; ++q) {
     const unsigned e = ((q < 4 ? pw0 : pw1) >> (8 * (q & 3))) & 0xFFu;
-    if (e != 0xFFu) {
-      const double v = L.part[2 * (e >> 1)];
-      acc = (e & 1u) ? acc + v : acc - v;
-    }
+    const bool use = e != 0xFFu;
+    const double v = L.part[use ? 2 * (e >> 1) : 0];
+    const double sv = (e & 1u) ? v : -v;
+    acc = use ? acc + sv : acc;
   }
   L.psum[lane] = acc;
   __syncthreads();
@@ -554,6 +564,30 @@ __device__ void jacobian_sparse(const NetDev &N, const CellParams &p, const Lds 
     for (int t = N.joff[z]; t < N.joff[z + 1]; ++t)
       acc += N.jt_coef[t] * L.part[2 * N.jt_rxn[t] + N.jt_slot[t]];
     L.J[z] = acc;
+  }
+  __syncthreads();
+}
+
+// jacobian_sparse's sums on the balanced schedule (N.jsched): every lane runs
+// the same N.jslots slots, so no lane waits on the longest entry's loop and
+// the slots' table reads are independent of the data.  Each entry is summed
+// by one lane in its term order from 0, bit for bit as jacobian_sparse; a
+// slot that ends no entry writes the spare slot J[kNNZ].
+template <int W>
+__device__ void jacobian_sched(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
+  eval_reactions<W>(N, p, L, lane, true);
+  const unsigned *sq = N.jsched + lane;
+  double acc = 0.0;
+#pragma unroll 4
+  for (int s = 0; s < N.jslots; ++s) {
+    const unsigned w = sq[64 * s];
+    const double v = L.part[w & 0xFFu];
+    const double c = (double)(int)(int8_t)((w >> 8) & 0xFFu);
+    const double nacc = acc + c * v;
+    acc = (w >> 26) & 1u ? nacc : acc;
+    const bool last = (w >> 25) & 1u;
+    L.J[last ? (int)((w >> 16) & 0x1FFu) : wave::kNNZ] = acc;
+    acc = last ? 0.0 : acc;
   }
   __syncthreads();
 }
@@ -1106,14 +1140,16 @@ struct LinStatic {
     const unsigned lm = row ? c_wave_lmask[lane] : 0u;
     const unsigned jm = row ? N.jr_mask[lane] : 0u;
     const int js = row ? N.jr_start[lane] : 0;
-    double *Mrow = L.M + (row ? lane : 0) * L.ld;
+    // lanes past the species write a spare row (kNS), so the loop has no branches
+    double *Mrow = L.M + (row ? lane : wave::kNS) * L.ld;
 #pragma unroll
     for (int q = 0; q < wave::kNS; ++q) {
       const int col = wave::kPerm[q];
       const bool has = (jm >> col) & 1u;
       const int z = has ? js + __builtin_popcount(jm & ((1u << col) - 1u)) : js;
-      const double jv = has ? L.J[z] : 0.0;
-      if (row) Mrow[q] = (col == lane ? 1.0 : 0.0) - c * jv;
+      const double jz = L.J[z];
+      const double jv = has ? jz : 0.0;
+      Mrow[q] = (col == lane ? 1.0 : 0.0) - c * jv;
     }
     LdsRow ar{Mrow};
     wave::lu_factor(ar, lm, lane, LdsRowBcast{L.M, L.ld}, pos, lev);
@@ -1126,11 +1162,12 @@ struct LinStatic {
     const bool row = lane < wave::kNS;
     const unsigned lm = row ? c_wave_lmask[lane] : 0u;
     const unsigned um = row ? c_wave_umask[lane] : 0u;
-    const double *Mrow = L.M + (row ? lane : 0) * L.ld;
+    // lanes past the species read the spare row: their values are never used
+    const double *Mrow = L.M + (row ? lane : wave::kNS) * L.ld;
     double a[32];
 #pragma unroll
-    for (int q = 0; q < 32; ++q) a[q] = (row && q < wave::kNS) ? Mrow[q] : 0.0;
-    const double dinv = row ? Mrow[pos] : 0.0;  // the factor's 1 / u_ii
+    for (int q = 0; q < 32; ++q) a[q] = q < wave::kNS ? Mrow[q] : 0.0;
+    const double dinv = Mrow[pos < wave::kNS ? pos : 0];  // the factor's 1 / u_ii
     if (lane >= N.ns) b = 0.0;
     const double x = wave::lu_solve(a, lm, um, lane, b, WaveBcast{}, L.xs, pos, lev, dinv);
     return row ? x : 0.0;
@@ -1144,7 +1181,7 @@ __device__ __forceinline__ double rms(double v, int ns) { return sqrt(wave_sum(v
 template <class Lin, class G>
 __device__ __forceinline__ void jac_eval_for(const NetDev &N, const CellParams &p, const Lds &L,
                                              int lane) {
-  if constexpr (std::is_same<Lin, LinStatic>::value) jacobian_sparse<64>(N, p, L, lane);
+  if constexpr (std::is_same<Lin, LinStatic>::value) jacobian_sched<64>(N, p, L, lane);
   else jac_eval<G>(N, p, L, lane);
 }
 
@@ -2254,7 +2291,7 @@ chem_tape_adjoint_kernel(NetDev N, int64_t n, const double *__restrict__ y0,
           const double scale = o.atol + o.rtol * fabs(ypred);
           if (!jac_ok) {
             publish_y(N, L, ypred, lane);
-            jacobian_sparse<64>(N, p, L, lane);
+            jacobian_sched<64>(N, p, L, lane);
             jac_ok = true;
             c_lu = -1.0;
           }
@@ -2294,7 +2331,7 @@ chem_tape_adjoint_kernel(NetDev N, int64_t n, const double *__restrict__ y0,
             }
             if (!conv) {  // refresh the Jacobian at the current iterate
               publish_y(N, L, ynew, lane);
-              jacobian_sparse<64>(N, p, L, lane);
+              jacobian_sched<64>(N, p, L, lane);
               c_lu = -1.0;
             }
           }
@@ -2311,7 +2348,7 @@ chem_tape_adjoint_kernel(NetDev N, int64_t n, const double *__restrict__ y0,
           // factors of I - c J(y_n+1): kept in L.M for the next step's
           // Newton, stored in the column layout for the backward pass
           publish_y(N, L, D[0], lane);
-          jacobian_sparse<64>(N, p, L, lane);
+          jacobian_sched<64>(N, p, L, lane);
           lin.factor(N, L, c, lane);
           c_lu = c;
           double *sp = seg + (size_t)(i - i0) * kTapeCol * 64;
@@ -2678,6 +2715,41 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
     if (jr_start[r] > jr_start[r + 1]) jr_start[r] = jr_start[r + 1];
   for (int z = 0; z < nnz; ++z)
     if (jcol[z] < 32) jr_mask[jrow[z]] |= 1u << jcol[z];
+  // balanced Jacobian schedule (longest-processing-time assignment of whole
+  // entries to the 64 lanes; ties to the lowest lane)
+  bool sched_ok = nnz < 512;
+  for (size_t t = 0; t < jt_rxn.size() && sched_ok; ++t)
+    sched_ok = 2 * jt_rxn[t] + jt_slot[t] < 256 && jt_coef[t] == std::nearbyint(jt_coef[t]) &&
+               std::fabs(jt_coef[t]) <= 127.0;
+  int jslots = 0;
+  std::vector<unsigned> jsched;
+  if (sched_ok) {
+    std::vector<int> order(nnz);
+    for (int z = 0; z < nnz; ++z) order[z] = z;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return joff[a + 1] - joff[a] > joff[b + 1] - joff[b];
+    });
+    std::vector<int> load(64, 0);
+    std::vector<std::vector<int>> mine(64);
+    for (int z : order) {
+      const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      mine[l].push_back(z);
+      load[l] += joff[z + 1] - joff[z];
+    }
+    jslots = *std::max_element(load.begin(), load.end());
+    jsched.assign(64 * (size_t)std::max(jslots, 1), 0u);
+    for (int l = 0; l < 64; ++l) {
+      int sl = 0;
+      for (int z : mine[l])
+        for (int t = joff[z]; t < joff[z + 1]; ++t, ++sl)
+          jsched[64 * (size_t)sl + l] =
+              (unsigned)(2 * jt_rxn[t] + jt_slot[t]) |
+              ((unsigned)(uint8_t)(int8_t)jt_coef[t] << 8) | ((unsigned)z << 16) |
+              ((t == joff[z + 1] - 1 ? 1u : 0u) << 25) | (1u << 26);
+    }
+  } else {
+    jsched.assign(64, 0u);
+  }
   // pieces of the balanced RHS gather: cuts at every species row start and
   // at 128 equal shares of the entries
   const int n_e = (int)rhs_pk.size();
@@ -2728,7 +2800,8 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
          o_pb = put(pc_beg.data(), 4 * pc_beg.size()), o_pl = put(pc_lane.data(), 4 * pc_lane.size()),
          o_pr = put(pc_row.data(), 4 * pc_row.size()), o_s5 = put(st5.data(), 4 * st5.size()),
          o_js = put(jr_start.data(), 4 * jr_start.size()),
-         o_jm = put(jr_mask.data(), 4 * jr_mask.size());
+         o_jm = put(jr_mask.data(), 4 * jr_mask.size()),
+         o_jq = put(jsched.data(), 4 * jsched.size());
   auto net = std::make_unique<Network>();
   if (hipMalloc(&net->blob, host.size()) != hipSuccess) {
     set_error("hipMalloc of %zu bytes for network tables failed", host.size());
@@ -2758,6 +2831,8 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   d.jt_rxn = (const int *)(b + o_tr);
   d.jt_slot = (const int *)(b + o_ts);
   d.jt_coef = (const double *)(b + o_tc);
+  d.jslots = jslots;
+  d.jsched = (const unsigned *)(b + o_jq);
   d.post_w = (const double *)(b + o_pw);
   d.n_pc = n_pc;
   d.pc_beg = (const int *)(b + o_pb);
@@ -2767,7 +2842,7 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   // reactions and Jacobian structure
   net->lane_ok = roles7 != nullptr &&
                  lane_network_matches(ns, nr, rtype, reactants, products, e_index, roles) &&
-                 ns == wave::kNS && nr == wave::kNR && nnz == wave::kNNZ;
+                 ns == wave::kNS && nr == wave::kNR && nnz == wave::kNNZ && sched_ok;
   std::lock_guard<std::mutex> lk(g_net_mu);
   int64_t h = g_next_handle++;
   g_nets[h] = std::move(net);
