@@ -520,9 +520,19 @@ __device__ __forceinline__ double rhs_pieces(const NetDev &N, const CellParams &
   }
   L.psum[lane] = acc;
   __syncthreads();
+  // the species' pieces in order, as a fixed-length predicated sum so the
+  // reads are issued together (slots past the row keep f by a select)
   double f = 0.0;
-  if (lane < N.ns)
-    for (int q = c_wave_rowpc[lane]; q < c_wave_rowpc[lane + 1]; ++q) f += L.psum[q];
+  {
+    const int q0 = lane < N.ns ? c_wave_rowpc[lane] : 0;
+    const int q1 = lane < N.ns ? c_wave_rowpc[lane + 1] : 0;
+#pragma unroll
+    for (int j = 0; j < wave::kRowPieceMax; ++j) {
+      const int q = q0 + j;
+      const double v = L.psum[q < 64 ? q : 63];
+      f = q < q1 ? f + v : f;
+    }
+  }
   __syncthreads();  // part[] and psum[] are reused by the next evaluation
   return f;
 }
@@ -1189,15 +1199,27 @@ __device__ __forceinline__ void jac_eval_for(const NetDev &N, const CellParams &
 // D <- (R(factor) U)^T D with R_im = prod_{q=1..i} (q-1 - factor m)/q and
 // U = R(1).  Lanes 0..35 build RU (entry (i, j) per lane) in LDS, then every
 // species lane applies it to its own differences.
+// U = R(1) of change_D: U_mj = prod_{q=1..m} (q-1 - j)/q, formed at compile
+// time with the same operations in the same order as the loop it replaces
+constexpr double bdf_u(int m, int j) {
+  double u = 1.0;
+  for (int q = 1; q <= m; ++q) u *= ((double)(q - 1) - (double)j) / (double)q;
+  return u;
+}
+#define P3D_U6(m) bdf_u(m, 0), bdf_u(m, 1), bdf_u(m, 2), bdf_u(m, 3), bdf_u(m, 4), bdf_u(m, 5)
+__constant__ double c_bdf_u[36] = {P3D_U6(0), P3D_U6(1), P3D_U6(2), P3D_U6(3), P3D_U6(4),
+                                   P3D_U6(5)};
+#undef P3D_U6
+
 __device__ void change_D(const Lds &L, int lane, double (&D)[kNumD], int order, double factor) {
   if (lane < 36) {
     const int i = lane / 6, j = lane % 6;
     double acc = 0.0;
     if (i <= order && j <= order) {
       for (int m = 0; m <= order; ++m) {
-        double r = 1.0, u = 1.0;
+        double r = 1.0;
         for (int q = 1; q <= i; ++q) r *= ((double)(q - 1) - factor * (double)m) / (double)q;
-        for (int q = 1; q <= m; ++q) u *= ((double)(q - 1) - (double)j) / (double)q;
+        const double u = c_bdf_u[6 * m + j];
         acc += r * u;
       }
     }

@@ -25,6 +25,7 @@ constexpr int kPieceLen = 6;
 #define P3D_WAVE_PIECE_W0 {0x0a080705u, 0x2e2c2010u, 0x4d4a4944u, 0x6f5c5952u, 0x8e8c8483u, 0xffffff94u, 0x03030000u, 0x19170c0au, 0x2423211fu, 0x312f2d2du, 0x3f3d3936u, 0x54454545u, 0x7a716d6bu, 0x87878181u, 0x8b8b8989u, 0xffff9593u, 0x0d040201u, 0x342a2827u, 0x4743403bu, 0x8b8a8a88u, 0x22150807u, 0xffffff62u, 0xffff0c0bu, 0xffff4005u, 0x42413a26u, 0x1614100fu, 0xff786f5eu, 0x1813110eu, 0x726e695fu, 0xff868581u, 0x1b191512u, 0x73685a55u, 0x7f7c7a1au, 0x47201e1du, 0xff807e1fu, 0x25221a13u, 0x695b3734u, 0x77757573u, 0x93918582u, 0xffff2423u, 0x302d1814u, 0x6c6b3836u, 0xff902827u, 0x39322f16u, 0x94922a29u, 0xff2e2c2bu, 0x76747271u, 0x7956543fu, 0x3e3c3b17u, 0xff614b48u, 0xff604a49u, 0x8f654f4cu, 0x8e644e4du, 0x8d675350u, 0x8c665251u, 0xff5d5b58u, 0xff5c5a59u, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu}
 #define P3D_WAVE_PIECE_W1 {0xffff0f0du, 0xffff423cu, 0xffff514eu, 0xffff807eu, 0xffff9290u, 0xffffffffu, 0xffff0906u, 0xffff1d1bu, 0xffff2b29u, 0xffff3533u, 0xffff4341u, 0xffff6357u, 0xffff7f7du, 0xffff8988u, 0xffff9391u, 0xffffffffu, 0xffff1e1cu, 0xffff3837u, 0xffff7b56u, 0xffffffffu, 0xffff5525u, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffff4644u, 0xffff5b1cu, 0xffffffffu, 0xffff4621u, 0xffff7d7bu, 0xffffffffu, 0xffff3d3au, 0xffffff87u, 0xffffff82u, 0xffffffffu, 0xffffffffu, 0xffff3126u, 0xffff706du, 0xffff7977u, 0xffffffffu, 0xffffffffu, 0xffff3533u, 0xffff9570u, 0xffffffffu, 0xffff866au, 0xffffffffu, 0xffffffffu, 0xffffff78u, 0xffffff84u, 0xffff8357u, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu}
 #define P3D_WAVE_ROW_PIECE {0, 6, 16, 20, 22, 23, 24, 25, 27, 30, 32, 33, 34, 35, 39, 40, 42, 43, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57}
+constexpr int kRowPieceMax = 10;  // pieces of the longest row
 
 // a: the lane's row of M (positions); lm: the lane's kLMask entry; pos, lev:
 // its elimination position and P3D_WAVE_LEVELS entry.  Afterwards a[k] holds

@@ -495,6 +495,8 @@ def generate_wave(net=None):
     w("#define P3D_WAVE_PIECE_W0 {" + ", ".join(f"0x{a:08x}u" for a, _ in words) + "}")
     w("#define P3D_WAVE_PIECE_W1 {" + ", ".join(f"0x{b_:08x}u" for _, b_ in words) + "}")
     w("#define P3D_WAVE_ROW_PIECE {" + ", ".join(str(x) for x in row_pc) + "}")
+    w("constexpr int kRowPieceMax = %d;  // pieces of the longest row" %
+      max(b_ - a_ for a_, b_ in zip(row_pc, row_pc[1:])))
     w("")
     w("// a: the lane's row of M (positions); lm: the lane's kLMask entry; pos, lev:")
     w("// its elimination position and P3D_WAVE_LEVELS entry.  Afterwards a[k] holds")
