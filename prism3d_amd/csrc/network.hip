@@ -583,9 +583,48 @@ __device__ void jacobian_sparse(const NetDev &N, const CellParams &p, const Lds 
 // the slots' table reads are independent of the data.  Each entry is summed
 // by one lane in its term order from 0, bit for bit as jacobian_sparse; a
 // slot that ends no entry writes the spare slot J[kNNZ].
+// eval_reactions' partials (want_jac) from the lane's register reaction
+// cache (reaction lane + 64 q), the same expressions in the same order
+__device__ __forceinline__ void partials_cached(const NetDev &N, const CellParams &p,
+                                                const Lds &L, const RxCache &rc, int lane) {
+  double pm, dpm;
+  pah_minus(L.ys[N.e_idx], p, pm, dpm);
+#pragma unroll
+  for (int q = 0; q < kRxPerLane; ++q) {
+    const int t = rc.t[q];
+    if (t < 0) continue;
+    const int a = rc.a[q], b = rc.b[q];
+    const double k = rc.k[q];
+    const double ya = L.ys[a];
+    const double act_a = ya > 1e-30 ? 1.0 : 0.0;
+    double d0 = 0.0, d1 = 0.0;
+    if (t == P3D_RX_TWO_BODY) {
+      if (b == P3D_SP_PAH_MINUS) {
+        d0 = k * pm * p.nH * act_a;
+        d1 = k * ya * p.nH * dpm * (L.ys[N.e_idx] > 1e-30 ? 1.0 : 0.0);
+      } else if (a == b) {
+        d0 = 2.0 * k * ya * p.nH * act_a;
+      } else {
+        const double yb = L.ys[b];
+        d0 = k * yb * p.nH * act_a;
+        d1 = k * ya * p.nH * (yb > 1e-30 ? 1.0 : 0.0);
+      }
+    } else if (t == P3D_RX_GRAIN) {
+      d0 = k * p.nH * 0.5 * act_a;
+    } else {
+      d0 = k * act_a;
+    }
+    L.part[2 * (lane + 64 * q)] = d0;
+    L.part[2 * (lane + 64 * q) + 1] = d1;
+  }
+  __syncthreads();
+}
+
 template <int W>
-__device__ void jacobian_sched(const NetDev &N, const CellParams &p, const Lds &L, int lane) {
-  eval_reactions<W>(N, p, L, lane, true);
+__device__ void jacobian_sched(const NetDev &N, const CellParams &p, const Lds &L, int lane,
+                               const RxCache *rc = nullptr) {
+  if (rc) partials_cached(N, p, L, *rc, lane);
+  else eval_reactions<W>(N, p, L, lane, true);
   const unsigned *sq = N.jsched + lane;
   double acc = 0.0;
 #pragma unroll 4
@@ -1190,8 +1229,8 @@ __device__ __forceinline__ double rms(double v, int ns) { return sqrt(wave_sum(v
 // register LU (LinStatic, carve_wave), otherwise jac_eval's.
 template <class Lin, class G>
 __device__ __forceinline__ void jac_eval_for(const NetDev &N, const CellParams &p, const Lds &L,
-                                             int lane) {
-  if constexpr (std::is_same<Lin, LinStatic>::value) jacobian_sched<64>(N, p, L, lane);
+                                             const RxCache &rc, int lane) {
+  if constexpr (std::is_same<Lin, LinStatic>::value) jacobian_sched<64>(N, p, L, lane, &rc);
   else jac_eval<G>(N, p, L, lane);
 }
 
@@ -1212,16 +1251,26 @@ __constant__ double c_bdf_u[36] = {P3D_U6(0), P3D_U6(1), P3D_U6(2), P3D_U6(3), P
 #undef P3D_U6
 
 __device__ void change_D(const Lds &L, int lane, double (&D)[kNumD], int order, double factor) {
+  // R_im once per lane (i = l / 6, m = l % 6 of the wavefront's lane l: at
+  // most 5 divisions), then lane (i, j) gathers R_i0..R_i5 from lanes 6i..6i+5
+  // and forms (RU)_ij = sum_m R_im U_mj in the same order as before
+  const int wl = lane & 63;
+  double r = 1.0;
+  {
+    const int i = wl / 6, m = wl % 6;
+    for (int q = 1; q <= i && q <= kMaxOrder; ++q)
+      r *= ((double)(q - 1) - factor * (double)m) / (double)q;
+  }
+  double rim[kMaxOrder + 1];
+#pragma unroll
+  for (int m = 0; m <= kMaxOrder; ++m) rim[m] = __shfl(r, 6 * (wl / 6) + m, 64);
   if (lane < 36) {
     const int i = lane / 6, j = lane % 6;
     double acc = 0.0;
     if (i <= order && j <= order) {
-      for (int m = 0; m <= order; ++m) {
-        double r = 1.0;
-        for (int q = 1; q <= i; ++q) r *= ((double)(q - 1) - factor * (double)m) / (double)q;
-        const double u = c_bdf_u[6 * m + j];
-        acc += r * u;
-      }
+#pragma unroll
+      for (int m = 0; m <= kMaxOrder; ++m)
+        if (m <= order) acc += rim[m] * c_bdf_u[6 * m + j];
     }
     L.ru[lane] = acc;
   }
@@ -1316,7 +1365,7 @@ __device__ __forceinline__ int bdf_integrate_rec(const NetDev &N, const CellPara
   int order = 1, n_equal = 0;
   double t = t0;
   publish_y(N, L, y, lane);
-  jac_eval_for<Lin, G>(N, p, L, lane);
+  jac_eval_for<Lin, G>(N, p, L, rc, lane);
   lin.load_jac(N, L, lane);
   st.jac++;
   bool jac_current = true, lu_valid = false;
@@ -1384,7 +1433,7 @@ __device__ __forceinline__ int bdf_integrate_rec(const NetDev &N, const CellPara
         if (converged) break;
         if (!jac_current) {
           publish_y(N, L, ypred, lane);
-          jac_eval_for<Lin, G>(N, p, L, lane);
+          jac_eval_for<Lin, G>(N, p, L, rc, lane);
           lin.load_jac(N, L, lane);
           st.jac++;
           jac_current = true;
