@@ -21,6 +21,8 @@ predicted time fits, and the line reports the steps and warmups run.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --kernel all [--kn 256]   (one line per whole-grid kernel)
+       python bench.py --forms 128               (batched vs plain drop-in)
+       python bench.py --scale-grid 16           (103-species network in the grid loop)
 """
 
 import argparse
@@ -659,6 +661,61 @@ def kernel_lines(args):
     sur.close()
 
 
+def scale_line(args):
+    """One JSON line for the scale configuration at a size one GPU finishes
+    in minutes: an args.scale_grid^3 log-normal grid (median n_H 1e4,
+    sigma_ln 1.2, 0.5 pc, G0 1e4 from -x over 48 rays, zeta 2e-16) with the
+    103-species synthetic network (prism3d_amd.network_synth) in the coupled
+    loop: RT, shielding, the two-wavefront MFMA chemistry from the atomic /
+    ionic initial state to 1e7 yr (rtol 1e-9, atol 1e-20, at most 1e5 BDF
+    steps per cell and span, no post-solve corrections: the upstream H+/e-
+    closure is written for the bundled network's ions) and the thermal
+    balance, for args.scale_iter fixed-point iterations.  value = cells /
+    mean seconds per iteration."""
+    import torch
+    from prism3d_amd.density import PC, lognormal_median
+    from prism3d_amd.grid import directional_G0
+    from prism3d_amd.network import Network
+    from prism3d_amd.network_synth import synthetic_network_wide
+    from prism3d_amd.solver3d import PDRSolverGPU
+    n = args.scale_grid
+    sp, rx = synthetic_network_wide()
+    net = Network(sp, rx)
+    init = {'H': 1.0, 'C+': 1.4e-4, 'O': 3e-4, 'N': 7.5e-5, 'He': 0.1, 'S+': 1e-6,
+            'Si+': 1e-7, 'Fe+': 1e-8, 'e-': 1.4e-4 + 1e-6 + 1e-7 + 1e-8}
+    dens, _ = lognormal_median(n, 0.5 * PC, 1e4, 1.2, seed=42)
+    sv = PDRSolverGPU(dens, 0.5 * PC, G0_external=directional_G0(1e4, [-1.0, 0.0, 0.0], nside=2),
+                      zeta_CR_0=2e-16, nside_rt=2, chemistry='network', network=net,
+                      initial_abundances=init, T_init=50.0, chem_atol=1e-20,
+                      chem_post_mode=-1, chem_max_steps=100000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sv.run(max_iterations=args.scale_iter, convergence_tol=0.0)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.scale_iter
+    st = sv.chem_stats.cpu().numpy()
+    y = sv.y.cpu().numpy()
+    out = {"metric": "cells/s through one coupled fixed-point iteration, 103-species network",
+           "value": sv.n_cells / sec, "unit": "cells/s", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{n}^3 log-normal grid (median n_H 1e4, sigma_ln 1.2, "
+                                  "0.5 pc), G0 1e4 from -x over 48 rays, zeta 2e-16, "
+                                  f"{len(sp)} species / {len(rx)} reactions, in-loop chemistry "
+                                  "to 1e7 yr at rtol 1e-9 (the scale configuration's grid "
+                                  "parameters at a smaller size)",
+                      "cells": sv.n_cells, "iterations": args.scale_iter,
+                      "kernel": "chem_solve_kernel<LinMfmaG<Wave2, true>, Wave2> "
+                                "(two wavefronts per cell)"},
+           "s_per_iteration": sec, "bdf_steps_per_cell": float(st[:, 0].mean()),
+           "bdf_steps_p99": float(np.percentile(st[:, 0], 99)),
+           "bdf_steps_max": int(st[:, 0].max()),
+           "cells_at_step_cap": int((st[:, 0] > 100000).sum()), "step_cap": 100000,
+           "state_finite": bool(np.isfinite(y).all()),
+           "x_H2_range": [float(sv.x_H2.min()), float(sv.x_H2.max())],
+           "T_gas_range": [float(sv.T_gas.min()), float(sv.T_gas.max())]}
+    print(json.dumps(out), flush=True)
+
+
 def clouds_line(args):
     """One JSON line for the ensemble workload: args.clouds independent
     log-normal clouds of args.cn^3 cells (G0 log-uniform 1e1-1e5, median n_H
@@ -841,6 +898,11 @@ def main():
                          "many cells")
     ap.add_argument("--synth-wide", action="store_true",
                     help="with --synth-cells: the 103-species network")
+    ap.add_argument("--scale-grid", type=int, default=0,
+                    help="scale-configuration line instead of the headline: the 103-species "
+                         "network in the coupled loop on a grid of this many cells per side")
+    ap.add_argument("--scale-iter", type=int, default=1,
+                    help="fixed-point iterations of the --scale-grid run")
     ap.add_argument("--clouds", type=int, default=0,
                     help="ensemble line instead of the headline: this many independent clouds")
     ap.add_argument("--cn", type=int, default=32, help="cells per side of each --clouds cloud")
@@ -863,6 +925,9 @@ def main():
         del a, b
     if args.clouds > 0:
         clouds_line(args)
+        return
+    if args.scale_grid > 0:
+        scale_line(args)
         return
     if args.synth_cells > 0:
         synth_line(args)

@@ -52,7 +52,8 @@ class PDRSolverGPU:
                  av_per_nh=grid.AV_PER_NH, t_chem=1e7 * YR, chem_rtol=1e-9, chem_atol=1e-20,
                  initial_abundances=None, T_init=50.0, chem_mode='integrate', chem_variant=0,
                  chem_waves_per_cu=0, chem_rescue_frac=0.0, chem_autotune=False, star=None,
-                 velocity_field=None, abundance_damping='reference', chem_post_mode=0):
+                 velocity_field=None, abundance_damping='reference', chem_post_mode=0,
+                 network=None, chem_max_steps=1000000):
         self.dev = device()
         dens = np.asarray(density, dtype=np.float64)
         self.shape = dens.shape
@@ -88,6 +89,9 @@ class PDRSolverGPU:
         if chem_mode not in ('steady', 'integrate'):
             raise ValueError("chem_mode must be 'steady' or 'integrate'")
         self.chem_mode = chem_mode
+        # BDF steps allowed per cell and span (a span that reaches it fails and
+        # leaves the cell's state unchanged, as a failed upstream span does)
+        self.chem_max_steps = int(chem_max_steps)
         # abundance damping after the chemistry of every iteration but the
         # first, for the explicit and the network chemistry (the surrogate
         # damps oscillating cells only, solver_3d.py:330-345):
@@ -134,8 +138,17 @@ class PDRSolverGPU:
         self.Gamma, self.Lambda = t(0.0), t(0.0)
         self.fsh_H2, self.fsh_CO = t(1.0), t(1.0)
         # abundance state (n_species, n_cells): initial values as upstream
-        # _init_state (solver_3d.py:124-136) unless given
-        self.species = list(TRACKED)
+        # _init_state (solver_3d.py:124-136) unless given.  network: a
+        # network.Network for the in-loop chemistry other than the bundled
+        # 31-species one (e.g. network_synth's 103-species network); it must
+        # hold the ten grid species (x_HI ... x_HCOp) and the charge and
+        # conservation roles, which the thermal step and the damping read by
+        # name
+        self.network = network
+        self.species = list(TRACKED) if network is None else list(network.species)
+        missing = [s for s in _GRID_SPECIES.values() if s not in self.species]
+        if missing:
+            raise ValueError(f"the network lacks the grid species {missing}")
         self.sidx = {s: i for i, s in enumerate(self.species)}
         self.y = torch.full((len(self.species), self.n_cells), 1e-30, dtype=torch.float64,
                             device=self.dev)
@@ -163,7 +176,9 @@ class PDRSolverGPU:
         self._ws = torch.empty(18 * self.n_cells, dtype=torch.float64, device=self.dev)
         self._tmean = torch.empty(1, dtype=torch.float64, device=self.dev)
         self._mean_ws = torch.empty(grid.MEAN_WORKSPACE, dtype=torch.float64, device=self.dev)
-        self.chem = ChemistrySolver() if chemistry == 'network' else None
+        if network is not None and chemistry != 'network':
+            raise ValueError("a custom network needs chemistry='network'")
+        self.chem = ChemistrySolver(network) if chemistry == 'network' else None
         # an ML surrogate (surrogate.ChemistrySurrogate) set here replaces the
         # chemistry of every inner iteration, as upstream (solver_3d.py:142-143)
         self.accelerator = None
@@ -295,7 +310,7 @@ class PDRSolverGPU:
         return solve_opts(t_max=t_max, t_start=1e4, n_checkpoints=2, early_stop=False,
                           rtol=self.chem_rtol, atol=self.chem_atol,
                           post_mode=self.chem_post_mode, x_c_total=self.x_c_total,
-                          x_o_total=self.x_o_total,
+                          x_o_total=self.x_o_total, max_steps=self.chem_max_steps,
                           variant=cfg[0], waves_per_cu=cfg[1], rescue_frac=cfg[2])
 
     def thermal_step(self):
@@ -511,7 +526,7 @@ class PDRSolverGPU:
         the mean T; then the thermal balance is solved again at the refined
         abundances.  Returns the number of early-stopped (converged) cells."""
         if self.chem is None:
-            self.chem = ChemistrySolver()
+            self.chem = ChemistrySolver(self.network)
         grid.mean(self.T_gas, out=self._tmean, workspace=self._mean_ws)
         fsh_H2, fsh_CO, _ = grid.shield_cr(self.N_H2, self.N_CO, self.N_H, self._tmean,
                                            self.zeta_CR_0)

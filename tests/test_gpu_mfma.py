@@ -206,3 +206,48 @@ def test_wide_network_conserves_nuclei(gpu):
     print(f"nuclei drift by element {dict(zip(elems, rel.max(axis=1)))}; "
           f"steps {st.cpu().numpy()[:, 0].mean():.0f}", flush=True)
     assert rel.max() < 1e-7
+
+
+def test_wide_network_in_the_grid_loop(gpu):
+    """The 103-species network in the coupled grid loop (PDRSolverGPU with
+    chemistry='network' and the custom network): a 6^3 log-normal cube,
+    distant field from -x, two fixed-point iterations (RT, shielding, the
+    two-wavefront chemistry, thermal), with 'uniform' damping and no
+    post-solve corrections so that every element's nuclei are a
+    size-independent check: each cell's totals stay at the initial ones
+    through the solves and the damping, to 1e-7 of the element total.  The
+    grid fields read the network state by name, and the thermal step moves
+    T_gas off its initial value."""
+    import torch
+    from prism3d_amd.density import PC, lognormal_median
+    from prism3d_amd.grid import directional_G0
+    from prism3d_amd.network import Network
+    from prism3d_amd.network_synth import formula, synthetic_network_wide
+    from prism3d_amd.solver3d import PDRSolverGPU
+    print("case: 103-species network on a 6^3 grid, 2 iterations", flush=True)
+    sp, rx = synthetic_network_wide()
+    net = Network(sp, rx)
+    init = {'H': 1.0, 'C+': 1.4e-4, 'O': 3e-4, 'N': 7.5e-5, 'He': 0.1, 'S+': 1e-6,
+            'Si+': 1e-7, 'Fe+': 1e-8, 'e-': 1.4e-4 + 1e-6 + 1e-7 + 1e-8}
+    dens, _ = lognormal_median(6, 0.5 * PC, 3e2, 1.0, seed=3)
+    sv = PDRSolverGPU(dens, 0.5 * PC, G0_external=directional_G0(1e2, [-1.0, 0.0, 0.0], nside=1),
+                      zeta_CR_0=2e-16, nside_rt=1, chemistry='network', network=net,
+                      initial_abundances=init, T_init=50.0, chem_atol=1e-20,
+                      abundance_damping='uniform', chem_post_mode=-1, chem_max_steps=100000)
+    sv.run(max_iterations=2, convergence_tol=0.0)
+    torch.cuda.synchronize()
+    y = sv.y.cpu().numpy()
+    assert y.shape == (len(sp), 216)
+    assert np.isfinite(y).all() and (y >= 1e-30).all()
+    elems = sorted({e for s in sp for e in formula(s)[0]})
+    C = np.array([[formula(s)[0].get(e, 0) for s in sp] for e in elems], dtype=np.float64)
+    y0 = sv.y_init.cpu().numpy()
+    tot0, tot = C @ y0, C @ y
+    big = tot0 > 1e-20
+    rel = np.abs(tot - tot0)[big] / tot0[big]
+    print(f"nuclei drift {rel.max():.2e}; x_H2 range {sv.x_H2.min().item():.2e} - "
+          f"{sv.x_H2.max().item():.2e}; steps mean {sv.chem_stats.cpu().numpy()[:, 0].mean():.0f}",
+          flush=True)
+    assert rel.max() < 1e-7
+    assert torch.equal(sv.x_CO.reshape(-1), sv.y[sp.index('CO')])
+    assert (sv.T_gas != 50.0).any()
