@@ -53,15 +53,44 @@ def flops_per_cell(stats, n=N_SP, n_rx=N_RX, n_terms=300):
             + jac * (2.0 * n_terms + 4 * n_rx) + steps * 40.0 * n)
 
 
+def sparse_lu_counts(net):
+    """FP64 operations of one factorisation and one pair of substitutions of
+    the static sparse LU (variant 7) on `net`: its multipliers (one multiply
+    each) and updates (multiply-add = 2) over the filled structure of the
+    elimination order p3d_sparse_lu_schedule returns, and 2 per filled
+    off-diagonal entry plus n per substitution pair."""
+    import ctypes
+    from prism3d_amd import _lib
+    from prism3d_amd.gen_lane import fill, structure
+    ns = net.n_species
+    _, _, terms = structure(net)
+    ij = np.array(sorted(terms), dtype=np.int32)
+    jr, jc = ij[:, 0].copy(), ij[:, 1].copy()
+    pos, lev, lvp = (np.zeros(ns, np.int32) for _ in range(3))
+    lvoff, uoff = np.zeros(ns + 1, np.int32), np.zeros(ns + 1, np.int32)
+    uj, counts = np.zeros(ns * ns, np.int32), np.zeros(2, np.int32)
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    rc = _lib.lib().p3d_sparse_lu_schedule(ns, jr.size, P(jr), P(jc), ns * ns, P(pos), P(lev),
+                                            P(lvoff), P(lvp), P(uoff), P(uj), P(counts))
+    assert rc == 0
+    perm = np.argsort(pos)
+    A = np.eye(ns, dtype=bool)
+    A[jr, jc] = True
+    F = fill(A[np.ix_(perm, perm)], ns)
+    low, up = np.tril(F, -1), np.triu(F, 1)
+    lu = int(sum(low[:, k].sum() * (1 + 2 * up[k].sum()) for k in range(ns)))
+    return {"lu": lu, "solve": int(2 * (low.sum() + up.sum()) + ns), "levels": int(counts[0])}
+
+
 def synth_line(args):
     """One JSON line for the 64-species synthetic network
     (prism3d_amd.network_synth): args.synth_cells cells with n_H, T, G0,
     A_V, zeta and shielding drawn log-uniform over PDR conditions, the
     initial atomic/ionic state, integrated to 1e7 yr at rtol 1e-9 by the
-    one-cell-per-workgroup solve with the MFMA trailing-update LU
-    (variant 5), beside the same solve with the LDS LU (variant 1).  With
-    --synth-wide, the 103-species network, whose only form is the
-    two-wavefront MFMA solve."""
+    automatic form, the static sparse LU by levels (variant 7, two
+    wavefronts per cell), beside the MFMA trailing-update LU (variant 5) and
+    the LDS LU (variant 1).  With --synth-wide, the 103-species network
+    (variants 7 and 5, both two wavefronts per cell)."""
     import torch
     from prism3d_amd.chemistry import ChemistrySolver, solve_opts
     from prism3d_amd.network import DeviceNetwork, Network
@@ -83,8 +112,9 @@ def synth_line(args):
     y0[sp.index('e-')] = 1.4e-4 + 1e-6 + 1e-7 + 1e-8
     n_terms = int(sum(len([x for x in r[2] if x in sp]) + len([x for x in r[3] if x in sp])
                       for r in rx))
+    cnt = sparse_lu_counts(net)
     res = {}
-    for var in ((5,) if wide else (5, 1)):
+    for var in ((7, 5) if wide else (7, 5, 1)):
         y = torch.tensor(np.repeat(y0[:, None], n, axis=1), device="cuda")
         o = solve_opts(t_max=1e7 * 3.15576e7, n_checkpoints=2, early_stop=False, rtol=1e-9,
                        atol=1e-24, post_mode=-1, variant=var, max_steps=100000)
@@ -97,6 +127,11 @@ def synth_line(args):
         sec = e0.elapsed_time(e1) * 1e-3
         stn = st.cpu().numpy()
         fl = float(flops_per_cell(stn, len(sp), len(rx), n_terms).sum())
+        if var == 7:  # the sparse algorithm's own LU and substitution counts
+            steps, lu, rhs = (stn[:, k].astype(np.float64) for k in range(3))
+            nn = len(sp)
+            fl -= float((lu * (2.0 * nn ** 3 / 3.0 + 2.0 * nn * nn) + rhs * 2.0 * nn * nn).sum())
+            fl += float((lu * cnt["lu"] + rhs * cnt["solve"]).sum())
         res[var] = dict(s=sec, y=y.cpu().numpy(), flops=fl, lu=float(stn[:, 1].sum()),
                         steps_mean=float(stn[:, 0].mean()), steps_max=int(stn[:, 0].max()),
                         steps_p99=float(np.percentile(stn[:, 0], 99)),
@@ -104,31 +139,34 @@ def synth_line(args):
                         lu_mean=float(stn[:, 1].mean()))
         print(f"[bench] synthetic network, variant {var}: {sec:.2f} s", file=sys.stderr,
               flush=True)
-    r5 = res[5]
-    achieved = r5["flops"] / r5["s"] / 1e12
-    lu_tf = r5["lu"] * 2.0 * len(sp) ** 3 / 3.0 / r5["s"] / 1e12
-    kern = ("chem_solve_kernel<LinMfmaG<Wave2, true>, Wave2> (two wavefronts per cell)" if wide
-            else "chem_solve_kernel<LinMfma> (variant 5)")
+    r7 = res[7]
+    achieved = r7["flops"] / r7["s"] / 1e12
+    kern = "chem_solve_kernel<LinSparseG<Wave2>, Wave2> (variant 7, two wavefronts per cell)"
     out = {"metric": f"cells/s to 1e7 yr, {len(sp)}-species synthetic network",
-           "value": n / r5["s"],
+           "value": n / r7["s"],
            "unit": "cells/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
            "data": "synthetic",
            "config": {"workload": f"{n} cells, {len(sp)} species / {len(rx)} reactions "
                                   "(prism3d_amd.network_synth), BDF to 1e7 yr at rtol 1e-9",
                       "kernel": kern},
-           "s_mfma_lu": r5["s"], "bdf_steps_per_cell": r5["steps_mean"],
-           "bdf_steps_p99": r5["steps_p99"], "bdf_steps_max": r5["steps_max"],
-           "cells_at_step_cap": r5["capped"], "step_cap": 100000,
-           "lu_per_cell": r5["lu_mean"]}
-    if not wide:
-        a, b = res[5]["y"], res[1]["y"]
+           "s_sparse_lu": r7["s"], "s_mfma_lu": res[5]["s"],
+           "bdf_steps_per_cell": r7["steps_mean"],
+           "bdf_steps_p99": r7["steps_p99"], "bdf_steps_max": r7["steps_max"],
+           "cells_at_step_cap": r7["capped"], "step_cap": 100000,
+           "lu_per_cell": r7["lu_mean"], "sparse_lu": cnt}
+    others = (5,) if wide else (5, 1)
+    for var in others:
+        a, b = res[7]["y"], res[var]["y"]
         m = b > 1e-14
         rel = np.abs(a[m] - b[m]) / b[m]
+        out[f"p999_rel_diff_variant7_vs_{var}"] = float(np.percentile(rel, 99.9)) if rel.size else 0.0
+    if not wide:
         out["s_lds_lu"] = res[1]["s"]
-        out["p999_rel_diff_mfma_vs_lds"] = float(np.percentile(rel, 99.9)) if rel.size else 0.0
     out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                        "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                       "lu_only_tflops": lu_tf}
+                       "flop_convention": "variant 7's sparse LU and substitution counts "
+                                          "(sparse_lu), the RHS / Jacobian / step terms of "
+                                          "flops_per_cell"}
     print(json.dumps(out), flush=True)
 
 
@@ -666,7 +704,8 @@ def scale_line(args):
     in minutes: an args.scale_grid^3 log-normal grid (median n_H 1e4,
     sigma_ln 1.2, 0.5 pc, G0 1e4 from -x over 48 rays, zeta 2e-16) with the
     103-species synthetic network (prism3d_amd.network_synth) in the coupled
-    loop: RT, shielding, the two-wavefront MFMA chemistry from the atomic /
+    loop: RT, shielding, the two-wavefront chemistry (the static sparse LU by
+    levels, the automatic form from 64 species) from the atomic /
     ionic initial state to 1e7 yr (rtol 1e-9, atol 1e-20, at most 1e5 BDF
     steps per cell and span, no post-solve corrections: the upstream H+/e-
     closure is written for the bundled network's ions) and the thermal
@@ -704,7 +743,7 @@ def scale_line(args):
                                   "to 1e7 yr at rtol 1e-9 (the scale configuration's grid "
                                   "parameters at a smaller size)",
                       "cells": sv.n_cells, "iterations": args.scale_iter,
-                      "kernel": "chem_solve_kernel<LinMfmaG<Wave2, true>, Wave2> "
+                      "kernel": "chem_solve_kernel<LinSparseG<Wave2>, Wave2> "
                                 "(two wavefronts per cell)"},
            "s_per_iteration": sec, "bdf_steps_per_cell": float(st[:, 0].mean()),
            "bdf_steps_p99": float(np.percentile(st[:, 0], 99)),

@@ -119,7 +119,7 @@ int p3d_pe_heating_vec(int64_t n_cells, const double *pe_table, int64_t n_bins,
 /* Upload a network.  reactants: 2 per reaction, products: 3 per reaction
  * (P3D_SP_NONE for unused slots); e_index is the electron species (its
  * abundance sets the PAH charge state).  n_species <= 128; above 64 species
- * only p3d_chem_solve (variant 0 or 5) takes the network, and the entry
+ * only p3d_chem_solve (variant 0, 5 or 7) takes the network, and the entry
  * points that map one species to one lane of a wavefront refuse it.
  * roles7 = species indices of H, H2, H+, C+, C, CO, O used by the post-solve
  * corrections (solver.py:227-388), -1 when the network has none (then only
@@ -134,6 +134,24 @@ int p3d_network_create(int32_t n_species, int32_t n_reactions,
                        int32_t e_index, const int32_t *roles7,
                        const double *post_w4, int64_t *handle);
 int p3d_network_destroy(int64_t handle);
+
+/* The static elimination schedule that variant 7 (the sparse no-pivot LU of
+ * M = I - c J, p3d_chem_solve) uses for a Jacobian structure of nnz entries
+ * (jrow[z], jcol[z]); the network handle builds the same one.  Host only,
+ * touches no device.  No counterpart in the reference: upstream factorises
+ * with the partially pivoted dense LU inside scipy's BDF
+ * (chemistry/solver.py:195-201); this exposes the order so callers and tests
+ * can check it.  Outputs (host arrays): pos[s] = elimination position of
+ * species s (greedy minimum Markowitz product, lowest species on ties);
+ * lev[p] = dependency level of position p; the pivots of level v are
+ * lvp[lvoff[v] .. lvoff[v+1]-1] (lvoff holds n_species + 1 entries at most);
+ * row position p of the filled structure has its upper entries at
+ * uj[uoff[p] .. uoff[p+1]-1].  cap = capacity of uj.  counts = {levels,
+ * upper entries}. */
+int p3d_sparse_lu_schedule(int32_t n_species, int32_t nnz, const int32_t *jrow,
+                           const int32_t *jcol, int32_t cap, int32_t *pos, int32_t *lev,
+                           int32_t *lvoff, int32_t *lvp, int32_t *uoff, int32_t *uj,
+                           int32_t *counts);
 
 /* dy/dt for n cells; y and dydt are (n_species, n) device arrays. */
 int p3d_chem_rhs(int64_t handle, int64_t n, const double *y, const double *n_h,
@@ -168,7 +186,7 @@ typedef struct p3d_solve_opts {
   int32_t max_steps;   /* per span                                          */
   double x_c_total, x_o_total; /* element totals for the conservation step */
   int32_t variant;     /* kernel variant: 0 auto (6 when the network is the
-                          bundled one, else 5 for n_species >= 64, else 1),
+                          bundled one, else 7 for n_species >= 64, else 1),
                           1 LU in LDS, 2 LU in registers
                           (n_species <= 32), 3 LU in LDS with unrolled loops
                           (n_species <= 32), 5 LU in LDS with the panel
@@ -180,7 +198,11 @@ typedef struct p3d_solve_opts {
                           wavefront kernel with a device work queue, lane =
                           species, the network's static no-pivot sparse LU
                           in registers (the bundled network structure
-                          only) */
+                          only); 7 one cell per two wavefronts, the
+                          static no-pivot sparse LU of the filled
+                          structure in LDS, factorised and substituted by
+                          dependency levels (any network;
+                          p3d_sparse_lu_schedule) */
   int32_t waves_per_cu; /* variant 4: resident wavefronts per compute unit
                           (0: library default 8); the kernel is compiled for
                           waves_per_cu / 4 wavefronts per SIMD (2..4);

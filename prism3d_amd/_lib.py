@@ -52,6 +52,7 @@ SIGNATURES = {
     "p3d_network_create": (_i32, [_i32, _i32, _I32P, _D, _D, _D, _I32P, _I32P, _i32, _I32P, _D,
                                   _I64P]),
     "p3d_network_destroy": (_i32, [_i64]),
+    "p3d_sparse_lu_schedule": (_i32, [_i32, _i32, _I32P, _I32P, _i32] + [_I32P] * 7),
     "p3d_chem_rhs": (_i32, [_i64, _i64] + [_VP] * 9 + [_VP]),
     "p3d_chem_jacobian": (_i32, [_i64, _i64] + [_VP] * 9 + [_VP]),
     "p3d_chem_post": (_i32, [_i64, _i64, _VP, _VP, _VP, _VP, _i32, _dbl, _dbl, _VP]),

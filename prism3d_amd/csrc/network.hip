@@ -77,6 +77,14 @@ struct NetDev {
   const int *pc_beg;   // n_pc + 1 entry offsets
   const int *pc_lane;  // 129: pieces of lane l are pc_lane[l] .. pc_lane[l+1]-1
   const int *pc_row;   // ns + 1: pieces of species s
+  // static elimination schedule of the sparse no-pivot LU (LinSparseG):
+  // species s sits at position sl_pos[s] of the minimum-Markowitz order;
+  // sl_lev[p] = dependency level of position p; the pivots of level v are
+  // sl_lvp[sl_lvoff[v] .. sl_lvoff[v+1]-1] (position order); row position p
+  // of the filled structure has upper entries at sl_uj[sl_uoff[p] ..
+  // sl_uoff[p+1]-1]
+  int sl_nlev;
+  const int *sl_pos, *sl_lev, *sl_lvoff, *sl_lvp, *sl_uoff, *sl_uj;
 };
 
 struct Network {
@@ -985,6 +993,132 @@ struct LinMfmaG {
   }
 };
 using LinMfma = LinMfmaG<Wave1, false>;
+
+// LinSparseG: M = I - c J in LDS in the network's static elimination order
+// (minimum Markowitz product, no pivoting).  Lane = species s, whose row
+// sits at position p = sl_pos[s].  The factorisation is right-looking over
+// dependency levels: in pass v every wavefront walks the pivots of level v
+// in position order (wave-uniform, so their upper-row positions come by
+// scalar loads), and each lane whose row has a multiplier for the pivot
+// applies it (a row without that entry holds an exact zero there, kept
+// from the zeroed row, and skips it).  A pivot's level is above those of
+// every pivot sharing an entry of its row or column, so a pass reads only
+// rows that earlier passes finished; a row inverts its diagonal at the end
+// of the pass before its own level.  One barrier per level.  Factors: unit
+// L below the diagonal, U above it, 1 / u_pp on it, zeros outside the
+// filled structure, so the substitutions are the dense serial ones in
+// position order, run by wavefront 0 as in lu_solve_wide.
+template <class G>
+struct LinSparseG {
+  __device__ void load_jac(const NetDev &, const Lds &, int) {}
+
+  __device__ void factor(const NetDev &N, const Lds &L, double c, int lane) {
+    const int ld = L.ld, ns = N.ns;
+    const bool act = lane < ns;
+    const int p = act ? N.sl_pos[lane] : ns;  // lanes past the species: no row
+    const int lev = act ? N.sl_lev[p] : -2;
+    double *Mp = L.M + (act ? p : 0) * ld;
+    if (act) {
+      for (int j = 0; j < ns; ++j) Mp[j] = 0.0;
+      Mp[p] = 1.0;
+      for (int z = N.jr_start[lane]; z < N.jr_start[lane + 1]; ++z) {
+        const int j = N.jcol[z];
+        Mp[N.sl_pos[j]] = (j == lane ? 1.0 : 0.0) - c * L.J[z];
+      }
+      if (lev == 0) Mp[p] = 1.0 / Mp[p];
+    }
+    __syncthreads();
+    for (int v = 0; v < N.sl_nlev; ++v) {
+      const int q0 = N.sl_lvoff[v], q1 = N.sl_lvoff[v + 1];
+      for (int q = q0; q < q1; ++q) {
+        const int k = N.sl_lvp[q];
+        const double *Mk = L.M + k * ld;
+        double l = 0.0;
+        if (p > k && p < ns) l = Mp[k];
+        if (l != 0.0) {
+          l *= Mk[k];
+          Mp[k] = l;
+          const int u1 = N.sl_uoff[k + 1];
+          int u = N.sl_uoff[k];
+          for (; u + 4 <= u1; u += 4) {
+            const int j0 = N.sl_uj[u], j1 = N.sl_uj[u + 1], j2 = N.sl_uj[u + 2],
+                      j3 = N.sl_uj[u + 3];
+            const double b0 = Mk[j0], b1 = Mk[j1], b2 = Mk[j2], b3 = Mk[j3];
+            const double a0 = Mp[j0], a1 = Mp[j1], a2 = Mp[j2], a3 = Mp[j3];
+            Mp[j0] = a0 - l * b0;
+            Mp[j1] = a1 - l * b1;
+            Mp[j2] = a2 - l * b2;
+            Mp[j3] = a3 - l * b3;
+          }
+          for (; u < u1; ++u) {
+            const int j = N.sl_uj[u];
+            Mp[j] -= l * Mk[j];
+          }
+        }
+      }
+      if (lev == v + 1) Mp[p] = 1.0 / Mp[p];
+      __syncthreads();
+    }
+  }
+
+  __device__ double solve(const NetDev &N, const Lds &L, double b, int lane) {
+    const int n = N.ns, ld = L.ld;
+    double *x = L.xs;
+    if (lane < n) x[N.sl_pos[lane]] = b;
+    __syncthreads();
+    if (lane < 64) {
+      const int r0 = lane, r1 = lane + 64;
+      const double *M0 = L.M + min(r0, n - 1) * ld, *M1 = L.M + min(r1, n - 1) * ld;
+      double b0 = r0 < n ? x[r0] : 0.0, b1 = r1 < n ? x[r1] : 0.0;
+      for (int k0 = 0; k0 < n - 1; k0 += kSolveChunk) {
+        double a0[kSolveChunk], a1[kSolveChunk];
+#pragma unroll
+        for (int q = 0; q < kSolveChunk; ++q) {
+          const int k = min(k0 + q, n - 1);
+          a0[q] = M0[k];
+          a1[q] = M1[k];
+        }
+#pragma unroll
+        for (int q = 0; q < kSolveChunk; ++q) {
+          const int k = k0 + q;
+          if (k < n - 1) {
+            const double bk = bcast(k < 64 ? b0 : b1, k & 63);
+            if (r0 > k && r0 < n) b0 -= a0[q] * bk;
+            if (r1 > k && r1 < n) b1 -= a1[q] * bk;
+          }
+        }
+      }
+      for (int k0 = n - 1; k0 >= 0; k0 -= kSolveChunk) {
+        double a0[kSolveChunk], a1[kSolveChunk];
+#pragma unroll
+        for (int q = 0; q < kSolveChunk; ++q) {
+          const int k = max(k0 - q, 0);
+          a0[q] = M0[k];
+          a1[q] = M1[k];
+        }
+#pragma unroll
+        for (int q = 0; q < kSolveChunk; ++q) {
+          const int k = k0 - q;
+          if (k >= 0) {
+            if (lane == (k & 63)) {  // the lane's own reciprocal diagonal
+              if (k < 64) b0 *= a0[q];
+              else b1 *= a1[q];
+            }
+            const double bk = bcast(k < 64 ? b0 : b1, k & 63);
+            if (r0 < k) b0 -= a0[q] * bk;
+            if (r1 < k) b1 -= a1[q] * bk;
+          }
+        }
+      }
+      x[r0] = b0;
+      if (r1 < kMaxSpeciesWide) x[r1] = b1;
+    }
+    __syncthreads();
+    b = lane < n ? x[N.sl_pos[lane]] : 0.0;
+    __syncthreads();  // x is reused by the next solve
+    return b;
+  }
+};
 
 // LinLdsU: M in LDS for n <= NP (32), every loop fully unrolled so that the
 // LDS loads of a step issue together, ahead of the dependency chain.  Lane L
@@ -2682,6 +2816,72 @@ using namespace p3d;
 
 extern "C" {
 
+// Static elimination schedule of LinSparseG from the Jacobian structure
+// (plus the diagonal).  Order: greedy minimum Markowitz product over the
+// remaining rows and columns, lowest species on ties, with the fill of each
+// elimination added as it goes.  Level of pivot p: one above the highest
+// level of the pivots k < p with a filled entry at (p, k) or (k, p), 0 when
+// none.
+struct SparseLuSched {
+  int nlev = 0;
+  std::vector<int> pos, lev, lvoff, lvp, uoff, uj;
+};
+
+static void sparse_lu_schedule(int ns, const std::vector<int> &jrow, const std::vector<int> &jcol,
+                               SparseLuSched &s) {
+  std::vector<char> A((size_t)ns * ns, 0);  // species coordinates
+  for (int i = 0; i < ns; ++i) A[(size_t)i * ns + i] = 1;
+  for (size_t z = 0; z < jrow.size(); ++z) A[(size_t)jrow[z] * ns + jcol[z]] = 1;
+  std::vector<char> left(ns, 1);
+  std::vector<int> order;
+  for (int step = 0; step < ns; ++step) {
+    long best = -1;
+    int bk = -1;
+    for (int k = 0; k < ns; ++k) {
+      if (!left[k]) continue;
+      long rc = 0, cc = 0;
+      for (int j = 0; j < ns; ++j)
+        if (left[j] && j != k) {
+          rc += A[(size_t)k * ns + j];
+          cc += A[(size_t)j * ns + k];
+        }
+      if (best < 0 || rc * cc < best) {
+        best = rc * cc;
+        bk = k;
+      }
+    }
+    order.push_back(bk);
+    left[bk] = 0;
+    for (int i = 0; i < ns; ++i)
+      if (left[i] && A[(size_t)i * ns + bk])
+        for (int j = 0; j < ns; ++j)
+          if (left[j] && A[(size_t)bk * ns + j]) A[(size_t)i * ns + j] = 1;
+  }
+  s.pos.assign(ns, 0);
+  for (int p = 0; p < ns; ++p) s.pos[order[p]] = p;
+  auto F = [&](int p, int q) { return A[(size_t)order[p] * ns + order[q]] != 0; };
+  s.lev.assign(ns, 0);
+  for (int p = 0; p < ns; ++p)
+    for (int k = 0; k < p; ++k)
+      if (F(p, k) || F(k, p)) s.lev[p] = std::max(s.lev[p], s.lev[k] + 1);
+  s.nlev = 0;
+  for (int p = 0; p < ns; ++p) s.nlev = std::max(s.nlev, s.lev[p] + 1);
+  s.lvoff.assign(1, 0);
+  s.lvp.clear();
+  for (int v = 0; v < s.nlev; ++v) {
+    for (int p = 0; p < ns; ++p)
+      if (s.lev[p] == v) s.lvp.push_back(p);
+    s.lvoff.push_back((int)s.lvp.size());
+  }
+  s.uoff.assign(1, 0);
+  s.uj.clear();
+  for (int p = 0; p < ns; ++p) {
+    for (int j = p + 1; j < ns; ++j)
+      if (F(p, j)) s.uj.push_back(j);
+    s.uoff.push_back((int)s.uj.size());
+  }
+}
+
 int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const double *alpha,
                        const double *beta, const double *gamma, const int32_t *reactants,
                        const int32_t *products, int32_t e_index, const int32_t *roles7,
@@ -2873,6 +3073,12 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
          o_js = put(jr_start.data(), 4 * jr_start.size()),
          o_jm = put(jr_mask.data(), 4 * jr_mask.size()),
          o_jq = put(jsched.data(), 4 * jsched.size());
+  SparseLuSched sl;
+  sparse_lu_schedule(ns, jrow, jcol, sl);
+  size_t o_sp = put(sl.pos.data(), 4 * sl.pos.size()), o_sv = put(sl.lev.data(), 4 * sl.lev.size()),
+         o_slo = put(sl.lvoff.data(), 4 * sl.lvoff.size()),
+         o_slp = put(sl.lvp.data(), 4 * sl.lvp.size()), o_suo = put(sl.uoff.data(), 4 * sl.uoff.size()),
+         o_suj = put(sl.uj.data(), 4 * sl.uj.size());
   auto net = std::make_unique<Network>();
   if (hipMalloc(&net->blob, host.size()) != hipSuccess) {
     set_error("hipMalloc of %zu bytes for network tables failed", host.size());
@@ -2909,6 +3115,13 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   d.pc_beg = (const int *)(b + o_pb);
   d.pc_lane = (const int *)(b + o_pl);
   d.pc_row = (const int *)(b + o_pr);
+  d.sl_nlev = sl.nlev;
+  d.sl_pos = (const int *)(b + o_sp);
+  d.sl_lev = (const int *)(b + o_sv);
+  d.sl_lvoff = (const int *)(b + o_slo);
+  d.sl_lvp = (const int *)(b + o_slp);
+  d.sl_uoff = (const int *)(b + o_suo);
+  d.sl_uj = (const int *)(b + o_suj);
   // the generated register-LU code and its LDS layout fix the species,
   // reactions and Jacobian structure
   net->lane_ok = roles7 != nullptr &&
@@ -2918,6 +3131,31 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   int64_t h = g_next_handle++;
   g_nets[h] = std::move(net);
   *handle = h;
+  return P3D_OK;
+}
+
+int p3d_sparse_lu_schedule(int32_t ns, int32_t nnz, const int32_t *jrow, const int32_t *jcol,
+                           int32_t cap, int32_t *pos, int32_t *lev, int32_t *lvoff, int32_t *lvp,
+                           int32_t *uoff, int32_t *uj, int32_t *counts) {
+  P3D_REQUIRE(ns >= 1 && ns <= kMaxSpeciesWide, "n_species must be in [1, %d] (got %d)",
+              kMaxSpeciesWide, ns);
+  P3D_REQUIRE(nnz >= 0 && (nnz == 0 || (jrow && jcol)), "null structure");
+  P3D_REQUIRE(pos && lev && lvoff && lvp && uoff && uj && counts, "null output");
+  std::vector<int> r(jrow, jrow + nnz), c(jcol, jcol + nnz);
+  for (int z = 0; z < nnz; ++z)
+    P3D_REQUIRE(r[z] >= 0 && r[z] < ns && c[z] >= 0 && c[z] < ns, "entry %d out of range", z);
+  SparseLuSched sl;
+  sparse_lu_schedule(ns, r, c, sl);
+  P3D_REQUIRE((int64_t)sl.uj.size() <= cap, "cap %d below the %zu upper entries", cap,
+              sl.uj.size());
+  std::copy(sl.pos.begin(), sl.pos.end(), pos);
+  std::copy(sl.lev.begin(), sl.lev.end(), lev);
+  std::copy(sl.lvoff.begin(), sl.lvoff.end(), lvoff);
+  std::copy(sl.lvp.begin(), sl.lvp.end(), lvp);
+  std::copy(sl.uoff.begin(), sl.uoff.end(), uoff);
+  std::copy(sl.uj.begin(), sl.uj.end(), uj);
+  counts[0] = sl.nlev;
+  counts[1] = (int32_t)sl.uj.size();
   return P3D_OK;
 }
 
@@ -3152,7 +3390,7 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
     for (int k = 0; k < 7; ++k)
       P3D_REQUIRE(net->d.roles[k] >= 0,
                   "post-solve corrections / early stop need the species roles of the network");
-  P3D_REQUIRE(o.variant >= 0 && o.variant <= 6, "variant must be 0 .. 6");
+  P3D_REQUIRE(o.variant >= 0 && o.variant <= 7, "variant must be 0 .. 7");
   P3D_REQUIRE(o.fast == 0 || o.fast == 1, "fast must be 0 or 1");
   if (o.fast) {
     P3D_REQUIRE(net->d.ns <= kMaxSpecies, "the fast mode takes at most %d species", kMaxSpecies);
@@ -3179,27 +3417,35 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
   if (o.variant == 6 || (o.variant == 0 && net->lane_ok))
     return queue_solve(net, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co, o, converged,
                        stats, stream);
-  if (net->d.ns > kMaxSpecies) {
-    // two wavefronts per cell, lane = species, MFMA panel LU
-    P3D_REQUIRE(o.variant == 0 || o.variant == 5,
-                "networks of more than %d species use the two-wavefront MFMA form "
-                "(variant 0 or 5; got %d)", kMaxSpecies, o.variant);
+  // auto from 64 species: the static sparse LU by levels (7), measured 1.2x
+  // the MFMA panel LU (5) on the 64- and 103-species synthetic networks with
+  // the same step counts
+  if (net->d.ns > kMaxSpecies || o.variant == 7 || (o.variant == 0 && net->d.ns >= 64)) {
+    // two wavefronts per cell, lane = species: the MFMA panel LU with
+    // partial pivoting (5), or the static sparse LU by levels (7)
+    P3D_REQUIRE(net->d.ns <= kMaxSpecies || o.variant == 0 || o.variant == 5 || o.variant == 7,
+                "networks of more than %d species use the two-wavefront forms "
+                "(variant 0, 5 or 7; got %d)", kMaxSpecies, o.variant);
     const size_t shw = lds_bytes_wide(net->d.ns, net->d.nr, net->d.nnz);
     P3D_REQUIRE(shw <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", shw);
-    hipLaunchKernelGGL((chem_solve_kernel<LinMfmaG<Wave2, true>, Wave2>), dim3((unsigned)n),
-                       dim3(Wave2::W), shw, as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v,
-                       zeta, fsh_h2, fsh_co, o, converged, stats, nullptr, nullptr);
+    if (o.variant == 7 || o.variant == 0)
+      hipLaunchKernelGGL((chem_solve_kernel<LinSparseG<Wave2>, Wave2>), dim3((unsigned)n),
+                         dim3(Wave2::W), shw, as_stream(stream), net->d, n, y, n_h, t_gas, g0,
+                         a_v, zeta, fsh_h2, fsh_co, o, converged, stats, nullptr, nullptr);
+    else
+      hipLaunchKernelGGL((chem_solve_kernel<LinMfmaG<Wave2, true>, Wave2>), dim3((unsigned)n),
+                         dim3(Wave2::W), shw, as_stream(stream), net->d, n, y, n_h, t_gas, g0,
+                         a_v, zeta, fsh_h2, fsh_co, o, converged, stats, nullptr, nullptr);
     return launch_status("chem_solve_kernel");
   }
   const size_t sh = lds_bytes(net->d.ns, net->d.nr, net->d.n_rhs_e);
   P3D_REQUIRE(sh <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", sh);
   int var = o.variant;
-  // auto: the MFMA trailing update from 64 species (where the panel products
-  // are 16 x 16 x 4 tiles), else the LU in LDS.  The register LU (2) gives
+  // auto below 64 species: the LU in LDS.  The register LU (2) gives
   // the same bits but needs 412 registers, one wavefront per SIMD: on the
   // 64^3 workload's converged inputs it took 10.1 s against 7.1 s for the
   // LDS LU (191 VGPRs, two per SIMD, no scratch).
-  if (var == 0) var = net->d.ns >= 64 ? 5 : 1;
+  if (var == 0) var = 1;
   if (var == 5)
     hipLaunchKernelGGL(chem_solve_kernel<LinMfma>, dim3((unsigned)n), dim3(64), sh,
                        as_stream(stream), net->d, n, y, n_h, t_gas, g0, a_v, zeta, fsh_h2, fsh_co,

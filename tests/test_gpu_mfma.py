@@ -1,6 +1,7 @@
 """GPU parity of the workgroup solve with the FP64-MFMA trailing-update LU
-(p3d_chem_solve variant 5): on the bundled network against the oracle and
-the upstream scipy fixture, and on the 64-species synthetic network against
+(p3d_chem_solve variant 5) and with the static sparse LU by levels
+(variant 7): on the bundled network against the oracle and the upstream
+scipy fixture, and on the 64- and 103-species synthetic networks against
 scipy's BDF on the oracle's restated RHS and against the LDS LU."""
 
 import numpy as np
@@ -11,18 +12,19 @@ from conftest import golden
 pytestmark = pytest.mark.gpu
 
 
-def test_mfma_lu_bundled_network_tight_fixture(gpu):
+@pytest.mark.parametrize("variant", [5, 7])
+def test_mfma_lu_bundled_network_tight_fixture(gpu, variant):
     import torch
     from oracle import oracle as O
     from prism3d_amd.chemistry import ChemistrySolver, solve_opts
-    print("case: variant 5 on the tight fixture", flush=True)
+    print(f"case: variant {variant} on the tight fixture", flush=True)
     g = golden("tight_golden.npz")
     n = g['n_H'].size
     y0 = np.repeat(g['y0'][:, None], n, axis=1)
     t_end = float(g['t_end'])
     y = torch.tensor(y0, device=gpu)
     opts = solve_opts(t_max=t_end, n_checkpoints=2, early_stop=False, rtol=1e-9, atol=1e-24,
-                      post_mode=-1, variant=5)
+                      post_mode=-1, variant=variant)
     _, st = ChemistrySolver().solve_steady_batch(y, g['n_H'], g['T'], g['G0'], g['A_V'],
                                                  g['zeta'], g['fsh_H2'], g['fsh_CO'], opts=opts)
     got = y.cpu().numpy()
@@ -34,7 +36,8 @@ def test_mfma_lu_bundled_network_tight_fixture(gpu):
     ref = g['y_end'].T
     m = ref > 1e-14
     e_up = np.max(np.abs(got[m] - ref[m]) / ref[m])
-    print(f"variant 5 tight: vs oracle {e_or:.2e}, vs upstream scipy {e_up:.2e}", flush=True)
+    print(f"variant {variant} tight: vs oracle {e_or:.2e}, vs upstream scipy {e_up:.2e}",
+          flush=True)
     assert e_or < 1e-6 and e_up < 1e-6
     assert (st.cpu().numpy()[:, 1] > 0).all()
 
@@ -48,9 +51,10 @@ def _synth_solver():
 
 
 def test_synthetic_network_against_scipy(gpu):
-    """Six cells of the 64-species network integrated to 1e7 yr: both LU
-    forms against scipy's BDF (rtol 1e-10) on the restated RHS, species
-    above 1e-14."""
+    """Six cells of the 64-species network integrated to 1e7 yr: the three LU
+    forms (MFMA panel, LDS, and the static sparse LU by levels on two
+    wavefronts) against scipy's BDF (rtol 1e-10) on the restated RHS,
+    species above 1e-14."""
     import torch
     from prism3d_amd.chemistry import solve_opts
     g = golden("synth_golden.npz")
@@ -59,7 +63,7 @@ def test_synthetic_network_against_scipy(gpu):
     n = par.shape[0]
     ref = g['y_end'].T
     m = ref > 1e-14
-    for var in (5, 1):
+    for var in (5, 1, 7):
         print(f"case: synthetic network, variant {var}", flush=True)
         y = torch.tensor(np.repeat(g['y0'][:, None], n, axis=1), device=gpu)
         opts = solve_opts(t_max=float(g['t_end']), n_checkpoints=2, early_stop=False, rtol=1e-9,
@@ -75,11 +79,13 @@ def test_synthetic_network_against_scipy(gpu):
 
 def test_mfma_and_lds_lu_agree_on_random_cells(gpu):
     """512 cells over PDR conditions on the 64-species network: the MFMA
-    panel LU and the unblocked LDS LU give the same steady states (they
-    differ only in the rounding of the trailing updates)."""
+    panel LU, the unblocked LDS LU and the static sparse LU by levels give
+    the same steady states (they differ only in the rounding of the
+    eliminations; the sparse LU also in its order and the absence of row
+    interchanges)."""
     import torch
     from prism3d_amd.chemistry import solve_opts
-    print("case: synthetic network, variant 5 vs 1 on 512 cells", flush=True)
+    print("case: synthetic network, variants 5 and 7 vs 1 on 512 cells", flush=True)
     sp, cs = _synth_solver()
     rng = np.random.RandomState(9)
     n = 512
@@ -88,25 +94,27 @@ def test_mfma_and_lds_lu_agree_on_random_cells(gpu):
            10 ** rng.uniform(-4, 0, n), 10 ** rng.uniform(-4, 0, n)]
     g = golden("synth_golden.npz")
     out = {}
-    for var in (5, 1):
+    for var in (5, 1, 7):
         y = torch.tensor(np.repeat(g['y0'][:, None], n, axis=1), device=gpu)
         opts = solve_opts(t_max=float(g['t_end']), n_checkpoints=2, early_stop=False, rtol=1e-9,
                           atol=1e-24, post_mode=-1, variant=var)
         _, st = cs.solve_steady_batch(y, *par, opts=opts)
         out[var] = (y.cpu().numpy(), st.cpu().numpy())
-    a, b = out[5][0], out[1][0]
+    b = out[1][0]
     m = b > 1e-14
-    rel = np.abs(a[m] - b[m]) / b[m]
-    print(f"variant 5 vs 1: p99.9 {np.percentile(rel, 99.9):.2e}, max {rel.max():.2e}; "
-          f"steps {out[5][1][:, 0].mean():.0f} / {out[1][1][:, 0].mean():.0f}", flush=True)
-    assert np.percentile(rel, 99.9) < 1e-6
-    assert rel.max() < 1e-4
+    for var in (5, 7):
+        a = out[var][0]
+        rel = np.abs(a[m] - b[m]) / b[m]
+        print(f"variant {var} vs 1: p99.9 {np.percentile(rel, 99.9):.2e}, max {rel.max():.2e}; "
+              f"steps {out[var][1][:, 0].mean():.0f} / {out[1][1][:, 0].mean():.0f}", flush=True)
+        assert np.percentile(rel, 99.9) < 1e-6
+        assert rel.max() < 1e-4
 
 
 def test_oversized_networks_are_refused(gpu):
     """Networks of up to 128 species are taken at creation, larger ones
     refused.  Above 64 species only the solve runs (two wavefronts per
-    cell, MFMA LU); the entry points that map a species to a lane of one
+    cell, MFMA or sparse LU); the entry points that map a species to a lane of one
     wavefront, and the one-wavefront solve variants, refuse such a
     network."""
     import torch
@@ -137,10 +145,11 @@ def _wide_solver():
 
 def test_wide_network_against_scipy(gpu):
     """The fixture cells of the 103-species network (two wavefronts per cell,
-    sparse J and the MFMA panel LU in LDS) integrated to 1e7 yr against
+    sparse J in LDS; the MFMA panel LU with partial pivoting, variant 5, and
+    the static sparse LU by levels, variant 7) integrated to 1e7 yr against
     scipy's BDF (rtol 1e-10) on the restated RHS, species above 1e-14; the
-    auto variant picks the same form.  (Of the six drawn cells, those scipy
-    did not finish within the fixture script's time limit are not here.)"""
+    auto variant picks variant 7.  (Of the six drawn cells, those scipy did
+    not finish within the fixture script's time limit are not here.)"""
     import torch
     from prism3d_amd.chemistry import solve_opts
     g = golden("synth_wide_golden.npz")
@@ -150,7 +159,7 @@ def test_wide_network_against_scipy(gpu):
     ref = g['y_end'].T
     m = ref > 1e-14
     got = {}
-    for var in (5, 0):
+    for var in (5, 7, 0):
         print(f"case: 103-species network, variant {var}", flush=True)
         y = torch.tensor(np.repeat(g['y0'][:, None], n, axis=1), device=gpu)
         opts = solve_opts(t_max=float(g['t_end']), n_checkpoints=2, early_stop=False, rtol=1e-9,
@@ -162,7 +171,7 @@ def test_wide_network_against_scipy(gpu):
               f"steps {st.cpu().numpy()[:, 0].mean():.0f}", flush=True)
         assert rel.max() < 1e-5
         assert np.median(rel) < 1e-8
-    assert np.array_equal(got[5], got[0])
+    assert np.array_equal(got[7], got[0])
 
 
 def test_wide_network_conserves_nuclei(gpu):
