@@ -85,6 +85,13 @@ struct NetDev {
   // sl_uoff[p+1]-1]
   int sl_nlev;
   const int *sl_pos, *sl_lev, *sl_lvoff, *sl_lvp, *sl_uoff, *sl_uj;
+  // the same schedule packed for LDS (LinSparseG): descriptor q of the
+  // pivots in level order = k | len << 7 | ubeg << 14 | last of its level
+  // << 31, len upper entries at bytes ubeg.. of sl_ub (each list starts on
+  // a 4-byte boundary); sl_zb[z] = position of column jcol[z]; sl_pb[s] =
+  // position of species s.  Byte arrays padded to whole words.
+  int sl_nub, sl_nzb;
+  const unsigned *sl_desc, *sl_ub, *sl_zb, *sl_pb;
 };
 
 struct Network {
@@ -137,6 +144,11 @@ struct Lds {
   double *red;   // two-wavefront form: 4 doubles of cross-wave reduction slots
   double *xs;    // components of the triangular solves (128 two-wavefront, 32 register LU)
   double *psum;  // two-wavefront form: partial sums of the RHS pieces
+  // static sparse LU (LinSparseG), staged from NetDev once per workgroup:
+  // one descriptor per pivot, the upper-row positions and the scatter
+  // positions of the Jacobian entries as bytes, the species positions
+  unsigned *sd;
+  unsigned char *su, *sz, *sp;
   int ld;
 };
 
@@ -184,6 +196,10 @@ __device__ __forceinline__ Lds carve_wide(double *base, const NetDev &N) {
   L.piv = reinterpret_cast<int *>(L.ru + 36);
   L.csr_off = nullptr;
   L.csr = nullptr;
+  L.sd = reinterpret_cast<unsigned *>(L.piv + kMaxSpeciesWide);
+  L.su = reinterpret_cast<unsigned char *>(L.sd + kMaxSpeciesWide);
+  L.sz = L.su + N.sl_nub;
+  L.sp = L.sz + N.sl_nzb;
   return L;
 }
 
@@ -231,6 +247,11 @@ static size_t lds_bytes_wide(int ns, int nr, int nnz) {
   return sizeof(double) * (size_t)(3 * nr + 2 * kMaxSpeciesWide + kMaxPieces + nnz + ns * ld +
                                    4 + 36) +
          sizeof(int) * (size_t)kMaxSpeciesWide;
+}
+// the two-wavefront form with the static sparse LU's staged schedule
+static size_t lds_bytes_wide_sparse(const NetDev &d) {
+  return lds_bytes_wide(d.ns, d.nr, d.nnz) + sizeof(unsigned) * (size_t)kMaxSpeciesWide +
+         (size_t)d.sl_nub + (size_t)d.sl_nzb + (size_t)kMaxSpeciesWide;
 }
 
 // Workgroup collectives.  Wave1: one wavefront, the DPP/readlane collectives.
@@ -1012,59 +1033,78 @@ template <class G>
 struct LinSparseG {
   __device__ void load_jac(const NetDev &, const Lds &, int) {}
 
+  // the schedule into LDS (before the first factorisation; every lane calls)
+  __device__ static void stage(const NetDev &N, const Lds &L, int lane) {
+    for (int i = lane; i < N.ns; i += G::W) L.sd[i] = N.sl_desc[i];
+    unsigned *su = reinterpret_cast<unsigned *>(L.su);
+    for (int i = lane; i < (N.sl_nub >> 2); i += G::W) su[i] = N.sl_ub[i];
+    unsigned *sz = reinterpret_cast<unsigned *>(L.sz);
+    for (int i = lane; i < (N.sl_nzb >> 2); i += G::W) sz[i] = N.sl_zb[i];
+    unsigned *sp = reinterpret_cast<unsigned *>(L.sp);
+    for (int i = lane; i < (kMaxSpeciesWide >> 2); i += G::W) sp[i] = N.sl_pb[i];
+  }
+
   __device__ void factor(const NetDev &N, const Lds &L, double c, int lane) {
     const int ld = L.ld, ns = N.ns;
     const bool act = lane < ns;
-    const int p = act ? N.sl_pos[lane] : ns;  // lanes past the species: no row
+    const int p = act ? (int)L.sp[lane] : ns;  // lanes past the species: no row
     const int lev = act ? N.sl_lev[p] : -2;
     double *Mp = L.M + (act ? p : 0) * ld;
     if (act) {
+      const int z0 = N.jr_start[lane], z1 = N.jr_start[lane + 1];
       for (int j = 0; j < ns; ++j) Mp[j] = 0.0;
       Mp[p] = 1.0;
-      for (int z = N.jr_start[lane]; z < N.jr_start[lane + 1]; ++z) {
-        const int j = N.jcol[z];
-        Mp[N.sl_pos[j]] = (j == lane ? 1.0 : 0.0) - c * L.J[z];
+      for (int z = z0; z < z1; ++z) {
+        const int j = L.sz[z];
+        Mp[j] = (j == p ? 1.0 : 0.0) - c * L.J[z];
       }
       if (lev == 0) Mp[p] = 1.0 / Mp[p];
     }
     __syncthreads();
-    for (int v = 0; v < N.sl_nlev; ++v) {
-      const int q0 = N.sl_lvoff[v], q1 = N.sl_lvoff[v + 1];
-      for (int q = q0; q < q1; ++q) {
-        const int k = N.sl_lvp[q];
-        const double *Mk = L.M + k * ld;
-        double l = 0.0;
-        if (p > k && p < ns) l = Mp[k];
-        if (l != 0.0) {
-          l *= Mk[k];
-          Mp[k] = l;
-          const int u1 = N.sl_uoff[k + 1];
-          int u = N.sl_uoff[k];
-          for (; u + 4 <= u1; u += 4) {
-            const int j0 = N.sl_uj[u], j1 = N.sl_uj[u + 1], j2 = N.sl_uj[u + 2],
-                      j3 = N.sl_uj[u + 3];
-            const double b0 = Mk[j0], b1 = Mk[j1], b2 = Mk[j2], b3 = Mk[j3];
-            const double a0 = Mp[j0], a1 = Mp[j1], a2 = Mp[j2], a3 = Mp[j3];
-            Mp[j0] = a0 - l * b0;
-            Mp[j1] = a1 - l * b1;
-            Mp[j2] = a2 - l * b2;
-            Mp[j3] = a3 - l * b3;
-          }
-          for (; u < u1; ++u) {
-            const int j = N.sl_uj[u];
+    int v = 0;
+    for (int q = 0; q < ns; ++q) {
+      // wave-uniform descriptor: scalar fields and branches
+      const unsigned d = __builtin_amdgcn_readfirstlane(L.sd[q]);
+      const int k = (int)(d & 127u), len = (int)((d >> 7) & 127u);
+      const int ub = (int)((d >> 14) & 0x1ffffu);
+      const double *Mk = L.M + k * ld;
+      double l = 0.0;
+      if (p > k && p < ns) l = Mp[k];
+      if (l != 0.0) {
+        l *= Mk[k];
+        Mp[k] = l;
+        const unsigned *Uw = reinterpret_cast<const unsigned *>(L.su + ub);
+        int w = 0;
+        for (; w + 4 <= len; w += 4) {
+          const unsigned jj = Uw[w >> 2];
+          const int j0 = jj & 0xff, j1 = (jj >> 8) & 0xff, j2 = (jj >> 16) & 0xff, j3 = jj >> 24;
+          const double b0 = Mk[j0], b1 = Mk[j1], b2 = Mk[j2], b3 = Mk[j3];
+          const double a0 = Mp[j0], a1 = Mp[j1], a2 = Mp[j2], a3 = Mp[j3];
+          Mp[j0] = a0 - l * b0;
+          Mp[j1] = a1 - l * b1;
+          Mp[j2] = a2 - l * b2;
+          Mp[j3] = a3 - l * b3;
+        }
+        if (w < len) {
+          const unsigned jj = Uw[w >> 2];
+          for (int t = 0; t < len - w; ++t) {
+            const int j = (jj >> (8 * t)) & 0xff;
             Mp[j] -= l * Mk[j];
           }
         }
       }
-      if (lev == v + 1) Mp[p] = 1.0 / Mp[p];
-      __syncthreads();
+      if (d >> 31) {  // the last pivot of level v
+        if (lev == v + 1) Mp[p] = 1.0 / Mp[p];
+        __syncthreads();
+        ++v;
+      }
     }
   }
 
   __device__ double solve(const NetDev &N, const Lds &L, double b, int lane) {
     const int n = N.ns, ld = L.ld;
     double *x = L.xs;
-    if (lane < n) x[N.sl_pos[lane]] = b;
+    if (lane < n) x[L.sp[lane]] = b;
     __syncthreads();
     if (lane < 64) {
       const int r0 = lane, r1 = lane + 64;
@@ -1114,7 +1154,7 @@ struct LinSparseG {
       if (r1 < kMaxSpeciesWide) x[r1] = b1;
     }
     __syncthreads();
-    b = lane < n ? x[N.sl_pos[lane]] : 0.0;
+    b = lane < n ? x[L.sp[lane]] : 0.0;
     __syncthreads();  // x is reused by the next solve
     return b;
   }
@@ -1821,6 +1861,7 @@ chem_solve_kernel(NetDev N, int64_t n, double *__restrict__ y, const double *__r
   if constexpr (G::W == 64) L = carve(smem, N);
   else L = carve_wide(smem, N);
   if constexpr (G::W == 64) stage_tables(N, L, threadIdx.x);
+  if constexpr (std::is_same<Lin, LinSparseG<G>>::value) Lin::stage(N, L, threadIdx.x);
   solve_cell<Lin, G>(N, L, cell, n, y, nH, T, G0, AV, zeta, fH2, fCO, o, conv_out, stats_out);
 }
 
@@ -3075,6 +3116,26 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
          o_jq = put(jsched.data(), 4 * jsched.size());
   SparseLuSched sl;
   sparse_lu_schedule(ns, jrow, jcol, sl);
+  // the schedule packed for LDS: pivot descriptors in level order, upper
+  // positions and scatter positions as bytes (positions < 128)
+  std::vector<unsigned> sdesc(ns, 0u);
+  std::vector<unsigned char> sub, szb(nnz), spb(kMaxSpeciesWide, 0);
+  for (int v = 0; v < sl.nlev; ++v)
+    for (int q = sl.lvoff[v]; q < sl.lvoff[v + 1]; ++q) {
+      const int k = sl.lvp[q], len = sl.uoff[k + 1] - sl.uoff[k];
+      while (sub.size() % 4) sub.push_back(0);
+      const size_t ubeg = sub.size();
+      for (int u = sl.uoff[k]; u < sl.uoff[k + 1]; ++u) sub.push_back((unsigned char)sl.uj[u]);
+      P3D_REQUIRE(len < 128 && ubeg < (1u << 17), "internal: sparse LU schedule too large");
+      sdesc[q] = (unsigned)k | ((unsigned)len << 7) | ((unsigned)ubeg << 14) |
+                 (q == sl.lvoff[v + 1] - 1 ? 1u << 31 : 0u);
+    }
+  do sub.push_back(0); while (sub.size() % 4);
+  for (int z = 0; z < nnz; ++z) szb[z] = (unsigned char)sl.pos[jcol[z]];
+  do szb.push_back(0); while (szb.size() % 4);
+  for (int i = 0; i < ns; ++i) spb[i] = (unsigned char)sl.pos[i];
+  const size_t o_sdd = put(sdesc.data(), 4 * sdesc.size()), o_sub = put(sub.data(), sub.size()),
+               o_szb = put(szb.data(), szb.size()), o_spb = put(spb.data(), spb.size());
   size_t o_sp = put(sl.pos.data(), 4 * sl.pos.size()), o_sv = put(sl.lev.data(), 4 * sl.lev.size()),
          o_slo = put(sl.lvoff.data(), 4 * sl.lvoff.size()),
          o_slp = put(sl.lvp.data(), 4 * sl.lvp.size()), o_suo = put(sl.uoff.data(), 4 * sl.uoff.size()),
@@ -3122,6 +3183,12 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
   d.sl_lvp = (const int *)(b + o_slp);
   d.sl_uoff = (const int *)(b + o_suo);
   d.sl_uj = (const int *)(b + o_suj);
+  d.sl_nub = (int)sub.size();
+  d.sl_nzb = (int)szb.size();
+  d.sl_desc = (const unsigned *)(b + o_sdd);
+  d.sl_ub = (const unsigned *)(b + o_sub);
+  d.sl_zb = (const unsigned *)(b + o_szb);
+  d.sl_pb = (const unsigned *)(b + o_spb);
   // the generated register-LU code and its LDS layout fix the species,
   // reactions and Jacobian structure
   net->lane_ok = roles7 != nullptr &&
@@ -3426,9 +3493,11 @@ int p3d_chem_solve(int64_t handle, int64_t n, double *y, const double *n_h, cons
     P3D_REQUIRE(net->d.ns <= kMaxSpecies || o.variant == 0 || o.variant == 5 || o.variant == 7,
                 "networks of more than %d species use the two-wavefront forms "
                 "(variant 0, 5 or 7; got %d)", kMaxSpecies, o.variant);
-    const size_t shw = lds_bytes_wide(net->d.ns, net->d.nr, net->d.nnz);
+    const bool sparse = o.variant == 7 || o.variant == 0;
+    const size_t shw = sparse ? lds_bytes_wide_sparse(net->d)
+                              : lds_bytes_wide(net->d.ns, net->d.nr, net->d.nnz);
     P3D_REQUIRE(shw <= 160 * 1024, "network too large for one LDS partition (%zu bytes)", shw);
-    if (o.variant == 7 || o.variant == 0)
+    if (sparse)
       hipLaunchKernelGGL((chem_solve_kernel<LinSparseG<Wave2>, Wave2>), dim3((unsigned)n),
                          dim3(Wave2::W), shw, as_stream(stream), net->d, n, y, n_h, t_gas, g0,
                          a_v, zeta, fsh_h2, fsh_co, o, converged, stats, nullptr, nullptr);
