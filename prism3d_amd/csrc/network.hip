@@ -77,12 +77,13 @@ struct NetDev {
   const int *pc_beg;   // n_pc + 1 entry offsets
   const int *pc_lane;  // 129: pieces of lane l are pc_lane[l] .. pc_lane[l+1]-1
   const int *pc_row;   // ns + 1: pieces of species s
-  // static elimination schedule of the sparse no-pivot LU (LinSparseG):
-  // species s sits at position sl_pos[s] of the minimum-Markowitz order;
-  // sl_lev[p] = dependency level of position p; the pivots of level v are
-  // sl_lvp[sl_lvoff[v] .. sl_lvoff[v+1]-1] (position order); row position p
-  // of the filled structure has upper entries at sl_uj[sl_uoff[p] ..
-  // sl_uoff[p+1]-1]
+  // static elimination schedule of the sparse no-pivot LU (LinSparseG), as
+  // p3d_sparse_lu_schedule returns it: species s sits at position sl_pos[s]
+  // of the minimum-Markowitz order; sl_lev[p] = dependency level of position
+  // p (read by the factorisation); the pivots of level v are sl_lvp[
+  // sl_lvoff[v] .. sl_lvoff[v+1]-1] (position order); row position p of the
+  // filled structure has upper entries at sl_uj[sl_uoff[p] .. sl_uoff[p+1]-1]
+  // (the kernels walk the packed LDS copy below instead)
   int sl_nlev;
   const int *sl_pos, *sl_lev, *sl_lvoff, *sl_lvp, *sl_uoff, *sl_uj;
   // the same schedule packed for LDS (LinSparseG): descriptor q of the
