@@ -60,7 +60,10 @@ def solve_opts(t_max=None, t_start=1e4, n_checkpoints=20, early_stop=True, rtol=
     """p3d_solve_opts; the defaults are the upstream refine-path settings
     (solver.py:150-223).  variant: 0 auto, 1-3 one cell per 64-lane
     workgroup (LU in LDS / registers / unrolled LDS), 4 persistent
-    one-cell-per-lane kernel; waves_per_cu sizes variant 4's resident grid and
+    one-cell-per-lane kernel, 5 the FP64-MFMA panel LU, 6 the persistent
+    one-cell-per-wavefront kernel (the bundled network's automatic form), 7
+    the static sparse LU by levels on two wavefronts per cell (the automatic
+    form from 64 species); waves_per_cu sizes variant 4's resident grid and
     rescue_frac sets when its long-tail cells move to the workgroup kernel;
     rate_fp32 evaluates the rate coefficients' exp/log in FP32 (the solve
     itself stays FP64).  max_steps caps the BDF steps of one span (a span
