@@ -55,13 +55,14 @@ def flops_per_cell(stats, n=N_SP, n_rx=N_RX, n_terms=300):
 
 def sparse_lu_counts(net):
     """FP64 operations of one factorisation and one pair of substitutions of
-    the static sparse LU (variant 7) on `net`: its multipliers (one multiply
-    each) and updates (multiply-add = 2) over the filled structure of the
-    elimination order p3d_sparse_lu_schedule returns, and 2 per filled
-    off-diagonal entry plus n per substitution pair."""
+    the static sparse LU (variant 7) on `net`: the sparse algorithm's counts
+    of prism3d_amd.gen_lane.op_counts (matrix build, multipliers, updates over
+    the filled structure of the minimum-Markowitz order, reciprocal pivots;
+    multiply-add = 2), the order p3d_sparse_lu_schedule builds, and the
+    number of dependency levels of that schedule."""
     import ctypes
     from prism3d_amd import _lib
-    from prism3d_amd.gen_lane import fill, structure
+    from prism3d_amd.gen_lane import op_counts, structure
     ns = net.n_species
     _, _, terms = structure(net)
     ij = np.array(sorted(terms), dtype=np.int32)
@@ -73,13 +74,8 @@ def sparse_lu_counts(net):
     rc = _lib.lib().p3d_sparse_lu_schedule(ns, jr.size, P(jr), P(jc), ns * ns, P(pos), P(lev),
                                             P(lvoff), P(lvp), P(uoff), P(uj), P(counts))
     assert rc == 0
-    perm = np.argsort(pos)
-    A = np.eye(ns, dtype=bool)
-    A[jr, jc] = True
-    F = fill(A[np.ix_(perm, perm)], ns)
-    low, up = np.tril(F, -1), np.triu(F, 1)
-    lu = int(sum(low[:, k].sum() * (1 + 2 * up[k].sum()) for k in range(ns)))
-    return {"lu": lu, "solve": int(2 * (low.sum() + up.sum()) + ns), "levels": int(counts[0])}
+    cnt = op_counts(net)
+    return {"lu": int(cnt["lu"]), "solve": int(cnt["solve"]), "levels": int(counts[0])}
 
 
 def synth_line(args):
