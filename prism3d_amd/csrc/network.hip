@@ -1018,12 +1018,13 @@ using LinMfma = LinMfmaG<Wave1, false>;
 
 // LinSparseG: M = I - c J in LDS in the network's static elimination order
 // (minimum Markowitz product, no pivoting).  Lane = species s, whose row
-// sits at position p = sl_pos[s].  The factorisation is right-looking over
-// dependency levels: in pass v every wavefront walks the pivots of level v
-// in position order (wave-uniform, so their upper-row positions come by
-// scalar loads), and each lane whose row has a multiplier for the pivot
-// applies it (a row without that entry holds an exact zero there, kept
-// from the zeroed row, and skips it).  A pivot's level is above those of
+// sits at position p = sl_pos[s].  The schedule is staged in LDS once per
+// cell (stage()).  The factorisation is right-looking over dependency
+// levels: in pass v every wavefront walks the pivots of level v in position
+// order (each pivot's descriptor made wave-uniform by readfirstlane, its
+// upper-row positions read as bytes by broadcast LDS reads), and each lane
+// whose row has a multiplier for the pivot applies it (a row without that
+// entry holds an exact zero there, kept from the zeroed row, and skips it).  A pivot's level is above those of
 // every pivot sharing an entry of its row or column, so a pass reads only
 // rows that earlier passes finished; a row inverts its diagonal at the end
 // of the pass before its own level.  One barrier per level.  Factors: unit
