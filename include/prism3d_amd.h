@@ -153,6 +153,18 @@ int p3d_sparse_lu_schedule(int32_t n_species, int32_t nnz, const int32_t *jrow,
                            int32_t *lvoff, int32_t *lvp, int32_t *uoff, int32_t *uj,
                            int32_t *counts);
 
+/* The same schedule in the packed form variant 7's kernel stages in LDS
+ * (host only, touches no device; no counterpart in the reference): desc[q]
+ * for the q-th pivot in level order = k | len << 7 | ubeg << 14 | (last
+ * pivot of its level) << 31, its len upper positions as bytes ub[ubeg ..]
+ * (each list starts on a 4-byte boundary); zb[z] = elimination position of
+ * column jcol[z]; pb[s] = position of species s (128 bytes).  cap =
+ * capacity of ub and zb in bytes; sizes = {bytes of ub, bytes of zb}, both
+ * padded to whole 4-byte words. */
+int p3d_sparse_lu_packed(int32_t n_species, int32_t nnz, const int32_t *jrow,
+                         const int32_t *jcol, int32_t cap, uint32_t *desc, uint8_t *ub,
+                         uint8_t *zb, uint8_t *pb, int32_t *sizes);
+
 /* dy/dt for n cells; y and dydt are (n_species, n) device arrays. */
 int p3d_chem_rhs(int64_t handle, int64_t n, const double *y, const double *n_h,
                  const double *t_gas, const double *g0, const double *a_v,

@@ -53,6 +53,7 @@ SIGNATURES = {
                                   _I64P]),
     "p3d_network_destroy": (_i32, [_i64]),
     "p3d_sparse_lu_schedule": (_i32, [_i32, _i32, _I32P, _I32P, _i32] + [_I32P] * 7),
+    "p3d_sparse_lu_packed": (_i32, [_i32, _i32, _I32P, _I32P, _i32, _VP, _VP, _VP, _VP, _I32P]),
     "p3d_chem_rhs": (_i32, [_i64, _i64] + [_VP] * 9 + [_VP]),
     "p3d_chem_jacobian": (_i32, [_i64, _i64] + [_VP] * 9 + [_VP]),
     "p3d_chem_post": (_i32, [_i64, _i64, _VP, _VP, _VP, _VP, _i32, _dbl, _dbl, _VP]),

@@ -2925,6 +2925,40 @@ static void sparse_lu_schedule(int ns, const std::vector<int> &jrow, const std::
   }
 }
 
+// The schedule packed for LDS (LinSparseG::stage): descriptor q of the
+// pivots in level order = k | len << 7 | ubeg << 14 | (last of its level)
+// << 31; the len upper positions of pivot k as bytes from ub[ubeg] (each
+// list starting on a 4-byte boundary); zb[z] = position of column jcol[z];
+// pb[s] = position of species s (kMaxSpeciesWide bytes).  Byte arrays are
+// padded to whole words.  False when a field would not fit its bits.
+struct SparseLuPacked {
+  std::vector<unsigned> desc;
+  std::vector<unsigned char> ub, zb, pb;
+};
+
+static bool pack_sparse_lu(int ns, const std::vector<int> &jcol, const SparseLuSched &sl,
+                           SparseLuPacked &pk) {
+  pk.desc.assign(ns, 0u);
+  pk.ub.clear();
+  pk.zb.assign(jcol.size(), 0);
+  pk.pb.assign(kMaxSpeciesWide, 0);
+  for (int v = 0; v < sl.nlev; ++v)
+    for (int q = sl.lvoff[v]; q < sl.lvoff[v + 1]; ++q) {
+      const int k = sl.lvp[q], len = sl.uoff[k + 1] - sl.uoff[k];
+      while (pk.ub.size() % 4) pk.ub.push_back(0);
+      const size_t ubeg = pk.ub.size();
+      for (int u = sl.uoff[k]; u < sl.uoff[k + 1]; ++u) pk.ub.push_back((unsigned char)sl.uj[u]);
+      if (k >= 128 || len >= 128 || ubeg >= (1u << 17)) return false;
+      pk.desc[q] = (unsigned)k | ((unsigned)len << 7) | ((unsigned)ubeg << 14) |
+                   (q == sl.lvoff[v + 1] - 1 ? 1u << 31 : 0u);
+    }
+  do pk.ub.push_back(0); while (pk.ub.size() % 4);
+  for (size_t z = 0; z < jcol.size(); ++z) pk.zb[z] = (unsigned char)sl.pos[jcol[z]];
+  do pk.zb.push_back(0); while (pk.zb.size() % 4);
+  for (int i = 0; i < ns; ++i) pk.pb[i] = (unsigned char)sl.pos[i];
+  return true;
+}
+
 int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const double *alpha,
                        const double *beta, const double *gamma, const int32_t *reactants,
                        const int32_t *products, int32_t e_index, const int32_t *roles7,
@@ -3118,24 +3152,10 @@ int p3d_network_create(int32_t ns, int32_t nr, const int32_t *rtype, const doubl
          o_jq = put(jsched.data(), 4 * jsched.size());
   SparseLuSched sl;
   sparse_lu_schedule(ns, jrow, jcol, sl);
-  // the schedule packed for LDS: pivot descriptors in level order, upper
-  // positions and scatter positions as bytes (positions < 128)
-  std::vector<unsigned> sdesc(ns, 0u);
-  std::vector<unsigned char> sub, szb(nnz), spb(kMaxSpeciesWide, 0);
-  for (int v = 0; v < sl.nlev; ++v)
-    for (int q = sl.lvoff[v]; q < sl.lvoff[v + 1]; ++q) {
-      const int k = sl.lvp[q], len = sl.uoff[k + 1] - sl.uoff[k];
-      while (sub.size() % 4) sub.push_back(0);
-      const size_t ubeg = sub.size();
-      for (int u = sl.uoff[k]; u < sl.uoff[k + 1]; ++u) sub.push_back((unsigned char)sl.uj[u]);
-      P3D_REQUIRE(len < 128 && ubeg < (1u << 17), "internal: sparse LU schedule too large");
-      sdesc[q] = (unsigned)k | ((unsigned)len << 7) | ((unsigned)ubeg << 14) |
-                 (q == sl.lvoff[v + 1] - 1 ? 1u << 31 : 0u);
-    }
-  do sub.push_back(0); while (sub.size() % 4);
-  for (int z = 0; z < nnz; ++z) szb[z] = (unsigned char)sl.pos[jcol[z]];
-  do szb.push_back(0); while (szb.size() % 4);
-  for (int i = 0; i < ns; ++i) spb[i] = (unsigned char)sl.pos[i];
+  SparseLuPacked pk;
+  P3D_REQUIRE(pack_sparse_lu(ns, jcol, sl, pk), "internal: sparse LU schedule too large");
+  const std::vector<unsigned> &sdesc = pk.desc;
+  const std::vector<unsigned char> &sub = pk.ub, &szb = pk.zb, &spb = pk.pb;
   const size_t o_sdd = put(sdesc.data(), 4 * sdesc.size()), o_sub = put(sub.data(), sub.size()),
                o_szb = put(szb.data(), szb.size()), o_spb = put(spb.data(), spb.size());
   size_t o_sp = put(sl.pos.data(), 4 * sl.pos.size()), o_sv = put(sl.lev.data(), 4 * sl.lev.size()),
@@ -3225,6 +3245,31 @@ int p3d_sparse_lu_schedule(int32_t ns, int32_t nnz, const int32_t *jrow, const i
   std::copy(sl.uj.begin(), sl.uj.end(), uj);
   counts[0] = sl.nlev;
   counts[1] = (int32_t)sl.uj.size();
+  return P3D_OK;
+}
+
+int p3d_sparse_lu_packed(int32_t ns, int32_t nnz, const int32_t *jrow, const int32_t *jcol,
+                         int32_t cap, uint32_t *desc, uint8_t *ub, uint8_t *zb, uint8_t *pb,
+                         int32_t *sizes) {
+  P3D_REQUIRE(ns >= 1 && ns <= kMaxSpeciesWide, "n_species must be in [1, %d] (got %d)",
+              kMaxSpeciesWide, ns);
+  P3D_REQUIRE(nnz >= 0 && (nnz == 0 || (jrow && jcol)), "null structure");
+  P3D_REQUIRE(desc && ub && zb && pb && sizes, "null output");
+  std::vector<int> r(jrow, jrow + nnz), c(jcol, jcol + nnz);
+  for (int z = 0; z < nnz; ++z)
+    P3D_REQUIRE(r[z] >= 0 && r[z] < ns && c[z] >= 0 && c[z] < ns, "entry %d out of range", z);
+  SparseLuSched sl;
+  sparse_lu_schedule(ns, r, c, sl);
+  SparseLuPacked pk;
+  P3D_REQUIRE(pack_sparse_lu(ns, c, sl, pk), "sparse LU schedule too large to pack");
+  P3D_REQUIRE((int64_t)pk.ub.size() <= cap && (int64_t)pk.zb.size() <= cap,
+              "cap %d below the %zu / %zu packed bytes", cap, pk.ub.size(), pk.zb.size());
+  std::copy(pk.desc.begin(), pk.desc.end(), desc);
+  std::copy(pk.ub.begin(), pk.ub.end(), ub);
+  std::copy(pk.zb.begin(), pk.zb.end(), zb);
+  std::copy(pk.pb.begin(), pk.pb.end(), pb);
+  sizes[0] = (int32_t)pk.ub.size();
+  sizes[1] = (int32_t)pk.zb.size();
   return P3D_OK;
 }
 
